@@ -1,0 +1,273 @@
+/*
+ * sfmba.h - C ABI of the MI355X bundle-adjustment engine.
+ *
+ * This is the drop-in boundary for the `sfm-ba` crate of sfmtory.  Every
+ * type and entry point below mirrors one public item of that crate's Rust
+ * API (crates/sfm-ba/src/lib.rs, crates/sfm-ba/src/intrinsics.rs) or of the
+ * sfm-core types it takes (crates/sfm-core/src/camera.rs, pose.rs).  A Rust
+ * facade binds these with `extern "C"` and converts its Vec/nalgebra values
+ * into the plain arrays used here; the library never sees Rust or torch
+ * types.
+ *
+ * Conventions
+ *  - All arrays are caller-owned host memory.  Inputs are read, never kept.
+ *  - Entry points return 0 on success and a negative SFMBA_ERR_* code on
+ *    failure; sfmba_last_error() then describes it (thread-local string).
+ *    The Rust API is infallible for numerical trouble (a failed solve just
+ *    stops LM and returns the last accepted state), and so is this ABI:
+ *    a non-zero status only reports invalid arguments (an index out of
+ *    range, which the Rust code would panic on) or a HIP/device error.
+ *    There is no CPU fallback: without a usable gfx950 device the compute
+ *    entry points fail with SFMBA_ERR_DEVICE.
+ *  - Re-entrant: each call owns its stream and device workspace.
+ */
+#ifndef SFMBA_H
+#define SFMBA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+#define SFMBA_OK 0
+#define SFMBA_ERR_INVALID_ARGUMENT (-1)
+#define SFMBA_ERR_DEVICE (-2)
+#define SFMBA_ERR_OUT_OF_MEMORY (-3)
+#define SFMBA_ERR_UNSUPPORTED (-4)
+
+/* ------------------------------------------------------------ camera model
+ * COLMAP numeric model ids and parameter order.
+ * Replaces: enum CameraModel, CameraModel::model_id, CameraModel::params,
+ *           CameraModel::from_name_and_params
+ *           (crates/sfm-core/src/camera.rs:21-58, :74-83, :85-133, :136-163)
+ */
+#define SFMBA_SIMPLE_PINHOLE 0 /* f, cx, cy */
+#define SFMBA_PINHOLE 1        /* fx, fy, cx, cy */
+#define SFMBA_SIMPLE_RADIAL 2  /* f, cx, cy, k */
+#define SFMBA_RADIAL 3         /* f, cx, cy, k1, k2 */
+#define SFMBA_OPENCV 4         /* fx, fy, cx, cy, k1, k2, p1, p2 */
+#define SFMBA_OPENCV_FISHEYE 5 /* fx, fy, cx, cy, k1, k2, k3, k4 */
+
+typedef struct sfmba_camera {
+  int32_t model_id;  /* one of SFMBA_* model ids above */
+  int32_t reserved;  /* must be 0 */
+  double params[8];  /* COLMAP order; entries past the model's count ignored */
+} sfmba_camera;
+
+/* World-to-camera pose, X_cam = R X_world + t, R a Hamilton quaternion.
+ * Replaces: struct Pose (crates/sfm-core/src/pose.rs:15-20); the quaternion
+ * is stored w,x,y,z as in Pose::quaternion_wxyz (pose.rs:69-72). */
+typedef struct sfmba_pose {
+  double qw, qx, qy, qz;
+  double tx, ty, tz;
+} sfmba_pose;
+
+/* Replaces: struct Observation (crates/sfm-ba/src/lib.rs:56-62). */
+typedef struct sfmba_observation {
+  uint64_t image_idx;
+  uint64_t point_idx;
+  double x;
+  double y;
+} sfmba_observation;
+
+/* Replaces: struct BaInput (crates/sfm-ba/src/lib.rs:64-96).
+ * The three `fixed_*` vectors keep the Rust semantics: a vector shorter than
+ * its index range means "free" for the missing entries (lib.rs:766, :772),
+ * and a per-camera mask shorter than the model's parameter count fixes only
+ * the entries it has (lib.rs:1052-1057).  fixed_camera_params is the
+ * concatenation of the per-camera masks; fixed_camera_params_len[c] is the
+ * length of mask c, for c < num_fixed_camera_params. */
+typedef struct sfmba_input {
+  size_t num_images;       /* == poses.len() */
+  size_t num_cameras;      /* == cameras.len() */
+  size_t num_points;       /* == points.len() */
+  size_t num_observations; /* == observations.len() */
+  const uint64_t* camera_of_image; /* num_camera_of_image entries */
+  size_t num_camera_of_image;
+  const sfmba_camera* cameras;
+  const sfmba_pose* poses;
+  const double* points; /* x,y,z per point */
+  const sfmba_observation* observations;
+  const uint8_t* fixed_poses;
+  size_t num_fixed_poses;
+  const uint8_t* fixed_cameras;
+  size_t num_fixed_cameras;
+  const uint8_t* fixed_camera_params;
+  const size_t* fixed_camera_params_len;
+  size_t num_fixed_camera_params;
+} sfmba_input;
+
+/* Replaces: enum RobustLoss (crates/sfm-ba/src/lib.rs:213-218). */
+#define SFMBA_LOSS_NONE 0
+#define SFMBA_LOSS_HUBER 1
+#define SFMBA_LOSS_CAUCHY 2
+
+/* Replaces: struct BaParams + Default (crates/sfm-ba/src/lib.rs:220-238).
+ * Defaults: max_iterations 50, initial_lambda 1e-3, Huber, 2.0 px. */
+typedef struct sfmba_params {
+  uint64_t max_iterations;
+  double initial_lambda;
+  int32_t robust_loss;
+  int32_t reserved;
+  double loss_scale_px;
+} sfmba_params;
+
+/* Replaces: struct BaOutput (crates/sfm-ba/src/lib.rs:240-247).  The caller
+ * provides the three arrays, sized like the input's. */
+typedef struct sfmba_output {
+  sfmba_pose* poses;
+  double* points;
+  sfmba_camera* cameras;
+  double initial_cost;
+  double final_cost;
+  uint64_t iterations_run;
+} sfmba_output;
+
+/* Fills *p with BaParams::default() (lib.rs:229-238). */
+void sfmba_params_default(sfmba_params* p);
+
+/* --------------------------------------------------------- bundle adjust
+ * Replaces: pub fn bundle_adjust(input: BaInput, params: &BaParams) -> BaOutput
+ * (crates/sfm-ba/src/lib.rs:747-1232).  Schur-complement Levenberg-Marquardt
+ * over free poses, free camera intrinsics and all points, computed on the
+ * GPU in FP64 throughout (the reference's arithmetic).  The input is not
+ * modified; the refined state is written to *out. */
+int sfmba_bundle_adjust(const sfmba_input* input, const sfmba_params* params,
+                        sfmba_output* out);
+
+/* Engine options (no counterpart in the reference: the precision and solver
+ * choices of this engine). */
+#define SFMBA_PRECISION_FP64 0  /* everything in FP64 (parity mode) */
+#define SFMBA_PRECISION_MIXED 1 /* FP32 residual/Jacobian/Schur assembly,
+                                   FP64 point inverses, FP64 Cholesky and
+                                   FP64 iterative refinement of the step */
+#define SFMBA_SOLVER_CHOLESKY 0 /* dense FP64 MFMA Cholesky (LU fallback) */
+#define SFMBA_SOLVER_PCG 1      /* block-Jacobi PCG on the reduced system */
+
+typedef struct sfmba_options {
+  int32_t precision;  /* SFMBA_PRECISION_* */
+  int32_t solver;     /* SFMBA_SOLVER_* */
+  int32_t device;     /* HIP device ordinal */
+  int32_t profile;    /* 1: time each phase with HIP events into stats */
+  int32_t refine_steps; /* mixed precision: max FP64 refinement sweeps (default 3) */
+  int32_t pcg_max_iterations; /* PCG: iteration cap (default 500) */
+  double pcg_tolerance;       /* PCG: relative residual target (default 1e-10) */
+} sfmba_options;
+
+void sfmba_options_default(sfmba_options* o);
+
+/* Optional diagnostics filled by the _ex entry points.  Arrays are
+ * caller-provided; *_len reports how many entries were written (capped by
+ * *_capacity). */
+typedef struct sfmba_stats {
+  double* outer_costs;       /* accepted cost after each outer iteration */
+  uint64_t outer_costs_capacity;
+  uint64_t outer_costs_len;
+  double* trial_costs;       /* cost of every trial step, in order */
+  uint64_t trial_costs_capacity;
+  uint64_t trial_costs_len;
+  uint64_t dim;              /* size of the reduced camera system solved */
+  uint64_t num_linearizations;
+  uint64_t num_factorizations;
+  uint64_t num_cholesky_failures;
+  uint64_t num_lu_solves;
+  uint64_t num_rejected_steps;
+  uint64_t num_schur_pairs;  /* point-observation pairs reduced per try */
+  double time_total_s;       /* wall-clock of the call, device-resident part */
+  double time_upload_s;      /* host->device packing and copies */
+  double time_linearize_s;   /* profile=1: residual + Jacobian kernels */
+  double time_schur_s;       /* profile=1: point inverses + Schur assembly */
+  double time_factor_s;      /* profile=1: factorization */
+  double time_solve_s;       /* profile=1: triangular solves + refinement */
+  double time_update_s;      /* profile=1: back-substitution, update, cost */
+  uint64_t count_linearize;
+  uint64_t count_schur;
+  uint64_t count_factor;
+} sfmba_stats;
+
+/* bundle_adjust with engine options and diagnostics (stats may be NULL). */
+int sfmba_bundle_adjust_ex(const sfmba_input* input, const sfmba_params* params,
+                           const sfmba_options* options, sfmba_output* out,
+                           sfmba_stats* stats);
+
+/* Many independent problems (local-BA windows) solved by one persistent
+ * batched launch.  inputs/outputs/stats are arrays of `count` entries
+ * (stats may be NULL).  No counterpart in the reference, whose callers loop
+ * over bundle_adjust (crates/sfm-reconstruction/src/lib.rs:708-719); each
+ * problem gets exactly the bundle_adjust semantics. */
+int sfmba_bundle_adjust_batch(const sfmba_input* inputs, size_t count,
+                              const sfmba_params* params,
+                              const sfmba_options* options,
+                              sfmba_output* outputs, sfmba_stats* stats);
+
+/* ------------------------------------------------ mean reprojection error
+ * Replaces: pub fn mean_reprojection_error(input: &BaInput) -> f64
+ * (crates/sfm-ba/src/lib.rs:708-725): mean of the plain residual norm over
+ * observations in front of their camera, 0 if there are none. */
+int sfmba_mean_reprojection_error(const sfmba_input* input, double* out);
+
+/* Robust cost sum_i w(|r_i|) |r_i|^2 over in-front observations, the
+ * quantity bundle_adjust reports as initial/final cost.
+ * Replaces: fn total_cost (crates/sfm-ba/src/lib.rs:727-745). */
+int sfmba_total_cost(const sfmba_input* input, const sfmba_params* params,
+                     double* out);
+
+/* ------------------------------------------------ fixed-parameter mask
+ * Replaces: pub fn default_fixed_params_mask(&[CameraModel]) -> Vec<Vec<bool>>
+ * (crates/sfm-ba/src/intrinsics.rs:33-61).  Writes num_cameras masks of 8
+ * entries each (mask[c*8 + k]) and the model's parameter count to
+ * lens[c]; entries k >= lens[c] are 0. */
+int sfmba_default_fixed_params_mask(const sfmba_camera* cameras,
+                                    size_t num_cameras, uint8_t* mask,
+                                    size_t* lens);
+
+/* ----------------------------------------------- intrinsics-only refine
+ * Replaces: struct IntrinsicsRefineParams + Default and pub fn
+ * refine_intrinsics (crates/sfm-ba/src/intrinsics.rs:65-146, 148-238).
+ * Poses and points fixed; each camera with >= 8 observations gets a small
+ * dense LM over its own parameters with central-difference Jacobians.
+ * `cameras` is updated in place. */
+typedef struct sfmba_intrinsics_params {
+  uint64_t max_iterations; /* default 30 */
+  double initial_lambda;   /* default 1e-3 */
+  int32_t robust_loss;     /* default SFMBA_LOSS_HUBER */
+  int32_t reserved;
+  double loss_scale_px;    /* default 2.0 */
+} sfmba_intrinsics_params;
+
+void sfmba_intrinsics_params_default(sfmba_intrinsics_params* p);
+
+int sfmba_refine_intrinsics(sfmba_camera* cameras, size_t num_cameras,
+                            const uint64_t* camera_of_image, size_t num_images,
+                            const sfmba_pose* poses, const double* points,
+                            size_t num_points,
+                            const sfmba_observation* observations,
+                            size_t num_observations,
+                            const sfmba_intrinsics_params* params);
+
+/* ------------------------------------------ per-observation linearization
+ * The residual + analytic Jacobian kernel on its own, for checking it.
+ * Replaces: fn analytic_jacobians (crates/sfm-ba/src/lib.rs:327-590) with
+ * fn pose_point_jacobians (lib.rs:282-310).  For observation i writes
+ * r[2i..], jp[12i..] (2x6 row-major: d(u,v)/d(omega, t)), jx[6i..] (2x3),
+ * jc[16i..] (2x8 row-major, columns in CameraModel::params() order, unused
+ * columns 0) and valid[i] (0 when the point is at or behind z = 1e-9).
+ * precision selects the FP64 or the FP32 instantiation of the kernel. */
+int sfmba_observation_jacobians(const sfmba_input* input, int32_t precision,
+                                int32_t device, double* r, double* jp,
+                                double* jx, double* jc, uint8_t* valid);
+
+/* Number of usable HIP devices (0 without a GPU). */
+int sfmba_device_count(void);
+
+/* Description of the last error on this thread ("" if none). */
+const char* sfmba_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SFMBA_H */

@@ -1,0 +1,181 @@
+// Internal interfaces of the MI355X bundle-adjustment engine (not installed).
+//
+// Device data layout (one problem):
+//   observations are sorted point-major (CSR by point) and stored SoA:
+//     obs_img u32, obs_pt u32, obs_xy double2;
+//   an observation touches at most two "incidences": the block of the
+//   reduced system its pose belongs to and the block its camera belongs to
+//   (one merged block when the camera is used by this image alone);
+//   every incidence owns `gdim` consecutive "rows" (one per unknown of its
+//   block) in the row-indexed Jacobian stores:
+//     J4[row] = (J0, J1, w*J0, w*J1)   residual Jacobian column, plain and weighted
+//     E4[row] = (E0, E1, E2, g)        E = w J^T Jx (camera-point coupling), g = w J^T r
+//     F4[row] = (F0, F1, F2, 0)        F = E * Cinv(point), per damping try;
+//   the reduced camera system S is dense, column-major, lower triangle,
+//   (Dp + NB) x Dp with Dp = D rounded up to NB; padding is the identity and
+//   row Dp carries the right-hand side, so the blocked Cholesky produces
+//   L^{-1} rhs in that row.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/sfmba.h"
+
+namespace sfmba {
+
+constexpr uint32_t NONE = 0xFFFFFFFFu;
+constexpr int NB = 128;           // dense factorization block
+constexpr int PAIR_CHUNK = 128;   // max point pairs reduced by one wavefront
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIP_CHECK(expr)                                                                  \
+  do {                                                                                   \
+    hipError_t _e = (expr);                                                              \
+    if (_e != hipSuccess)                                                                \
+      throw ::sfmba::Error(_e == hipErrorOutOfMemory ? SFMBA_ERR_OUT_OF_MEMORY           \
+                                                     : SFMBA_ERR_DEVICE,                 \
+                           std::string(#expr) + ": " + hipGetErrorString(_e));           \
+  } while (0)
+
+inline int model_param_count_host(int model) {
+  switch (model) {
+    case SFMBA_SIMPLE_PINHOLE: return 3;
+    case SFMBA_PINHOLE: return 4;
+    case SFMBA_SIMPLE_RADIAL: return 4;
+    case SFMBA_RADIAL: return 5;
+    case SFMBA_OPENCV: return 8;
+    case SFMBA_OPENCV_FISHEYE: return 8;
+    default: return -1;
+  }
+}
+
+// Static structure of one problem on the device.
+struct Layout {
+  uint32_t n_img = 0, n_cam = 0, n_pt = 0, n_obs = 0, n_inc = 0, n_grp = 0;
+  uint32_t n_rows = 0;      // total incidence rows
+  uint32_t D = 0, Dp = 0, ld = 0;
+  uint64_t n_pairs = 0;
+  uint32_t n_chunks = 0, n_dests = 0, n_multi = 0, n_parts = 0;
+  // observations (point-major)
+  uint32_t* obs_img = nullptr;
+  uint32_t* obs_pt = nullptr;
+  double2* obs_xy = nullptr;
+  uint32_t* obs_inc = nullptr;  // 2 per obs, NONE if absent
+  uint32_t* pt_ptr = nullptr;   // n_pt + 1
+  // incidences
+  uint32_t* inc_grp = nullptr;
+  uint32_t* inc_pt = nullptr;
+  uint32_t* inc_obs = nullptr;
+  uint32_t* inc_roff = nullptr;  // first row
+  uint32_t* inc_row0 = nullptr;  // first entry of the observation's side vector
+  uint32_t* pt_inc_ptr = nullptr;  // incidences of each point (CSR, n_pt + 1)
+  // cameras and images
+  uint32_t* cam_of_img = nullptr;
+  int32_t* cam_model = nullptr;
+  uint32_t* img_slot = nullptr;   // first unknown of the pose, NONE if fixed
+  uint32_t* cam_slot = nullptr;   // first unknown of the camera's free params
+  uint8_t* cam_free = nullptr;    // 8 per camera: free param indices
+  uint32_t* cam_nfree = nullptr;
+  uint32_t* grp_off = nullptr;
+  uint32_t* grp_dim = nullptr;
+  // Schur pair lists (destination-sorted)
+  uint32_t* pairs = nullptr;      // 2 per pair (incidence a, incidence b)
+  uint4* chunks = nullptr;        // (dest, begin, end, part or NONE)
+  uint32_t* multi_dest = nullptr; // destinations reduced from partials
+  uint32_t* dest_part0 = nullptr; // first partial of each multi destination
+  uint32_t* dest_nparts = nullptr;
+  // host copies needed by the driver
+  std::vector<uint32_t> h_img_slot, h_cam_slot, h_cam_nfree, h_grp_off, h_grp_dim;
+  std::vector<uint8_t> h_cam_free;
+};
+
+// Per-problem mutable state and scratch.
+template <typename T>
+struct Work {
+  double* poses[2] = {nullptr, nullptr};   // 7 per image (w x y z tx ty tz)
+  double* cams[2] = {nullptr, nullptr};    // 8 per camera
+  double* pts[2] = {nullptr, nullptr};     // 3 per point
+  T* J4 = nullptr;
+  T* E4 = nullptr;
+  T* F4 = nullptr;
+  T* obs_pc = nullptr;        // 9 per obs: w Jx^T Jx (6), w Jx^T r (3)
+  uint8_t* obs_valid = nullptr;
+  double* pt_C = nullptr;     // 6 per point (xx xy xz yy yz zz)
+  double* pt_g = nullptr;     // 3 per point
+  uint8_t* pt_active = nullptr;
+  double* pt_cinv = nullptr;  // 9 per point
+  uint8_t* pt_ok = nullptr;
+  double* dpt = nullptr;      // 3 per point
+  double* S = nullptr;        // ld x Dp
+  double* S_copy = nullptr;   // refinement copy (mixed precision)
+  double* Linv = nullptr;     // Dp/NB blocks of NB x NB
+  double* delta = nullptr;    // Dp
+  double* vec = nullptr;      // Dp scratch
+  double* rhs = nullptr;      // Dp
+  double* parts = nullptr;    // partial blocks of multi-chunk destinations
+  double* cost_part = nullptr;
+  double* cost = nullptr;     // [0] cost, [1] sum |r|, [2] count
+  int* flags = nullptr;       // [0] cholesky failure, [1] lu failure
+  double* h_pinned = nullptr; // pinned host mirror of cost + flags
+};
+
+// ---- kernels (engine_kernels.hip) ----
+template <typename T>
+void launch_linearize(const Layout& L, Work<T>& W, int cur, int loss, double scale,
+                      hipStream_t s);
+template <typename T>
+void launch_point_accumulate(const Layout& L, Work<T>& W, hipStream_t s);
+template <typename T>
+void launch_point_prepare(const Layout& L, Work<T>& W, double lambda, hipStream_t s);
+template <typename T>
+void launch_schur(const Layout& L, Work<T>& W, double lambda, hipStream_t s);
+template <typename T>
+void launch_backsub_update(const Layout& L, Work<T>& W, int cur, hipStream_t s);
+void launch_cost(const Layout& L, const double* poses, const double* cams, const double* pts,
+                 int loss, double scale, double* part, double* out, int mode, hipStream_t s);
+void launch_jacobians_debug(const Layout& L, const double* poses, const double* cams,
+                            const double* pts, int fp32, double* r, double* jp, double* jx,
+                            double* jc, uint8_t* valid, hipStream_t s);
+
+// ---- intrinsics-only refinement (intrinsics_kernels.hip) ----
+void launch_refine_intrinsics(uint32_t n_cams_refined, const uint32_t* cam_list,
+                              const uint32_t* cam_obs_ptr, const uint32_t* obs_img,
+                              const uint32_t* obs_pt, const double2* obs_xy, const double* poses,
+                              const double* pts, const int32_t* cam_model, double* cams,
+                              const sfmba_intrinsics_params* p, hipStream_t s);
+
+// ---- dense reduced-system solver (dense_kernels.hip) ----
+// Cholesky of the lower triangle of S (ld x Dp) with the rhs row at Dp; on
+// return row Dp holds y = L^{-1} rhs; flags[0] != 0 on a non-positive pivot.
+void dense_cholesky(double* S, double* Linv, uint32_t Dp, uint32_t ld, uint32_t D, int* flags,
+                    hipStream_t s);
+// Solve L^T x = y; y (Dp) is consumed as scratch, x (Dp) receives the result.
+void dense_backsolve(const double* S, const double* Linv, uint32_t Dp, uint32_t ld, double* y,
+                     double* x, hipStream_t s);
+// Solve L y = b; b (Dp) is consumed as scratch, y (Dp) receives the result.
+void dense_forwardsolve(const double* S, const double* Linv, uint32_t Dp, uint32_t ld, double* b,
+                        double* y, hipStream_t s);
+// r = b - A x with A symmetric given by its lower triangle (ld x Dp).
+void dense_symv_residual(const double* A, uint32_t Dp, uint32_t ld, const double* x,
+                         const double* b, double* r, hipStream_t s);
+void dense_copy(const double* src, double* dst, size_t n, hipStream_t s);
+// dst (full symmetric, ld x Dp) <- lower triangle of src; src may equal dst.
+void dense_mirror_copy(const double* src, double* dst, uint32_t Dp, uint32_t ld, hipStream_t s);
+void dense_vec_add(double* x, const double* d, uint32_t n, hipStream_t s);
+void dense_extract_row(const double* S, uint32_t Dp, uint32_t ld, uint32_t row, double* out,
+                       hipStream_t s);
+// LU with partial pivoting of the full symmetric matrix given by its lower
+// triangle (mirrored first), then solve; rhs read from row Dp.
+void dense_lu_solve(double* S, uint32_t Dp, uint32_t ld, uint32_t D, double* x, int* piv,
+                    int* flags, hipStream_t s);
+
+}  // namespace sfmba

@@ -1,0 +1,909 @@
+// Host side of the engine: C ABI entry points, problem packing and the
+// Levenberg-Marquardt driver.  All arithmetic on the problem runs in the
+// kernels of engine_kernels.hip and dense_kernels.hip; the host only builds
+// index structures, launches, and reads back two scalars per damping try
+// (the trial cost and the factorization status), exactly the values the
+// reference's control flow branches on (crates/sfm-ba/src/lib.rs:1081-1221).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/sfmba.h"
+#include "camera_models.hpp"
+#include "engine.hpp"
+
+namespace sfmba {
+
+static thread_local std::string g_last_error;
+
+static void set_error(const std::string& m) { g_last_error = m; }
+
+// Device allocations owned by one call; freed in the destructor.
+struct DevArena {
+  std::vector<void*> ptrs;
+  ~DevArena() {
+    for (void* p : ptrs) (void)hipFree(p);
+  }
+  template <typename X>
+  X* alloc(size_t n) {
+    if (n == 0) n = 1;
+    void* p = nullptr;
+    HIP_CHECK(hipMalloc(&p, n * sizeof(X)));
+    ptrs.push_back(p);
+    return static_cast<X*>(p);
+  }
+  template <typename X>
+  X* upload(const std::vector<X>& v, hipStream_t s) {
+    X* d = alloc<X>(v.size());
+    if (!v.empty()) HIP_CHECK(hipMemcpyAsync(d, v.data(), v.size() * sizeof(X), hipMemcpyHostToDevice, s));
+    return d;
+  }
+};
+
+struct PinnedHost {
+  void* p = nullptr;
+  explicit PinnedHost(size_t bytes) { HIP_CHECK(hipHostMalloc(&p, bytes, hipHostMallocDefault)); }
+  ~PinnedHost() {
+    if (p) (void)hipHostFree(p);
+  }
+};
+
+struct StreamGuard {
+  hipStream_t s = nullptr;
+  explicit StreamGuard(int device) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
+      throw Error(SFMBA_ERR_DEVICE, "no HIP device available (this engine has no CPU path)");
+    if (device < 0 || device >= n) throw Error(SFMBA_ERR_INVALID_ARGUMENT, "device ordinal out of range");
+    HIP_CHECK(hipSetDevice(device));
+    HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  }
+  ~StreamGuard() {
+    if (s) (void)hipStreamDestroy(s);
+  }
+};
+
+static inline bool flag_at(const uint8_t* v, size_t n, size_t i) { return v && i < n && v[i] != 0; }
+
+static void validate(const sfmba_input* in) {
+  if (!in) throw Error(SFMBA_ERR_INVALID_ARGUMENT, "input is NULL");
+  auto need = [](bool ok, const char* m) {
+    if (!ok) throw Error(SFMBA_ERR_INVALID_ARGUMENT, m);
+  };
+  need(in->num_images == 0 || in->poses, "poses is NULL");
+  need(in->num_cameras == 0 || in->cameras, "cameras is NULL");
+  need(in->num_points == 0 || in->points, "points is NULL");
+  need(in->num_observations == 0 || in->observations, "observations is NULL");
+  need(in->num_camera_of_image >= in->num_images && (in->num_images == 0 || in->camera_of_image),
+       "camera_of_image shorter than poses");
+  need(in->num_observations < 0xFFFFFFF0ull && in->num_points < 0xFFFFFFF0ull &&
+           in->num_images < 0xFFFFFFF0ull && in->num_cameras < 0xFFFFFFF0ull,
+       "problem too large for 32-bit indices");
+  need(in->num_fixed_camera_params == 0 || in->fixed_camera_params_len,
+       "fixed_camera_params_len is NULL");
+  for (size_t i = 0; i < in->num_images; ++i)
+    if (in->camera_of_image[i] >= in->num_cameras)
+      throw Error(SFMBA_ERR_INVALID_ARGUMENT, "camera_of_image[" + std::to_string(i) + "] out of range");
+  for (size_t c = 0; c < in->num_cameras; ++c)
+    if (model_param_count_host(in->cameras[c].model_id) < 0)
+      throw Error(SFMBA_ERR_INVALID_ARGUMENT, "camera " + std::to_string(c) + " has an unknown model id");
+  for (size_t o = 0; o < in->num_observations; ++o) {
+    if (in->observations[o].image_idx >= in->num_images)
+      throw Error(SFMBA_ERR_INVALID_ARGUMENT, "observation " + std::to_string(o) + ": image_idx out of range");
+    if (in->observations[o].point_idx >= in->num_points)
+      throw Error(SFMBA_ERR_INVALID_ARGUMENT, "observation " + std::to_string(o) + ": point_idx out of range");
+  }
+}
+
+// Host-built structure of one problem (see engine.hpp for the layout).
+struct Pack {
+  uint32_t n_img = 0, n_cam = 0, n_pt = 0, n_obs = 0;
+  uint32_t D = 0, Dp = 0, ld = 0, n_grp = 0, n_rows = 0;
+  std::vector<uint32_t> obs_img, obs_pt, obs_inc, pt_ptr;
+  std::vector<double2> obs_xy;
+  std::vector<uint32_t> inc_grp, inc_pt, inc_obs, inc_roff, inc_row0, pt_inc_ptr;
+  std::vector<uint32_t> cam_of_img, img_slot, cam_slot, cam_nfree, grp_off, grp_dim;
+  std::vector<int32_t> cam_model;
+  std::vector<uint8_t> cam_free;
+  std::vector<uint32_t> pairs;
+  std::vector<uint4> chunks;
+  std::vector<uint32_t> multi_dest, dest_part0, dest_nparts;
+  uint32_t n_parts = 0;
+  std::vector<double> poses, cams, pts;  // packed FP64 state
+};
+
+static void build_pack(const sfmba_input* in, Pack& P, bool need_pairs) {
+  P.n_img = (uint32_t)in->num_images;
+  P.n_cam = (uint32_t)in->num_cameras;
+  P.n_pt = (uint32_t)in->num_points;
+  P.n_obs = (uint32_t)in->num_observations;
+  P.cam_of_img.resize(P.n_img);
+  for (uint32_t i = 0; i < P.n_img; ++i) P.cam_of_img[i] = (uint32_t)in->camera_of_image[i];
+  P.cam_model.resize(P.n_cam);
+  P.cams.assign(8 * (size_t)P.n_cam, 0.0);
+  std::vector<int> dof(P.n_cam);
+  for (uint32_t c = 0; c < P.n_cam; ++c) {
+    P.cam_model[c] = in->cameras[c].model_id;
+    dof[c] = model_param_count_host(in->cameras[c].model_id);
+    for (int k = 0; k < dof[c]; ++k) P.cams[8 * (size_t)c + k] = in->cameras[c].params[k];
+  }
+  P.poses.resize(7 * (size_t)P.n_img);
+  for (uint32_t i = 0; i < P.n_img; ++i) {
+    const sfmba_pose& q = in->poses[i];
+    const double v[7] = {q.qw, q.qx, q.qy, q.qz, q.tx, q.ty, q.tz};
+    std::memcpy(&P.poses[7 * (size_t)i], v, sizeof(v));
+  }
+  P.pts.assign(in->points, in->points + 3 * (size_t)P.n_pt);
+
+  // Free unknowns: free poses (6) and the unpinned parameters of free
+  // cameras (lib.rs:753-776, 1045-1070; pinned parameters are left out of
+  // the system, which is the exact decoupling the reference builds).
+  std::vector<uint8_t> pose_free(P.n_img), cam_is_free(P.n_cam);
+  for (uint32_t i = 0; i < P.n_img; ++i) pose_free[i] = !flag_at(in->fixed_poses, in->num_fixed_poses, i);
+  P.cam_free.assign(8 * (size_t)P.n_cam, 0);
+  P.cam_nfree.assign(P.n_cam, 0);
+  size_t moff = 0;
+  std::vector<size_t> mask_off(in->num_fixed_camera_params + 1, 0);
+  for (size_t c = 0; c < in->num_fixed_camera_params; ++c) {
+    mask_off[c] = moff;
+    moff += in->fixed_camera_params_len[c];
+  }
+  for (uint32_t c = 0; c < P.n_cam; ++c) {
+    cam_is_free[c] = dof[c] > 0 && !flag_at(in->fixed_cameras, in->num_fixed_cameras, c);
+    if (!cam_is_free[c]) continue;
+    uint32_t nf = 0;
+    for (int k = 0; k < dof[c]; ++k) {
+      bool pinned = false;
+      if (c < in->num_fixed_camera_params && (size_t)k < in->fixed_camera_params_len[c])
+        pinned = in->fixed_camera_params[mask_off[c] + k] != 0;
+      if (!pinned) P.cam_free[8 * (size_t)c + nf++] = (uint8_t)k;
+    }
+    P.cam_nfree[c] = nf;
+    if (nf == 0) cam_is_free[c] = 0;
+  }
+  std::vector<uint32_t> imgs_of_cam(P.n_cam, 0);
+  for (uint32_t i = 0; i < P.n_img; ++i) ++imgs_of_cam[P.cam_of_img[i]];
+
+  // Blocks of the reduced system: a free pose (merged with its camera when
+  // that camera is free and used by this image alone), then the remaining
+  // free cameras.
+  P.img_slot.assign(P.n_img, NONE);
+  P.cam_slot.assign(P.n_cam, NONE);
+  std::vector<uint32_t> grp_of_pose(P.n_img, NONE), grp_of_cam(P.n_cam, NONE);
+  uint32_t D = 0;
+  for (uint32_t i = 0; i < P.n_img; ++i) {
+    if (!pose_free[i]) continue;
+    const uint32_t g = (uint32_t)P.grp_off.size();
+    P.grp_off.push_back(D);
+    uint32_t dim = 6;
+    P.img_slot[i] = D;
+    grp_of_pose[i] = g;
+    const uint32_t c = P.cam_of_img[i];
+    if (cam_is_free[c] && imgs_of_cam[c] == 1) {
+      P.cam_slot[c] = D + 6;
+      grp_of_cam[c] = g;
+      dim += P.cam_nfree[c];
+    }
+    P.grp_dim.push_back(dim);
+    D += dim;
+  }
+  for (uint32_t c = 0; c < P.n_cam; ++c) {
+    if (!cam_is_free[c] || grp_of_cam[c] != NONE) continue;
+    const uint32_t g = (uint32_t)P.grp_off.size();
+    P.grp_off.push_back(D);
+    P.grp_dim.push_back(P.cam_nfree[c]);
+    P.cam_slot[c] = D;
+    grp_of_cam[c] = g;
+    D += P.cam_nfree[c];
+  }
+  P.n_grp = (uint32_t)P.grp_off.size();
+  P.D = D;
+  P.Dp = ((D + NB - 1) / NB) * NB;
+  P.ld = P.Dp + NB;
+
+  // Point-major observation order (stable counting sort).
+  P.pt_ptr.assign((size_t)P.n_pt + 1, 0);
+  for (uint32_t o = 0; o < P.n_obs; ++o) ++P.pt_ptr[in->observations[o].point_idx + 1];
+  for (uint32_t p = 0; p < P.n_pt; ++p) P.pt_ptr[p + 1] += P.pt_ptr[p];
+  std::vector<uint32_t> fill(P.pt_ptr.begin(), P.pt_ptr.end() - 1), order(P.n_obs);
+  for (uint32_t o = 0; o < P.n_obs; ++o) order[fill[in->observations[o].point_idx]++] = o;
+  P.obs_img.resize(P.n_obs);
+  P.obs_pt.resize(P.n_obs);
+  P.obs_xy.resize(P.n_obs);
+  P.obs_inc.assign(2 * (size_t)P.n_obs, NONE);
+  for (uint32_t s = 0; s < P.n_obs; ++s) {
+    const sfmba_observation& ob = in->observations[order[s]];
+    P.obs_img[s] = (uint32_t)ob.image_idx;
+    P.obs_pt[s] = (uint32_t)ob.point_idx;
+    P.obs_xy[s] = double2{ob.x, ob.y};
+  }
+
+  // Incidences, in observation (hence point) order.
+  P.pt_inc_ptr.assign((size_t)P.n_pt + 1, 0);
+  uint32_t rows = 0;
+  auto add_inc = [&](uint32_t s, uint32_t g, uint32_t row0, int slot) {
+    const uint32_t a = (uint32_t)P.inc_grp.size();
+    P.inc_grp.push_back(g);
+    P.inc_pt.push_back(P.obs_pt[s]);
+    P.inc_obs.push_back(s);
+    P.inc_roff.push_back(rows);
+    P.inc_row0.push_back(row0);
+    rows += P.grp_dim[g];
+    P.obs_inc[2 * (size_t)s + slot] = a;
+  };
+  for (uint32_t s = 0; s < P.n_obs; ++s) {
+    const uint32_t img = P.obs_img[s], c = P.cam_of_img[img];
+    const bool pf = pose_free[img], cf = cam_is_free[c];
+    if (pf && cf && grp_of_cam[c] == grp_of_pose[img]) {
+      add_inc(s, grp_of_pose[img], 0, 0);
+    } else {
+      int slot = 0;
+      if (pf) add_inc(s, grp_of_pose[img], 0, slot++);
+      if (cf) add_inc(s, grp_of_cam[c], pf ? 6u : 0u, slot++);
+    }
+    P.pt_inc_ptr[P.obs_pt[s] + 1] = (uint32_t)P.inc_grp.size();
+  }
+  for (uint32_t p = 0; p < P.n_pt; ++p) P.pt_inc_ptr[p + 1] = std::max(P.pt_inc_ptr[p + 1], P.pt_inc_ptr[p]);
+  P.n_rows = rows;
+  if (!need_pairs || D == 0) return;
+
+  // Destination-sorted point pairs.  Every lower-triangular block (ga >= gb)
+  // is a destination, so each try rewrites all of S.
+  const uint64_t G = P.n_grp;
+  const uint64_t n_dests = G * (G + 1) / 2;
+  if (n_dests > 400000000ull) throw Error(SFMBA_ERR_UNSUPPORTED, "reduced system has too many blocks");
+  std::vector<uint32_t> cnt(n_dests + 1, 0);
+  for (uint32_t p = 0; p < P.n_pt; ++p) {
+    const uint32_t a0 = P.pt_inc_ptr[p], a1 = P.pt_inc_ptr[p + 1];
+    for (uint32_t a = a0; a < a1; ++a)
+      for (uint32_t b = a0; b < a1; ++b) {
+        const uint64_t ga = P.inc_grp[a], gb = P.inc_grp[b];
+        if (ga < gb) continue;
+        ++cnt[ga * (ga + 1) / 2 + gb + 1];
+      }
+  }
+  for (uint64_t d = 0; d < n_dests; ++d) cnt[d + 1] += cnt[d];
+  const uint64_t n_pairs = cnt[n_dests];
+  if (n_pairs > 0x7FFFFFFFull) throw Error(SFMBA_ERR_UNSUPPORTED, "too many point pairs");
+  P.pairs.resize(2 * n_pairs);
+  {
+    std::vector<uint32_t> pos(cnt.begin(), cnt.end() - 1);
+    for (uint32_t p = 0; p < P.n_pt; ++p) {
+      const uint32_t a0 = P.pt_inc_ptr[p], a1 = P.pt_inc_ptr[p + 1];
+      for (uint32_t a = a0; a < a1; ++a)
+        for (uint32_t b = a0; b < a1; ++b) {
+          const uint64_t ga = P.inc_grp[a], gb = P.inc_grp[b];
+          if (ga < gb) continue;
+          const uint32_t q = pos[ga * (ga + 1) / 2 + gb]++;
+          P.pairs[2 * (size_t)q] = a;
+          P.pairs[2 * (size_t)q + 1] = b;
+        }
+    }
+  }
+  for (uint64_t d = 0; d < n_dests; ++d) {
+    const uint32_t b0 = cnt[d], b1 = cnt[d + 1];
+    const uint32_t nch = std::max<uint32_t>(1, (b1 - b0 + PAIR_CHUNK - 1) / PAIR_CHUNK);
+    if (nch == 1) {
+      P.chunks.push_back(uint4{(uint32_t)d, b0, b1, NONE});
+      continue;
+    }
+    P.multi_dest.push_back((uint32_t)d);
+    P.dest_part0.push_back(P.n_parts);
+    P.dest_nparts.push_back(nch);
+    for (uint32_t k = 0; k < nch; ++k) {
+      const uint32_t lo = b0 + k * PAIR_CHUNK, hi = std::min(b1, lo + PAIR_CHUNK);
+      P.chunks.push_back(uint4{(uint32_t)d, lo, hi, P.n_parts++});
+    }
+  }
+}
+
+struct Problem {
+  DevArena arena;
+  Layout L;
+};
+
+static void upload_layout(const Pack& P, Problem& pr, hipStream_t s) {
+  Layout& L = pr.L;
+  DevArena& A = pr.arena;
+  L.n_img = P.n_img;
+  L.n_cam = P.n_cam;
+  L.n_pt = P.n_pt;
+  L.n_obs = P.n_obs;
+  L.n_inc = (uint32_t)P.inc_grp.size();
+  L.n_grp = P.n_grp;
+  L.n_rows = P.n_rows;
+  L.D = P.D;
+  L.Dp = P.Dp;
+  L.ld = P.ld;
+  L.n_pairs = P.pairs.size() / 2;
+  L.n_chunks = (uint32_t)P.chunks.size();
+  L.n_multi = (uint32_t)P.multi_dest.size();
+  L.n_parts = P.n_parts;
+  L.obs_img = A.upload(P.obs_img, s);
+  L.obs_pt = A.upload(P.obs_pt, s);
+  L.obs_xy = A.upload(P.obs_xy, s);
+  L.obs_inc = A.upload(P.obs_inc, s);
+  L.pt_ptr = A.upload(P.pt_ptr, s);
+  L.inc_grp = A.upload(P.inc_grp, s);
+  L.inc_pt = A.upload(P.inc_pt, s);
+  L.inc_obs = A.upload(P.inc_obs, s);
+  L.inc_roff = A.upload(P.inc_roff, s);
+  L.inc_row0 = A.upload(P.inc_row0, s);
+  L.pt_inc_ptr = A.upload(P.pt_inc_ptr, s);
+  L.cam_of_img = A.upload(P.cam_of_img, s);
+  L.cam_model = A.upload(P.cam_model, s);
+  L.img_slot = A.upload(P.img_slot, s);
+  L.cam_slot = A.upload(P.cam_slot, s);
+  L.cam_free = A.upload(P.cam_free, s);
+  L.cam_nfree = A.upload(P.cam_nfree, s);
+  L.grp_off = A.upload(P.grp_off, s);
+  L.grp_dim = A.upload(P.grp_dim, s);
+  L.pairs = A.upload(P.pairs, s);
+  L.chunks = A.upload(P.chunks, s);
+  L.multi_dest = A.upload(P.multi_dest, s);
+  L.dest_part0 = A.upload(P.dest_part0, s);
+  L.dest_nparts = A.upload(P.dest_nparts, s);
+  L.h_img_slot = P.img_slot;
+  L.h_cam_slot = P.cam_slot;
+  L.h_cam_nfree = P.cam_nfree;
+  L.h_grp_off = P.grp_off;
+  L.h_grp_dim = P.grp_dim;
+  L.h_cam_free = P.cam_free;
+}
+
+// Phase timing with HIP events on the solve stream.
+struct Profiler {
+  bool on = false;
+  hipStream_t s = nullptr;
+  struct Rec {
+    int phase;
+    hipEvent_t a, b;
+  };
+  std::vector<Rec> recs;
+  int open_phase = -1;
+  hipEvent_t open_ev = nullptr;
+  ~Profiler() {
+    for (auto& r : recs) {
+      (void)hipEventDestroy(r.a);
+      (void)hipEventDestroy(r.b);
+    }
+    if (open_ev) (void)hipEventDestroy(open_ev);
+  }
+  void begin(int phase) {
+    if (!on) return;
+    HIP_CHECK(hipEventCreate(&open_ev));
+    HIP_CHECK(hipEventRecord(open_ev, s));
+    open_phase = phase;
+  }
+  void end() {
+    if (!on) return;
+    hipEvent_t e;
+    HIP_CHECK(hipEventCreate(&e));
+    HIP_CHECK(hipEventRecord(e, s));
+    recs.push_back(Rec{open_phase, open_ev, e});
+    open_ev = nullptr;
+  }
+  void collect(sfmba_stats* st) {
+    if (!on || !st) return;
+    HIP_CHECK(hipStreamSynchronize(s));
+    for (auto& r : recs) {
+      float ms = 0.f;
+      HIP_CHECK(hipEventElapsedTime(&ms, r.a, r.b));
+      const double sec = ms * 1e-3;
+      switch (r.phase) {
+        case 0: st->time_linearize_s += sec; st->count_linearize++; break;
+        case 1: st->time_schur_s += sec; st->count_schur++; break;
+        case 2: st->time_factor_s += sec; st->count_factor++; break;
+        case 3: st->time_solve_s += sec; break;
+        case 4: st->time_update_s += sec; break;
+      }
+    }
+  }
+};
+
+static void push_stat(double* arr, uint64_t cap, uint64_t* len, double v) {
+  if (arr && *len < cap) arr[*len] = v;
+  ++*len;
+}
+
+template <typename T>
+static void run_lm(const sfmba_input* in, const sfmba_params* prm, const sfmba_options* opt,
+                   sfmba_output* out, sfmba_stats* st) {
+  using clock = std::chrono::steady_clock;
+  const auto t0 = clock::now();
+  StreamGuard sg(opt->device);
+  hipStream_t s = sg.s;
+  Pack P;
+  build_pack(in, P, true);
+  Problem pr;
+  upload_layout(P, pr, s);
+  Layout& L = pr.L;
+  DevArena& A = pr.arena;
+  Work<T> W;
+  for (int k = 0; k < 2; ++k) {
+    W.poses[k] = A.alloc<double>(7 * (size_t)L.n_img);
+    W.cams[k] = A.alloc<double>(8 * (size_t)L.n_cam);
+    W.pts[k] = A.alloc<double>(3 * (size_t)L.n_pt);
+  }
+  HIP_CHECK(hipMemcpyAsync(W.poses[0], P.poses.data(), P.poses.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  HIP_CHECK(hipMemcpyAsync(W.cams[0], P.cams.data(), P.cams.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  HIP_CHECK(hipMemcpyAsync(W.pts[0], P.pts.data(), P.pts.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  const uint32_t n_cost_part = (L.n_obs + 2047) / 2048 + 1;
+  W.cost_part = A.alloc<double>(2 * (size_t)n_cost_part);
+  W.cost = A.alloc<double>(4);
+  W.flags = A.alloc<int>(4);
+  PinnedHost pin(64);
+  double* h = static_cast<double*>(pin.p);
+  const int loss = prm->robust_loss;
+  const double scale = prm->loss_scale_px;
+  const bool mixed = sizeof(T) == 4;
+  Profiler prof;
+  prof.on = opt->profile != 0;
+  prof.s = s;
+
+  auto read_cost = [&](int which) {
+    launch_cost(L, W.poses[which], W.cams[which], W.pts[which], loss, scale, W.cost_part, W.cost, 0, s);
+    HIP_CHECK(hipMemcpyAsync(h, W.cost, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    return std::pair<double, double>(h[0], h[1]);
+  };
+
+  if (st) {
+    st->dim = L.D;
+    st->num_schur_pairs = L.n_pairs;
+  }
+  int cur = 0;
+  double cost;
+  size_t iterations_run = 0;
+  double initial_cost;
+  if (L.D == 0) {
+    HIP_CHECK(hipStreamSynchronize(s));
+    const auto t1 = clock::now();
+    initial_cost = read_cost(0).first;
+    cost = initial_cost;
+    if (st) {
+      st->time_upload_s = std::chrono::duration<double>(t1 - t0).count();
+      st->time_total_s = std::chrono::duration<double>(clock::now() - t1).count();
+    }
+  } else {
+    W.J4 = A.alloc<T>(4 * (size_t)L.n_rows);
+    W.E4 = A.alloc<T>(4 * (size_t)L.n_rows);
+    W.F4 = A.alloc<T>(4 * (size_t)L.n_rows);
+    W.obs_pc = A.alloc<T>(9 * (size_t)L.n_obs);
+    W.obs_valid = A.alloc<uint8_t>(L.n_obs);
+    W.pt_C = A.alloc<double>(6 * (size_t)L.n_pt);
+    W.pt_g = A.alloc<double>(3 * (size_t)L.n_pt);
+    W.pt_active = A.alloc<uint8_t>(L.n_pt);
+    W.pt_cinv = A.alloc<double>(9 * (size_t)L.n_pt);
+    W.pt_ok = A.alloc<uint8_t>(L.n_pt);
+    const size_t s_elems = (size_t)L.ld * L.Dp;
+    W.S = A.alloc<double>(s_elems);
+    W.Linv = A.alloc<double>((size_t)L.Dp * NB);
+    W.delta = A.alloc<double>(L.Dp);
+    W.vec = A.alloc<double>(L.Dp);
+    W.rhs = A.alloc<double>(L.Dp);
+    W.parts = A.alloc<double>((size_t)std::max<uint32_t>(L.n_parts, 1) * (2 * 196 + 16));
+    double* vec2 = A.alloc<double>(L.Dp);
+    double* vec3 = A.alloc<double>(L.Dp);
+    int* piv = A.alloc<int>(L.Dp);
+    if (mixed && opt->refine_steps > 0) W.S_copy = A.alloc<double>(s_elems);
+    // S: zero everything once, identity on the padding diagonal.
+    HIP_CHECK(hipMemsetAsync(W.S, 0, s_elems * sizeof(double), s));
+    {
+      std::vector<double> one(1, 1.0);
+      for (uint32_t d = L.D; d < L.Dp; ++d)
+        HIP_CHECK(hipMemcpyAsync(W.S + (size_t)d * L.ld + d, one.data(), sizeof(double), hipMemcpyHostToDevice, s));
+      HIP_CHECK(hipStreamSynchronize(s));
+    }
+    const auto t1 = clock::now();
+    if (st) st->time_upload_s = std::chrono::duration<double>(t1 - t0).count();
+
+    auto c0 = read_cost(0);
+    initial_cost = c0.first;
+    double n_front = c0.second;
+    cost = initial_cost;
+    double lambda = prm->initial_lambda;
+    double nu = 2.0;
+
+    auto factor_and_solve = [&](double lam, bool& ok) {
+      prof.begin(1);
+      launch_point_prepare<T>(L, W, lam, s);
+      launch_schur<T>(L, W, lam, s);
+      prof.end();
+      if (W.S_copy) {
+        dense_mirror_copy(W.S, W.S_copy, L.Dp, L.ld, s);
+        dense_extract_row(W.S, L.Dp, L.ld, L.Dp, W.rhs, s);
+      }
+      prof.begin(2);
+      HIP_CHECK(hipMemsetAsync(W.flags, 0, 4 * sizeof(int), s));
+      dense_cholesky(W.S, W.Linv, L.Dp, L.ld, L.D, W.flags, s);
+      prof.end();
+      HIP_CHECK(hipMemcpyAsync(h + 4, W.flags, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
+      HIP_CHECK(hipStreamSynchronize(s));
+      const int* fl = reinterpret_cast<const int*>(h + 4);
+      if (fl[0] == 0) {
+        prof.begin(3);
+        dense_extract_row(W.S, L.Dp, L.ld, L.Dp, W.vec, s);
+        dense_backsolve(W.S, W.Linv, L.Dp, L.ld, W.vec, W.delta, s);
+        for (int it = 0; W.S_copy && it < opt->refine_steps; ++it) {
+          // FP64 iterative refinement of the step: r = rhs - S delta,
+          // delta += S^{-1} r with the same factor.
+          dense_symv_residual(W.S_copy, L.Dp, L.ld, W.delta, W.rhs, W.vec, s);
+          dense_forwardsolve(W.S, W.Linv, L.Dp, L.ld, W.vec, vec2, s);
+          dense_backsolve(W.S, W.Linv, L.Dp, L.ld, vec2, vec3, s);
+          dense_vec_add(W.delta, vec3, L.Dp, s);
+        }
+        prof.end();
+        ok = true;
+        return;
+      }
+      if (st) st->num_cholesky_failures++;
+      // Not positive definite: rebuild S and solve by LU (lib.rs:1083-1093).
+      launch_point_prepare<T>(L, W, lam, s);
+      launch_schur<T>(L, W, lam, s);
+      dense_lu_solve(W.S, L.Dp, L.ld, L.D, W.delta, piv, W.flags, s);
+      HIP_CHECK(hipMemcpyAsync(h + 4, W.flags, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
+      HIP_CHECK(hipStreamSynchronize(s));
+      if (st) st->num_lu_solves++;
+      ok = fl[1] == 0;
+    };
+
+    const double FUNCTION_TOLERANCE = 1e-6;
+    for (uint64_t outer = 0; outer < prm->max_iterations; ++outer) {
+      ++iterations_run;
+      const double cost_before = cost;
+      if (n_front == 0.0) break;  // no observation in front of its camera
+      prof.begin(0);
+      launch_linearize<T>(L, W, cur, loss, scale, s);
+      launch_point_accumulate<T>(L, W, s);
+      prof.end();
+      if (st) st->num_linearizations++;
+      bool accepted = false;
+      for (int inner = 0; inner < 8; ++inner) {
+        bool ok = false;
+        factor_and_solve(lambda, ok);
+        if (st) st->num_factorizations++;
+        if (!ok) {
+          lambda *= nu;
+          nu *= 2.0;
+          continue;
+        }
+        prof.begin(4);
+        launch_backsub_update<T>(L, W, cur, s);
+        launch_cost(L, W.poses[1 - cur], W.cams[1 - cur], W.pts[1 - cur], loss, scale, W.cost_part, W.cost, 0, s);
+        prof.end();
+        HIP_CHECK(hipMemcpyAsync(h, W.cost, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+        const double trial = h[0];
+        if (st) push_stat(st->trial_costs, st->trial_costs_capacity, &st->trial_costs_len, trial);
+        if (std::isfinite(trial) && trial < cost) {
+          cur = 1 - cur;
+          cost = trial;
+          n_front = h[1];
+          lambda = std::max(lambda / 3.0, 1e-12);
+          nu = 2.0;
+          accepted = true;
+          break;
+        }
+        if (st) st->num_rejected_steps++;
+        lambda *= nu;
+        nu *= 2.0;
+      }
+      if (st) push_stat(st->outer_costs, st->outer_costs_capacity, &st->outer_costs_len, cost);
+      if (!accepted) break;
+      const double rel = (cost_before - cost) / std::max(cost_before, 1e-12);
+      if (rel < FUNCTION_TOLERANCE) break;
+    }
+    HIP_CHECK(hipStreamSynchronize(s));
+    if (st) st->time_total_s = std::chrono::duration<double>(clock::now() - t1).count();
+    prof.collect(st);
+  }
+
+  // Results back to the caller's arrays.
+  std::vector<double> hp(7 * (size_t)L.n_img), hc(8 * (size_t)L.n_cam);
+  HIP_CHECK(hipMemcpyAsync(hp.data(), W.poses[cur], hp.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipMemcpyAsync(hc.data(), W.cams[cur], hc.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  if (L.n_pt)
+    HIP_CHECK(hipMemcpyAsync(out->points, W.pts[cur], 3 * (size_t)L.n_pt * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  for (uint32_t i = 0; i < L.n_img; ++i) {
+    sfmba_pose& q = out->poses[i];
+    const double* v = &hp[7 * (size_t)i];
+    q.qw = v[0]; q.qx = v[1]; q.qy = v[2]; q.qz = v[3];
+    q.tx = v[4]; q.ty = v[5]; q.tz = v[6];
+  }
+  for (uint32_t c = 0; c < L.n_cam; ++c) {
+    out->cameras[c] = in->cameras[c];
+    const int k = model_param_count_host(in->cameras[c].model_id);
+    for (int j = 0; j < k; ++j) out->cameras[c].params[j] = hc[8 * (size_t)c + j];
+  }
+  out->initial_cost = initial_cost;
+  out->final_cost = cost;
+  out->iterations_run = iterations_run;
+}
+
+// Layout without pair lists for the cost / Jacobian entry points.
+static void upload_state(const sfmba_input* in, Problem& pr, Pack& P, hipStream_t s, double** poses,
+                         double** cams, double** pts) {
+  build_pack(in, P, false);
+  upload_layout(P, pr, s);
+  *poses = pr.arena.upload(P.poses, s);
+  *cams = pr.arena.upload(P.cams, s);
+  *pts = pr.arena.upload(P.pts, s);
+}
+
+}  // namespace sfmba
+
+using namespace sfmba;
+
+template <typename F>
+static int guarded(F&& f) {
+  try {
+    g_last_error.clear();
+    f();
+    return SFMBA_OK;
+  } catch (const Error& e) {
+    set_error(e.what());
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    set_error("host out of memory");
+    return SFMBA_ERR_OUT_OF_MEMORY;
+  } catch (const std::exception& e) {
+    set_error(e.what());
+    return SFMBA_ERR_DEVICE;
+  }
+}
+
+extern "C" {
+
+void sfmba_params_default(sfmba_params* p) {
+  if (!p) return;
+  p->max_iterations = 50;
+  p->initial_lambda = 1e-3;
+  p->robust_loss = SFMBA_LOSS_HUBER;
+  p->reserved = 0;
+  p->loss_scale_px = 2.0;
+}
+
+void sfmba_options_default(sfmba_options* o) {
+  if (!o) return;
+  o->precision = SFMBA_PRECISION_FP64;
+  o->solver = SFMBA_SOLVER_CHOLESKY;
+  o->device = 0;
+  o->profile = 0;
+  o->refine_steps = 1;
+  o->pcg_max_iterations = 500;
+  o->pcg_tolerance = 1e-10;
+}
+
+void sfmba_intrinsics_params_default(sfmba_intrinsics_params* p) {
+  if (!p) return;
+  p->max_iterations = 30;
+  p->initial_lambda = 1e-3;
+  p->robust_loss = SFMBA_LOSS_HUBER;
+  p->reserved = 0;
+  p->loss_scale_px = 2.0;
+}
+
+const char* sfmba_last_error(void) { return g_last_error.c_str(); }
+
+int sfmba_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int sfmba_bundle_adjust_ex(const sfmba_input* input, const sfmba_params* params,
+                           const sfmba_options* options, sfmba_output* out, sfmba_stats* stats) {
+  return guarded([&] {
+    validate(input);
+    if (!params || !out) throw Error(SFMBA_ERR_INVALID_ARGUMENT, "params/output is NULL");
+    if ((input->num_images && !out->poses) || (input->num_points && !out->points) ||
+        (input->num_cameras && !out->cameras))
+      throw Error(SFMBA_ERR_INVALID_ARGUMENT, "output arrays are NULL");
+    sfmba_options o;
+    sfmba_options_default(&o);
+    if (options) o = *options;
+    if (stats) {
+      // keep the caller's array pointers and capacities, reset the rest
+      sfmba_stats z{};
+      z.outer_costs = stats->outer_costs;
+      z.outer_costs_capacity = stats->outer_costs_capacity;
+      z.trial_costs = stats->trial_costs;
+      z.trial_costs_capacity = stats->trial_costs_capacity;
+      *stats = z;
+    }
+    if (o.solver != SFMBA_SOLVER_CHOLESKY && o.solver != SFMBA_SOLVER_PCG)
+      throw Error(SFMBA_ERR_INVALID_ARGUMENT, "unknown solver");
+    if (o.solver == SFMBA_SOLVER_PCG) throw Error(SFMBA_ERR_UNSUPPORTED, "PCG solver not built");
+    if (o.precision == SFMBA_PRECISION_MIXED)
+      run_lm<float>(input, params, &o, out, stats);
+    else if (o.precision == SFMBA_PRECISION_FP64)
+      run_lm<double>(input, params, &o, out, stats);
+    else
+      throw Error(SFMBA_ERR_INVALID_ARGUMENT, "unknown precision");
+  });
+}
+
+int sfmba_bundle_adjust(const sfmba_input* input, const sfmba_params* params, sfmba_output* out) {
+  return sfmba_bundle_adjust_ex(input, params, nullptr, out, nullptr);
+}
+
+int sfmba_total_cost(const sfmba_input* input, const sfmba_params* params, double* out) {
+  return guarded([&] {
+    validate(input);
+    if (!params || !out) throw Error(SFMBA_ERR_INVALID_ARGUMENT, "params/out is NULL");
+    StreamGuard sg(0);
+    Problem pr;
+    Pack P;
+    double *po, *ca, *pt;
+    upload_state(input, pr, P, sg.s, &po, &ca, &pt);
+    double* part = pr.arena.alloc<double>(2 * ((size_t)pr.L.n_obs / 2048 + 2));
+    double* res = pr.arena.alloc<double>(2);
+    launch_cost(pr.L, po, ca, pt, params->robust_loss, params->loss_scale_px, part, res, 0, sg.s);
+    double h[2];
+    HIP_CHECK(hipMemcpyAsync(h, res, sizeof(h), hipMemcpyDeviceToHost, sg.s));
+    HIP_CHECK(hipStreamSynchronize(sg.s));
+    *out = h[0];
+  });
+}
+
+int sfmba_mean_reprojection_error(const sfmba_input* input, double* out) {
+  return guarded([&] {
+    validate(input);
+    if (!out) throw Error(SFMBA_ERR_INVALID_ARGUMENT, "out is NULL");
+    StreamGuard sg(0);
+    Problem pr;
+    Pack P;
+    double *po, *ca, *pt;
+    upload_state(input, pr, P, sg.s, &po, &ca, &pt);
+    double* part = pr.arena.alloc<double>(2 * ((size_t)pr.L.n_obs / 2048 + 2));
+    double* res = pr.arena.alloc<double>(2);
+    launch_cost(pr.L, po, ca, pt, SFMBA_LOSS_NONE, 1.0, part, res, 1, sg.s);
+    double h[2];
+    HIP_CHECK(hipMemcpyAsync(h, res, sizeof(h), hipMemcpyDeviceToHost, sg.s));
+    HIP_CHECK(hipStreamSynchronize(sg.s));
+    *out = h[1] == 0.0 ? 0.0 : h[0] / h[1];
+  });
+}
+
+int sfmba_observation_jacobians(const sfmba_input* input, int32_t precision, int32_t device,
+                                double* r, double* jp, double* jx, double* jc, uint8_t* valid) {
+  return guarded([&] {
+    validate(input);
+    if (!r || !jp || !jx || !jc || !valid) throw Error(SFMBA_ERR_INVALID_ARGUMENT, "output is NULL");
+    StreamGuard sg(device);
+    Problem pr;
+    Pack P;
+    // Keep the caller's observation order: pack without sorting by point.
+    sfmba_input one = *input;
+    double *po, *ca, *pt;
+    upload_state(&one, pr, P, sg.s, &po, &ca, &pt);
+    // Re-upload observations in input order.
+    const size_t n = input->num_observations;
+    std::vector<uint32_t> oi(n), op(n);
+    std::vector<double2> oxy(n);
+    for (size_t o = 0; o < n; ++o) {
+      oi[o] = (uint32_t)input->observations[o].image_idx;
+      op[o] = (uint32_t)input->observations[o].point_idx;
+      oxy[o] = double2{input->observations[o].x, input->observations[o].y};
+    }
+    Layout L = pr.L;
+    L.obs_img = pr.arena.upload(oi, sg.s);
+    L.obs_pt = pr.arena.upload(op, sg.s);
+    L.obs_xy = pr.arena.upload(oxy, sg.s);
+    double* dr = pr.arena.alloc<double>(2 * n);
+    double* djp = pr.arena.alloc<double>(12 * n);
+    double* djx = pr.arena.alloc<double>(6 * n);
+    double* djc = pr.arena.alloc<double>(16 * n);
+    uint8_t* dv = pr.arena.alloc<uint8_t>(n);
+    launch_jacobians_debug(L, po, ca, pt, precision == SFMBA_PRECISION_MIXED, dr, djp, djx, djc, dv, sg.s);
+    HIP_CHECK(hipMemcpyAsync(r, dr, 2 * n * sizeof(double), hipMemcpyDeviceToHost, sg.s));
+    HIP_CHECK(hipMemcpyAsync(jp, djp, 12 * n * sizeof(double), hipMemcpyDeviceToHost, sg.s));
+    HIP_CHECK(hipMemcpyAsync(jx, djx, 6 * n * sizeof(double), hipMemcpyDeviceToHost, sg.s));
+    HIP_CHECK(hipMemcpyAsync(jc, djc, 16 * n * sizeof(double), hipMemcpyDeviceToHost, sg.s));
+    HIP_CHECK(hipMemcpyAsync(valid, dv, n, hipMemcpyDeviceToHost, sg.s));
+    HIP_CHECK(hipStreamSynchronize(sg.s));
+  });
+}
+
+int sfmba_default_fixed_params_mask(const sfmba_camera* cameras, size_t num_cameras, uint8_t* mask,
+                                    size_t* lens) {
+  return guarded([&] {
+    if (num_cameras && (!cameras || !mask || !lens)) throw Error(SFMBA_ERR_INVALID_ARGUMENT, "NULL argument");
+    for (size_t c = 0; c < num_cameras; ++c) {
+      const int m = cameras[c].model_id;
+      const int k = model_param_count_host(m);
+      if (k < 0) throw Error(SFMBA_ERR_INVALID_ARGUMENT, "unknown model id");
+      lens[c] = (size_t)k;
+      std::memset(mask + 8 * c, 0, 8);
+      // principal-point indices (intrinsics.rs:33-42)
+      const bool f_first = m == SFMBA_SIMPLE_PINHOLE || m == SFMBA_SIMPLE_RADIAL || m == SFMBA_RADIAL;
+      mask[8 * c + (f_first ? 1 : 2)] = 1;
+      mask[8 * c + (f_first ? 2 : 3)] = 1;
+    }
+  });
+}
+
+int sfmba_refine_intrinsics(sfmba_camera* cameras, size_t num_cameras,
+                            const uint64_t* camera_of_image, size_t num_images,
+                            const sfmba_pose* poses, const double* points, size_t num_points,
+                            const sfmba_observation* observations, size_t num_observations,
+                            const sfmba_intrinsics_params* params) {
+  return guarded([&] {
+    if (!params) throw Error(SFMBA_ERR_INVALID_ARGUMENT, "params is NULL");
+    if ((num_cameras && !cameras) || (num_images && (!camera_of_image || !poses)) ||
+        (num_points && !points) || (num_observations && !observations))
+      throw Error(SFMBA_ERR_INVALID_ARGUMENT, "NULL argument");
+    for (size_t c = 0; c < num_cameras; ++c)
+      if (model_param_count_host(cameras[c].model_id) < 0)
+        throw Error(SFMBA_ERR_INVALID_ARGUMENT, "unknown model id");
+    // Group observations by camera, keeping their order (intrinsics.rs:125-128).
+    std::vector<std::vector<uint32_t>> by_cam(num_cameras);
+    for (size_t o = 0; o < num_observations; ++o) {
+      const sfmba_observation& ob = observations[o];
+      if (ob.image_idx >= num_images || ob.point_idx >= num_points)
+        throw Error(SFMBA_ERR_INVALID_ARGUMENT, "observation " + std::to_string(o) + " out of range");
+      const uint64_t c = camera_of_image[ob.image_idx];
+      if (c >= num_cameras) throw Error(SFMBA_ERR_INVALID_ARGUMENT, "camera_of_image out of range");
+      by_cam[c].push_back((uint32_t)o);
+    }
+    std::vector<uint32_t> cam_list, cam_ptr(1, 0), oi, op;
+    std::vector<double2> oxy;
+    for (size_t c = 0; c < num_cameras; ++c) {
+      if (by_cam[c].size() < 8) continue;  // left exactly as is (intrinsics.rs:135-140)
+      cam_list.push_back((uint32_t)c);
+      for (uint32_t o : by_cam[c]) {
+        oi.push_back((uint32_t)observations[o].image_idx);
+        op.push_back((uint32_t)observations[o].point_idx);
+        oxy.push_back(double2{observations[o].x, observations[o].y});
+      }
+      cam_ptr.push_back((uint32_t)oi.size());
+    }
+    if (cam_list.empty()) return;
+    StreamGuard sg(0);
+    DevArena A;
+    std::vector<double> hp(7 * num_images), hc(8 * num_cameras, 0.0), hx(points, points + 3 * num_points);
+    std::vector<int32_t> hm(num_cameras);
+    for (size_t i = 0; i < num_images; ++i) {
+      const sfmba_pose& q = poses[i];
+      const double v[7] = {q.qw, q.qx, q.qy, q.qz, q.tx, q.ty, q.tz};
+      std::memcpy(&hp[7 * i], v, sizeof(v));
+    }
+    for (size_t c = 0; c < num_cameras; ++c) {
+      hm[c] = cameras[c].model_id;
+      const int k = model_param_count_host(hm[c]);
+      for (int j = 0; j < k; ++j) hc[8 * c + j] = cameras[c].params[j];
+    }
+    double* dc = A.upload(hc, sg.s);
+    launch_refine_intrinsics((uint32_t)cam_list.size(), A.upload(cam_list, sg.s), A.upload(cam_ptr, sg.s),
+                             A.upload(oi, sg.s), A.upload(op, sg.s), A.upload(oxy, sg.s), A.upload(hp, sg.s),
+                             A.upload(hx, sg.s), A.upload(hm, sg.s), dc, params, sg.s);
+    HIP_CHECK(hipMemcpyAsync(hc.data(), dc, hc.size() * sizeof(double), hipMemcpyDeviceToHost, sg.s));
+    HIP_CHECK(hipStreamSynchronize(sg.s));
+    for (uint32_t c : cam_list) {
+      const int k = model_param_count_host(cameras[c].model_id);
+      for (int j = 0; j < k; ++j) cameras[c].params[j] = hc[8 * (size_t)c + j];
+    }
+  });
+}
+
+int sfmba_bundle_adjust_batch(const sfmba_input* inputs, size_t count, const sfmba_params* params,
+                              const sfmba_options* options, sfmba_output* outputs,
+                              sfmba_stats* stats) {
+  return guarded([&] {
+    if (count && (!inputs || !outputs)) throw Error(SFMBA_ERR_INVALID_ARGUMENT, "NULL argument");
+    for (size_t i = 0; i < count; ++i) {
+      const int rc = sfmba_bundle_adjust_ex(&inputs[i], params, options, &outputs[i], stats ? &stats[i] : nullptr);
+      if (rc != SFMBA_OK) throw Error(rc, "problem " + std::to_string(i) + ": " + g_last_error);
+    }
+  });
+}
+
+}  // extern "C"
