@@ -260,6 +260,17 @@ int sfmba_observation_jacobians(const sfmba_input* input, int32_t precision,
                                 int32_t device, double* r, double* jp,
                                 double* jx, double* jc, uint8_t* valid);
 
+/* ------------------------------------------------ dense reduced-system solve
+ * The factorization kernels on their own, for checking them.  Replaces the
+ * reference's nalgebra Cholesky::new(S).solve(rhs) with its LU fallback
+ * (crates/sfm-ba/src/lib.rs:1076-1094).  A is n x n column-major (only its
+ * lower triangle is read: the matrix is symmetric), b has n entries.
+ * method 0: Cholesky, *status = 1 when a pivot is not > 0;
+ * method 1: LU with partial pivoting of the full symmetric matrix,
+ * *status = 1 when U has a zero diagonal.  x receives the solution. */
+int sfmba_dense_solve(const double* A, size_t n, const double* b, int32_t method,
+                      int32_t device, double* x, int32_t* status);
+
 /* Number of usable HIP devices (0 without a GPU). */
 int sfmba_device_count(void);
 

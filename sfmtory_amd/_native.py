@@ -115,6 +115,8 @@ SIGNATURES = {
                                           C.POINTER(IntrinsicsParams)]),
     "sfmba_observation_jacobians": (C.c_int, [C.POINTER(Input), C.c_int32, C.c_int32, C.c_void_p,
                                               C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "sfmba_dense_solve": (C.c_int, [C.c_void_p, C.c_size_t, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p,
+                                    C.POINTER(C.c_int32)]),
     "sfmba_device_count": (C.c_int, []),
     "sfmba_last_error": (C.c_char_p, []),
 }

@@ -385,5 +385,19 @@ def observation_jacobians(inp: BaInput, precision: str = "fp64", device: int = 0
     return {"r": r, "jp": jp, "jx": jx, "jc": jc, "valid": valid.astype(bool)}
 
 
+def dense_solve(A: np.ndarray, b: np.ndarray, method: str = "cholesky", device: int = 0):
+    """Solve the symmetric system A x = b with the engine's dense kernels
+    (lower triangle of A read).  Returns (x, ok)."""
+    lib = N.load_engine()
+    A = np.asfortranarray(np.asarray(A, dtype=np.float64))
+    b = np.ascontiguousarray(np.asarray(b, dtype=np.float64))
+    n = A.shape[0]
+    x = np.zeros(n)
+    st = C.c_int32()
+    N.check(lib.sfmba_dense_solve(A.ctypes.data, n, b.ctypes.data, {"cholesky": 0, "lu": 1}[method], device,
+                                  x.ctypes.data, C.byref(st)))
+    return x, st.value == 0
+
+
 def device_count() -> int:
     return int(N.load_engine().sfmba_device_count())

@@ -30,7 +30,8 @@ constexpr int KC = 16;  // K chunk staged through LDS
 // acc(128x128 tile, 4 waves of 64x64) += sign * A(128 x K) * B(128 x K)^T,
 // A and B column-major (rows contiguous) with leading dims lda, ldb.
 // The v_mfma_f64_16x16x4_f64 operand layout: lane l supplies A[l%16][l/16]
-// and B^T[l/16][l%16]; result register v of lane l is D[4*(l/16)+v][l%16].
+// and B^T[l/16][l%16]; result register v of lane l is D[l/16 + 4v][l%16]
+// (checked on gfx950 with a one-wavefront probe of the builtin).
 __device__ inline void tile_mma(double4_t acc[4][4], const double* __restrict__ A, size_t lda,
                                 const double* __restrict__ B, size_t ldb, int K, double sign,
                                 double* As, double* Bs) {
@@ -73,9 +74,9 @@ __device__ inline void tile_load(double4_t acc[4][4], const double* C, size_t ld
 #pragma unroll
     for (int n = 0; n < 4; ++n) {
       const int col = wc * 64 + n * 16 + (lane & 15);
-      const int row = wr * 64 + m * 16 + 4 * (lane >> 4);
+      const int row = wr * 64 + m * 16 + (lane >> 4);
 #pragma unroll
-      for (int v = 0; v < 4; ++v) acc[m][n][v] = C[(size_t)col * ldc + row + v];
+      for (int v = 0; v < 4; ++v) acc[m][n][v] = C[(size_t)col * ldc + row + 4 * v];
     }
 }
 
@@ -87,10 +88,10 @@ __device__ inline void tile_store(const double4_t acc[4][4], double* C, size_t l
 #pragma unroll
     for (int n = 0; n < 4; ++n) {
       const int col = wc * 64 + n * 16 + (lane & 15);
-      const int row = wr * 64 + m * 16 + 4 * (lane >> 4);
+      const int row = wr * 64 + m * 16 + (lane >> 4);
 #pragma unroll
       for (int v = 0; v < 4; ++v)
-        if (!lower_only || row + v >= col) C[(size_t)col * ldc + row + v] = acc[m][n][v];
+        if (!lower_only || row + 4 * v >= col) C[(size_t)col * ldc + row + 4 * v] = acc[m][n][v];
     }
 }
 

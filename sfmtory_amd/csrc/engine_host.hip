@@ -813,6 +813,47 @@ int sfmba_observation_jacobians(const sfmba_input* input, int32_t precision, int
   });
 }
 
+int sfmba_dense_solve(const double* A, size_t n, const double* b, int32_t method, int32_t device,
+                      double* x, int32_t* status) {
+  return guarded([&] {
+    if (!A || !b || !x || !status) throw Error(SFMBA_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (method != 0 && method != 1) throw Error(SFMBA_ERR_INVALID_ARGUMENT, "unknown method");
+    if (n == 0) {
+      *status = 0;
+      return;
+    }
+    if (n > 60000) throw Error(SFMBA_ERR_UNSUPPORTED, "matrix too large");
+    StreamGuard sg(device);
+    DevArena ar;
+    const uint32_t D = (uint32_t)n, Dp = ((D + NB - 1) / NB) * NB, ld = Dp + NB;
+    std::vector<double> h((size_t)ld * Dp, 0.0);
+    for (uint32_t c = 0; c < D; ++c) {
+      for (uint32_t r = c; r < D; ++r) h[(size_t)c * ld + r] = A[(size_t)c * n + r];
+      h[(size_t)c * ld + Dp] = b[c];
+    }
+    for (uint32_t d = D; d < Dp; ++d) h[(size_t)d * ld + d] = 1.0;
+    double* S = ar.upload(h, sg.s);
+    double* Linv = ar.alloc<double>((size_t)Dp * NB);
+    double* y = ar.alloc<double>(Dp);
+    double* xd = ar.alloc<double>(Dp);
+    int* flags = ar.alloc<int>(4);
+    int* piv = ar.alloc<int>(Dp);
+    HIP_CHECK(hipMemsetAsync(flags, 0, 4 * sizeof(int), sg.s));
+    if (method == 0) {
+      dense_cholesky(S, Linv, Dp, ld, D, flags, sg.s);
+      dense_extract_row(S, Dp, ld, Dp, y, sg.s);
+      dense_backsolve(S, Linv, Dp, ld, y, xd, sg.s);
+    } else {
+      dense_lu_solve(S, Dp, ld, D, xd, piv, flags, sg.s);
+    }
+    int hf[4];
+    HIP_CHECK(hipMemcpyAsync(hf, flags, sizeof(hf), hipMemcpyDeviceToHost, sg.s));
+    HIP_CHECK(hipMemcpyAsync(x, xd, n * sizeof(double), hipMemcpyDeviceToHost, sg.s));
+    HIP_CHECK(hipStreamSynchronize(sg.s));
+    *status = method == 0 ? (hf[0] != 0) : (hf[1] != 0);
+  });
+}
+
 int sfmba_default_fixed_params_mask(const sfmba_camera* cameras, size_t num_cameras, uint8_t* mask,
                                     size_t* lens) {
   return guarded([&] {

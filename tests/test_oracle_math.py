@@ -1,0 +1,118 @@
+"""Pins the oracle's building blocks against textbook routines: rotations
+(scipy), the left-multiplicative pose update (lib.rs:263-269), projection
+formulas (camera.rs:204-244, restated here independently in numpy), robust
+weights (lib.rs:677-694), 3x3 inverse, Cholesky and LU (numpy/LAPACK)."""
+import numpy as np
+import pytest
+from scipy.spatial.transform import Rotation
+
+import oracle
+from sfmtory_amd.ba import RobustLoss
+
+rng = np.random.default_rng(7)
+
+
+def rand_pose():
+    q = Rotation.random(random_state=rng).as_quat()  # x y z w
+    return np.array([q[3], q[0], q[1], q[2], *rng.normal(size=3)])
+
+
+def test_transform_and_rotation_match_scipy():
+    for _ in range(20):
+        p = rand_pose()
+        R = Rotation.from_quat([p[1], p[2], p[3], p[0]]).as_matrix()
+        X = rng.normal(size=3)
+        assert np.allclose(oracle.rotation_matrix(p), R, atol=1e-14)
+        assert np.allclose(oracle.transform_point(p, X), R @ X + p[4:], atol=1e-13)
+
+
+def test_perturb_pose_left_multiplication():
+    for _ in range(20):
+        p = rand_pose()
+        d = rng.normal(size=6) * 0.3
+        out = oracle.perturb_pose(p, d)
+        R0 = Rotation.from_quat([p[1], p[2], p[3], p[0]])
+        want = (Rotation.from_rotvec(d[:3]) * R0).as_matrix()
+        assert np.allclose(oracle.rotation_matrix(out), want, atol=1e-13)
+        assert np.allclose(out[4:], p[4:] + d[3:], atol=0)  # additive translation, not SE(3)
+
+
+def test_perturb_pose_zero_is_exact_identity():
+    p = rand_pose()
+    assert np.array_equal(oracle.perturb_pose(p, np.zeros(6)), p)
+
+
+def np_project(model, P, pc):
+    x, y = pc[0] / pc[2], pc[1] / pc[2]
+    if model == 0:
+        return P[0] * x + P[1], P[0] * y + P[2]
+    if model == 1:
+        return P[0] * x + P[2], P[1] * y + P[3]
+    r2 = x * x + y * y
+    if model == 2:
+        d = 1 + P[3] * r2
+        return P[0] * x * d + P[1], P[0] * y * d + P[2]
+    if model == 3:
+        d = 1 + P[3] * r2 + P[4] * r2 ** 2
+        return P[0] * x * d + P[1], P[0] * y * d + P[2]
+    if model == 4:
+        k1, k2, p1, p2 = P[4:]
+        rad = 1 + k1 * r2 + k2 * r2 ** 2
+        return (P[0] * (x * rad + 2 * p1 * x * y + p2 * (r2 + 2 * x * x)) + P[2],
+                P[1] * (y * rad + p1 * (r2 + 2 * y * y) + 2 * p2 * x * y) + P[3])
+    r = max(np.sqrt(r2), 1e-12)
+    th = np.arctan(r)
+    t2 = th * th
+    td = th * (1 + P[4] * t2 + P[5] * t2 ** 2 + P[6] * t2 ** 3 + P[7] * t2 ** 4)
+    return P[0] * x * td / r + P[2], P[1] * y * td / r + P[3]
+
+
+@pytest.mark.parametrize("model,params", [
+    (0, [500, 320, 240]), (1, [500, 480, 320, 240]), (2, [500, 320, 240, -0.2]),
+    (3, [500, 320, 240, -0.2, 0.05]), (4, [500, 480, 320, 240, -0.2, 0.05, 0.01, -0.02]),
+    (5, [500, 480, 320, 240, -0.05, 0.02, -0.01, 0.005])])
+def test_projection_formulas(model, params):
+    for _ in range(10):
+        pc = np.array([*rng.normal(size=2) * 0.5, rng.uniform(1, 5)])
+        assert np.allclose(oracle.project(model, params, pc), np_project(model, params, pc), rtol=1e-14)
+
+
+def test_fisheye_radius_clamp_at_optical_axis():
+    # r clamped to >= 1e-12 (camera.rs:231): the principal point, not NaN
+    uv = oracle.project(5, [500, 480, 320, 240, 0.1, 0, 0, 0], np.array([0.0, 0.0, 2.0]))
+    assert np.allclose(uv, [320, 240])
+
+
+def test_robust_weights():
+    assert oracle.robust_weight(5.0, RobustLoss.NONE, 2.0) == 1.0
+    assert oracle.robust_weight(1.5, RobustLoss.HUBER, 2.0) == 1.0
+    assert oracle.robust_weight(2.0, RobustLoss.HUBER, 2.0) == 1.0
+    assert oracle.robust_weight(8.0, RobustLoss.HUBER, 2.0) == 0.25
+    assert np.isclose(oracle.robust_weight(4.0, RobustLoss.CAUCHY, 2.0), 1 / (1 + 4.0))
+    # scale floor 1e-9
+    assert oracle.robust_weight(1.0, RobustLoss.HUBER, 0.0) == 1e-9
+
+
+def test_inverse3():
+    for _ in range(20):
+        M = rng.normal(size=(3, 3))
+        assert np.allclose(oracle.inverse3(M), np.linalg.inv(M), rtol=1e-10, atol=1e-12)
+    assert oracle.inverse3(np.array([[1, 2, 3], [2, 4, 6], [0, 0, 1.0]])) is None
+
+
+def test_cholesky_and_lu_match_lapack():
+    for n in (1, 3, 17, 64):
+        M = rng.normal(size=(n, n))
+        A = M @ M.T + n * np.eye(n)
+        b = rng.normal(size=n)
+        want = np.linalg.solve(A, b)
+        assert np.allclose(oracle.cholesky_solve(A, b), want, rtol=1e-10)
+        assert np.allclose(oracle.lu_solve(A, b), want, rtol=1e-10)
+
+
+def test_cholesky_fails_on_indefinite_lu_does_not():
+    A = np.diag([1.0, -2.0, 3.0])
+    b = np.ones(3)
+    assert oracle.cholesky_solve(A, b) is None
+    assert np.allclose(oracle.lu_solve(A, b), [1, -0.5, 1 / 3])
+    assert oracle.lu_solve(np.zeros((2, 2)), np.ones(2)) is None
