@@ -177,7 +177,7 @@ typedef struct sfmba_stats {
   uint64_t num_rejected_steps;
   uint64_t num_schur_pairs;  /* point-observation pairs reduced per try */
   double time_total_s;       /* wall-clock of the call, device-resident part */
-  double time_upload_s;      /* host->device packing and copies */
+  double time_upload_s;      /* host-side packing and host->device copies (at create) */
   double time_linearize_s;   /* profile=1: residual + Jacobian kernels */
   double time_schur_s;       /* profile=1: point inverses + Schur assembly */
   double time_factor_s;      /* profile=1: factorization */
@@ -192,6 +192,20 @@ typedef struct sfmba_stats {
 int sfmba_bundle_adjust_ex(const sfmba_input* input, const sfmba_params* params,
                            const sfmba_options* options, sfmba_output* out,
                            sfmba_stats* stats);
+
+/* A problem made resident on a device once and solved many times.  No
+ * counterpart in the reference; for callers that re-solve one structure
+ * (the fixed- and free-intrinsics solves of one bundle,
+ * crates/sfm-reconstruction/src/lib.rs:1623, :1656) and for timing the
+ * device-resident solve.  Every solve starts from the input's initial
+ * state and has bundle_adjust's semantics; calls on one handle must not
+ * overlap.  sfmba_bundle_adjust_ex == create + solve + destroy. */
+typedef struct sfmba_problem sfmba_problem;
+
+int sfmba_problem_create(const sfmba_input* input, int32_t device, sfmba_problem** problem);
+int sfmba_problem_solve(sfmba_problem* problem, const sfmba_params* params,
+                        const sfmba_options* options, sfmba_output* out, sfmba_stats* stats);
+void sfmba_problem_destroy(sfmba_problem* problem);
 
 /* Many independent problems (local-BA windows) solved by one persistent
  * batched launch.  inputs/outputs/stats are arrays of `count` entries

@@ -52,6 +52,8 @@ def load() -> C.CDLL:
             "oracle_cholesky_solve": (C.c_int, [P, C.c_size_t, P]),
             "oracle_lu_solve": (C.c_int, [P, C.c_size_t, P]),
             "oracle_inverse3": (C.c_int, [P, P]),
+            "oracle_stage_times": (C.c_int, [P, P, C.c_int, P]),
+            "oracle_cholesky_seconds": (d, [C.c_size_t]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(lib, name)
@@ -235,3 +237,20 @@ def inverse3(m):
     o = np.zeros(9)
     ok = load().oracle_inverse3(a.ctypes.data, o.ctypes.data)
     return o.reshape(3, 3) if ok else None
+
+
+def stage_times(inp: BaInput, params: BaParams | None = None, nthreads: int = 1) -> dict:
+    """Seconds of the reference's stages for one outer iteration: initial
+    cost, Jacobians (parallel), accumulation (serial), reduced-system
+    assembly (Schur, parallel above 1500 points); plus D of the reference
+    layout."""
+    ci = _CInput(inp)
+    cp = _c_params(params or BaParams())
+    t = np.zeros(5)
+    if load().oracle_stage_times(C.addressof(ci.c), C.addressof(cp), int(nthreads), t.ctypes.data):
+        raise ValueError("oracle: invalid input")
+    return dict(cost=t[0], jacobians=t[1], accumulate=t[2], schur=t[3], dim=int(t[4]))
+
+
+def cholesky_seconds(n: int) -> float:
+    return float(load().oracle_cholesky_seconds(int(n)))

@@ -38,6 +38,7 @@
 #include <cstdint>
 #include <cstring>
 #include <limits>
+#include <chrono>
 #include <thread>
 #include <vector>
 
@@ -597,9 +598,22 @@ struct Trace {
   }
 };
 
+// Stage timing of one outer iteration (CPU baseline): when non-null,
+// bundle_adjust records wall-clock seconds of [0] the initial cost,
+// [1] hot loop A (Jacobians), [2] hot loop B (accumulation), [3] the first
+// reduced-system assembly (damped blocks, point inverses, Schur, pinning,
+// symmetrization), and returns right after that assembly.
+struct StageTimes {
+  double t[4];
+};
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
 // lib.rs:747-1232
 int bundle_adjust(const sfmba_input* in, const sfmba_params* prm, sfmba_output* out,
-                  Trace tr, int nthreads) {
+                  Trace tr, int nthreads, StageTimes* timing = nullptr) {
   const size_t num_images = in->num_images, num_points = in->num_points;
   const size_t num_cameras = in->num_cameras;
   std::vector<sfmba_pose> poses(in->poses, in->poses + num_images);
@@ -639,7 +653,12 @@ int bundle_adjust(const sfmba_input* in, const sfmba_params* prm, sfmba_output* 
     mask_off[c + 1] = mask_off[c] + in->fixed_camera_params_len[c];
 
   std::vector<double> S(D * D), rhs(D);
+  double tmark = now_s();
   const double initial_cost = total_cost(in, poses, cams, pts, prm, nthreads);
+  if (timing) {
+    timing->t[0] = now_s() - tmark;
+    tmark = now_s();
+  }
   double lambda = prm->initial_lambda;
   double cost = initial_cost;
   double nu = 2.0;
@@ -677,6 +696,10 @@ int bundle_adjust(const sfmba_input* in, const sfmba_params* prm, sfmba_output* 
     }
     size_t nprep = 0;
     for (size_t o = 0; o < nobs; ++o) nprep += prep_ok[o];
+    if (timing) {
+      timing->t[1] = now_s() - tmark;
+      tmark = now_s();
+    }
     if (nprep == 0) break;
 
     // Hot loop B (lib.rs:859-909)
@@ -742,6 +765,10 @@ int bundle_adjust(const sfmba_input* in, const sfmba_params* prm, sfmba_output* 
       }
     }
 
+    if (timing) {
+      timing->t[2] = now_s() - tmark;
+      tmark = now_s();
+    }
     bool accepted = false;
     std::vector<double> cp_inv(num_points * 9);
     std::vector<uint8_t> cp_ok(num_points);
@@ -844,6 +871,10 @@ int bundle_adjust(const sfmba_input* in, const sfmba_params* prm, sfmba_output* 
       for (size_t c = 0; c < D; ++c)
         for (size_t r = 0; r < D; ++r) Ssym[c * D + r] = 0.5 * (S[c * D + r] + S[r * D + c]);
 
+      if (timing) {
+        timing->t[3] = now_s() - tmark;
+        return finish(initial_cost, cost, iterations_run);
+      }
       std::vector<double> delta = rhs;
       if (!cholesky_solve(Ssym, D, delta)) {
         for (size_t c = 0; c < D; ++c)
@@ -1130,6 +1161,42 @@ int oracle_bundle_adjust(const sfmba_input* in, const sfmba_params* prm, sfmba_o
 }
 
 // intrinsics.rs:49-61
+// CPU-baseline stage timing on the reference's threading structure
+// (parallel Jacobians, serial accumulation, per-thread D x D Schur
+// accumulators when num_points >= 1500); times[0..3] as in StageTimes,
+// times[4] = D of the reference layout.  The state is not advanced.
+int oracle_stage_times(const sfmba_input* in, const sfmba_params* prm, int nthreads, double* times) {
+  if (!validate(in) || !prm || !times) return SFMBA_ERR_INVALID_ARGUMENT;
+  std::vector<sfmba_pose> p(in->num_images);
+  std::vector<sfmba_camera> c(in->num_cameras);
+  std::vector<double> x(3 * in->num_points);
+  sfmba_output out{p.data(), x.data(), c.data(), 0, 0, 0};
+  size_t a = 0, b = 0;
+  Trace tr{nullptr, 0, &a, nullptr, 0, &b};
+  StageTimes st{{0, 0, 0, 0}};
+  sfmba_params one = *prm;
+  one.max_iterations = 1;
+  bundle_adjust(in, &one, &out, tr, nthreads, &st);
+  for (int i = 0; i < 4; ++i) times[i] = st.t[i];
+  size_t D = 0;
+  for (size_t i = 0; i < in->num_images; ++i) D += flag_at(in->fixed_poses, in->num_fixed_poses, i) ? 0 : 6;
+  for (size_t k = 0; k < in->num_cameras; ++k)
+    D += flag_at(in->fixed_cameras, in->num_fixed_cameras, k) ? 0 : param_count(in->cameras[k].model_id);
+  times[4] = (double)D;
+  return 0;
+}
+
+// Seconds of one single-threaded Cholesky factorization + solve (the
+// reference's dense solve, lib.rs:1081-1082) of an n x n SPD matrix.
+double oracle_cholesky_seconds(size_t n) {
+  std::vector<double> A(n * n), b(n, 1.0);
+  for (size_t c = 0; c < n; ++c)
+    for (size_t r = 0; r < n; ++r) A[c * n + r] = (r == c ? (double)n : 0.0) + 1.0 / (1.0 + r + c);
+  const double t0 = now_s();
+  cholesky_solve(A, n, b);
+  return now_s() - t0;
+}
+
 int oracle_default_fixed_params_mask(const sfmba_camera* cams, size_t n, uint8_t* mask, size_t* lens) {
   for (size_t c = 0; c < n; ++c) {
     const int k = param_count(cams[c].model_id);

@@ -12,6 +12,7 @@ from .ba import (  # noqa: F401
     IntrinsicsRefineParams,
     Observation,
     Pose,
+    Problem,
     RobustLoss,
     bundle_adjust,
     bundle_adjust_batch,

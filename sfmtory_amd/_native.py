@@ -106,6 +106,10 @@ SIGNATURES = {
                                          C.POINTER(Output), C.POINTER(Stats)]),
     "sfmba_bundle_adjust_batch": (C.c_int, [C.POINTER(Input), C.c_size_t, C.POINTER(Params),
                                             C.POINTER(Options), C.POINTER(Output), C.POINTER(Stats)]),
+    "sfmba_problem_create": (C.c_int, [C.POINTER(Input), C.c_int32, C.POINTER(C.c_void_p)]),
+    "sfmba_problem_solve": (C.c_int, [C.c_void_p, C.POINTER(Params), C.POINTER(Options), C.POINTER(Output),
+                                      C.POINTER(Stats)]),
+    "sfmba_problem_destroy": (None, [C.c_void_p]),
     "sfmba_mean_reprojection_error": (C.c_int, [C.POINTER(Input), C.POINTER(C.c_double)]),
     "sfmba_total_cost": (C.c_int, [C.POINTER(Input), C.POINTER(Params), C.POINTER(C.c_double)]),
     "sfmba_default_fixed_params_mask": (C.c_int, [C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p]),

@@ -298,6 +298,48 @@ def bundle_adjust(inp: BaInput, params: BaParams | None = None, *, precision: st
                     int(outs.c.iterations_run), st.to_dict())
 
 
+class Problem:
+    """A problem made resident on the device once and solved many times
+    (``sfmba_problem_create`` / ``_solve``).  Each solve starts from the
+    input's initial state with bundle_adjust's semantics."""
+
+    def __init__(self, inp: BaInput, device: int = 0):
+        self._lib = N.load_engine()
+        self._ci = _CInput(inp)
+        h = C.c_void_p()
+        N.check(self._lib.sfmba_problem_create(C.byref(self._ci.c), int(device), C.byref(h)))
+        self._h = h
+        self.device = device
+
+    def solve(self, params: BaParams | None = None, *, precision: str = "fp64", solver: str = "cholesky",
+              profile: bool = False, refine_steps: int | None = None) -> BaOutput:
+        params = params or BaParams()
+        outs = _Outputs(self._ci)
+        st = _Stats()
+        opt = _options(precision, solver, self.device, profile, refine_steps)
+        N.check(self._lib.sfmba_problem_solve(self._h, C.byref(_c_params(params)), C.byref(opt), C.byref(outs.c),
+                                              C.byref(st.c)))
+        return BaOutput(outs.poses, outs.points, outs.cams, outs.c.initial_cost, outs.c.final_cost,
+                        int(outs.c.iterations_run), st.to_dict())
+
+    def close(self) -> None:
+        if self._h:
+            self._lib.sfmba_problem_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
 def bundle_adjust_batch(inputs: Sequence[BaInput], params: BaParams | None = None, *,
                         precision: str = "fp64", device: int = 0) -> list:
     """Many independent problems (local-BA windows) in one batched call."""

@@ -80,7 +80,13 @@ __device__ inline void tile_load(double4_t acc[4][4], const double* C, size_t ld
     }
 }
 
-__device__ inline void tile_store(const double4_t acc[4][4], double* C, size_t ldc, bool lower_only) {
+// Stores only entries that carry data: global column < D, and global row
+// < D (the system) or == Dp (the right-hand side).  Padding and the spare
+// rows of the rhs block keep their initial values, so a failed (NaN)
+// factorization cannot leak into storage the next assembly does not rewrite
+// (NaN * 0 is NaN).
+__device__ inline void tile_store(const double4_t acc[4][4], double* C, size_t ldc, bool lower_only,
+                                  uint32_t row0, uint32_t col0, uint32_t D, uint32_t Dp) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wr = wave >> 1, wc = wave & 1;
 #pragma unroll
@@ -90,8 +96,11 @@ __device__ inline void tile_store(const double4_t acc[4][4], double* C, size_t l
       const int col = wc * 64 + n * 16 + (lane & 15);
       const int row = wr * 64 + m * 16 + (lane >> 4);
 #pragma unroll
-      for (int v = 0; v < 4; ++v)
-        if (!lower_only || row + 4 * v >= col) C[(size_t)col * ldc + row + 4 * v] = acc[m][n][v];
+      for (int v = 0; v < 4; ++v) {
+        const uint32_t g = row0 + row + 4 * v;
+        if ((!lower_only || row + 4 * v >= col) && (g < D || g == Dp) && col0 + col < D)
+          C[(size_t)col * ldc + row + 4 * v] = acc[m][n][v];
+      }
     }
 }
 
@@ -141,7 +150,7 @@ __global__ __launch_bounds__(256) void k_potrf_block(double* __restrict__ S, siz
   }
   for (int idx = t; idx < NB * NB; idx += 256) {
     const int c = idx >> 7, r = idx & 127;
-    if (r >= c) A[(size_t)c * ld + r] = Lp[pk(r, c)];
+    if (r >= c && (uint32_t)(k * NB + r) < D) A[(size_t)c * ld + r] = Lp[pk(r, c)];
   }
   // Inverse: X = L^{-1}, row by row; column c of row i handled by thread
   // pair (2c, 2c+1) splitting the inner sum.
@@ -161,13 +170,15 @@ __global__ __launch_bounds__(256) void k_potrf_block(double* __restrict__ S, siz
   double* X = Linv + (size_t)k * NB * NB;
   for (int idx = t; idx < NB * NB; idx += 256) {
     const int c = idx >> 7, r = idx & 127;
-    X[idx] = r >= c ? Xp[pk(r, c)] : 0.0;
+    const bool pad = (uint32_t)(k * NB + r) >= D || (uint32_t)(k * NB + c) >= D;
+    X[idx] = pad ? (r == c ? 1.0 : 0.0) : (r >= c ? Xp[pk(r, c)] : 0.0);
   }
 }
 
 // L_ik = A_ik Linv^T for row blocks i = k+1 .. nrb-1 (nrb includes the rhs block).
 __global__ __launch_bounds__(256) void k_trsm_panel(double* __restrict__ S, size_t ld, int k,
-                                                    const double* __restrict__ Linv) {
+                                                    const double* __restrict__ Linv, uint32_t D,
+                                                    uint32_t Dp) {
   __shared__ double As[KC * 128];
   __shared__ double Bs[KC * 128];
   const int i = k + 1 + blockIdx.x;
@@ -180,12 +191,12 @@ __global__ __launch_bounds__(256) void k_trsm_panel(double* __restrict__ S, size
     for (int n = 0; n < 4; ++n) acc[m][n] = double4_t{0.0, 0.0, 0.0, 0.0};
   tile_mma(acc, A, ld, X, NB, NB, 1.0, As, Bs);
   __syncthreads();
-  tile_store(acc, A, ld, false);
+  tile_store(acc, A, ld, false, (uint32_t)i * NB, (uint32_t)k * NB, D, Dp);
 }
 
 // A_ij -= L_ik L_jk^T for k < j <= i, j < nblk, i <= nblk (i == nblk: rhs block).
 __global__ __launch_bounds__(256) void k_syrk_update(double* __restrict__ S, size_t ld, int k,
-                                                     int nblk) {
+                                                     int nblk, uint32_t D, uint32_t Dp) {
   __shared__ double As[KC * 128];
   __shared__ double Bs[KC * 128];
   const int j = k + 1 + blockIdx.x;
@@ -197,7 +208,7 @@ __global__ __launch_bounds__(256) void k_syrk_update(double* __restrict__ S, siz
   double4_t acc[4][4];
   tile_load(acc, C, ld);
   tile_mma(acc, Ai, ld, Aj, ld, NB, -1.0, As, Bs);
-  tile_store(acc, C, ld, i == j);
+  tile_store(acc, C, ld, i == j, (uint32_t)i * NB, (uint32_t)j * NB, D, Dp);
 }
 
 // ------------------------------------------------------ triangular solves
@@ -398,11 +409,11 @@ void dense_cholesky(double* S, double* Linv, uint32_t Dp, uint32_t ld, uint32_t 
   for (int k = 0; k < nblk; ++k) {
     hipLaunchKernelGGL(k_potrf_block, dim3(1), dim3(256), 0, s, S, (size_t)ld, k, D, Linv, flags);
     const int nbelow = nblk - k;  // row blocks k+1 .. nblk (rhs block included)
-    hipLaunchKernelGGL(k_trsm_panel, dim3(nbelow), dim3(256), 0, s, S, (size_t)ld, k, Linv);
+    hipLaunchKernelGGL(k_trsm_panel, dim3(nbelow), dim3(256), 0, s, S, (size_t)ld, k, Linv, D, Dp);
     const int ntrail = nblk - 1 - k;
     if (ntrail > 0)
       hipLaunchKernelGGL(k_syrk_update, dim3(ntrail, ntrail + 1), dim3(256), 0, s, S, (size_t)ld,
-                         k, nblk);
+                         k, nblk, D, Dp);
   }
   HIP_CHECK(hipGetLastError());
 }

@@ -412,111 +412,159 @@ static void push_stat(double* arr, uint64_t cap, uint64_t* len, double v) {
   ++*len;
 }
 
-template <typename T>
-static void run_lm(const sfmba_input* in, const sfmba_params* prm, const sfmba_options* opt,
-                   sfmba_output* out, sfmba_stats* st) {
+}  // namespace sfmba
+
+// A problem resident on one device: packed structure, initial state and the
+// per-precision work buffers, reused by every solve of the problem.
+struct sfmba_problem {
+  int device = 0;
+  hipStream_t s = nullptr;
+  sfmba::Problem pr;
+  double* init_poses = nullptr;
+  double* init_cams = nullptr;
+  double* init_pts = nullptr;
+  std::vector<sfmba_camera> cams_in;
+  sfmba::Work<float> w32;
+  sfmba::Work<double> w64;
+  bool have32 = false, have64 = false;
+  double* vec2 = nullptr;
+  double* vec3 = nullptr;
+  int* piv = nullptr;
+  double* pinned = nullptr;
+  double pack_s = 0.0, upload_s = 0.0;
+  ~sfmba_problem() {
+    // device buffers (pr.arena) are freed after this body, by member destruction
+    if (s) (void)hipStreamSynchronize(s);
+    if (pinned) (void)hipHostFree(pinned);
+    if (s) (void)hipStreamDestroy(s);
+  }
+};
+
+namespace sfmba {
+
+static sfmba_problem* create_problem(const sfmba_input* in, int device) {
   using clock = std::chrono::steady_clock;
+  validate(in);
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
+    throw Error(SFMBA_ERR_DEVICE, "no HIP device available (this engine has no CPU path)");
+  if (device < 0 || device >= n) throw Error(SFMBA_ERR_INVALID_ARGUMENT, "device ordinal out of range");
+  HIP_CHECK(hipSetDevice(device));
+  std::unique_ptr<sfmba_problem> H(new sfmba_problem());
+  H->device = device;
+  HIP_CHECK(hipStreamCreateWithFlags(&H->s, hipStreamNonBlocking));
+  HIP_CHECK(hipHostMalloc((void**)&H->pinned, 64, hipHostMallocDefault));
   const auto t0 = clock::now();
-  StreamGuard sg(opt->device);
-  hipStream_t s = sg.s;
   Pack P;
   build_pack(in, P, true);
-  Problem pr;
-  upload_layout(P, pr, s);
-  Layout& L = pr.L;
-  DevArena& A = pr.arena;
-  Work<T> W;
+  const auto t1 = clock::now();
+  upload_layout(P, H->pr, H->s);
+  H->init_poses = H->pr.arena.upload(P.poses, H->s);
+  H->init_cams = H->pr.arena.upload(P.cams, H->s);
+  H->init_pts = H->pr.arena.upload(P.pts, H->s);
+  const Layout& L = H->pr.L;
+  H->vec2 = H->pr.arena.alloc<double>(std::max<uint32_t>(L.Dp, 1));
+  H->vec3 = H->pr.arena.alloc<double>(std::max<uint32_t>(L.Dp, 1));
+  H->piv = H->pr.arena.alloc<int>(std::max<uint32_t>(L.Dp, 1));
+  H->cams_in.assign(in->cameras, in->cameras + in->num_cameras);
+  HIP_CHECK(hipStreamSynchronize(H->s));
+  H->pack_s = std::chrono::duration<double>(t1 - t0).count();
+  H->upload_s = std::chrono::duration<double>(clock::now() - t1).count();
+  return H.release();
+}
+
+template <typename T>
+static void alloc_work(sfmba_problem& H, Work<T>& W) {
+  const Layout& L = H.pr.L;
+  DevArena& A = H.pr.arena;
+  hipStream_t s = H.s;
   for (int k = 0; k < 2; ++k) {
     W.poses[k] = A.alloc<double>(7 * (size_t)L.n_img);
     W.cams[k] = A.alloc<double>(8 * (size_t)L.n_cam);
     W.pts[k] = A.alloc<double>(3 * (size_t)L.n_pt);
   }
-  HIP_CHECK(hipMemcpyAsync(W.poses[0], P.poses.data(), P.poses.size() * sizeof(double), hipMemcpyHostToDevice, s));
-  HIP_CHECK(hipMemcpyAsync(W.cams[0], P.cams.data(), P.cams.size() * sizeof(double), hipMemcpyHostToDevice, s));
-  HIP_CHECK(hipMemcpyAsync(W.pts[0], P.pts.data(), P.pts.size() * sizeof(double), hipMemcpyHostToDevice, s));
   const uint32_t n_cost_part = (L.n_obs + 2047) / 2048 + 1;
   W.cost_part = A.alloc<double>(2 * (size_t)n_cost_part);
-  W.cost = A.alloc<double>(4);
+  W.cost = A.alloc<double>(8);
   W.flags = A.alloc<int>(4);
-  PinnedHost pin(64);
-  double* h = static_cast<double*>(pin.p);
+  if (L.D == 0) return;
+  W.J4 = A.alloc<T>(4 * (size_t)L.n_rows);
+  W.E4 = A.alloc<T>(4 * (size_t)L.n_rows);
+  W.F4 = A.alloc<T>(4 * (size_t)L.n_rows);
+  W.obs_pc = A.alloc<T>(9 * (size_t)L.n_obs);
+  W.obs_valid = A.alloc<uint8_t>(L.n_obs);
+  W.pt_C = A.alloc<double>(6 * (size_t)L.n_pt);
+  W.pt_g = A.alloc<double>(3 * (size_t)L.n_pt);
+  W.pt_active = A.alloc<uint8_t>(L.n_pt);
+  W.pt_cinv = A.alloc<double>(9 * (size_t)L.n_pt);
+  W.pt_ok = A.alloc<uint8_t>(L.n_pt);
+  const size_t s_elems = (size_t)L.ld * L.Dp;
+  W.S = A.alloc<double>(s_elems);
+  W.Linv = A.alloc<double>((size_t)L.Dp * NB);
+  W.delta = A.alloc<double>(L.Dp);
+  W.vec = A.alloc<double>(L.Dp);
+  W.rhs = A.alloc<double>(L.Dp);
+  W.parts = A.alloc<double>((size_t)std::max<uint32_t>(L.n_parts, 1) * (2 * 196 + 16));
+  // S: zero once, identity on the padding diagonal; the kernels rewrite the
+  // D x D lower triangle and the rhs row on every try and never touch the
+  // padding.
+  HIP_CHECK(hipMemsetAsync(W.S, 0, s_elems * sizeof(double), s));
+  std::vector<double> one(1, 1.0);
+  for (uint32_t d = L.D; d < L.Dp; ++d)
+    HIP_CHECK(hipMemcpyAsync(W.S + (size_t)d * L.ld + d, one.data(), sizeof(double), hipMemcpyHostToDevice, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+}
+
+template <typename T>
+static void solve(sfmba_problem& H, Work<T>& W, const sfmba_params* prm, const sfmba_options* opt,
+                  sfmba_output* out, sfmba_stats* st) {
+  using clock = std::chrono::steady_clock;
+  HIP_CHECK(hipSetDevice(H.device));
+  const Layout& L = H.pr.L;
+  hipStream_t s = H.s;
+  const bool mixed = sizeof(T) == 4;
+  if (mixed && opt->refine_steps > 0 && L.D && !W.S_copy)
+    W.S_copy = H.pr.arena.alloc<double>((size_t)L.ld * L.Dp);
+  const bool refine = mixed && opt->refine_steps > 0 && L.D;
+  double* h = H.pinned;
   const int loss = prm->robust_loss;
   const double scale = prm->loss_scale_px;
-  const bool mixed = sizeof(T) == 4;
   Profiler prof;
   prof.on = opt->profile != 0;
   prof.s = s;
-
-  auto read_cost = [&](int which) {
-    launch_cost(L, W.poses[which], W.cams[which], W.pts[which], loss, scale, W.cost_part, W.cost, 0, s);
-    HIP_CHECK(hipMemcpyAsync(h, W.cost, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipStreamSynchronize(s));
-    return std::pair<double, double>(h[0], h[1]);
-  };
-
   if (st) {
     st->dim = L.D;
     st->num_schur_pairs = L.n_pairs;
+    st->time_upload_s = H.pack_s + H.upload_s;
   }
-  int cur = 0;
-  double cost;
-  size_t iterations_run = 0;
-  double initial_cost;
-  if (L.D == 0) {
-    HIP_CHECK(hipStreamSynchronize(s));
-    const auto t1 = clock::now();
-    initial_cost = read_cost(0).first;
-    cost = initial_cost;
-    if (st) {
-      st->time_upload_s = std::chrono::duration<double>(t1 - t0).count();
-      st->time_total_s = std::chrono::duration<double>(clock::now() - t1).count();
-    }
-  } else {
-    W.J4 = A.alloc<T>(4 * (size_t)L.n_rows);
-    W.E4 = A.alloc<T>(4 * (size_t)L.n_rows);
-    W.F4 = A.alloc<T>(4 * (size_t)L.n_rows);
-    W.obs_pc = A.alloc<T>(9 * (size_t)L.n_obs);
-    W.obs_valid = A.alloc<uint8_t>(L.n_obs);
-    W.pt_C = A.alloc<double>(6 * (size_t)L.n_pt);
-    W.pt_g = A.alloc<double>(3 * (size_t)L.n_pt);
-    W.pt_active = A.alloc<uint8_t>(L.n_pt);
-    W.pt_cinv = A.alloc<double>(9 * (size_t)L.n_pt);
-    W.pt_ok = A.alloc<uint8_t>(L.n_pt);
-    const size_t s_elems = (size_t)L.ld * L.Dp;
-    W.S = A.alloc<double>(s_elems);
-    W.Linv = A.alloc<double>((size_t)L.Dp * NB);
-    W.delta = A.alloc<double>(L.Dp);
-    W.vec = A.alloc<double>(L.Dp);
-    W.rhs = A.alloc<double>(L.Dp);
-    W.parts = A.alloc<double>((size_t)std::max<uint32_t>(L.n_parts, 1) * (2 * 196 + 16));
-    double* vec2 = A.alloc<double>(L.Dp);
-    double* vec3 = A.alloc<double>(L.Dp);
-    int* piv = A.alloc<int>(L.Dp);
-    if (mixed && opt->refine_steps > 0) W.S_copy = A.alloc<double>(s_elems);
-    // S: zero everything once, identity on the padding diagonal.
-    HIP_CHECK(hipMemsetAsync(W.S, 0, s_elems * sizeof(double), s));
-    {
-      std::vector<double> one(1, 1.0);
-      for (uint32_t d = L.D; d < L.Dp; ++d)
-        HIP_CHECK(hipMemcpyAsync(W.S + (size_t)d * L.ld + d, one.data(), sizeof(double), hipMemcpyHostToDevice, s));
-      HIP_CHECK(hipStreamSynchronize(s));
-    }
-    const auto t1 = clock::now();
-    if (st) st->time_upload_s = std::chrono::duration<double>(t1 - t0).count();
+  // Every solve starts from the problem's initial state.
+  HIP_CHECK(hipMemcpyAsync(W.poses[0], H.init_poses, 7 * (size_t)L.n_img * sizeof(double), hipMemcpyDeviceToDevice, s));
+  HIP_CHECK(hipMemcpyAsync(W.cams[0], H.init_cams, 8 * (size_t)L.n_cam * sizeof(double), hipMemcpyDeviceToDevice, s));
+  HIP_CHECK(hipMemcpyAsync(W.pts[0], H.init_pts, 3 * (size_t)L.n_pt * sizeof(double), hipMemcpyDeviceToDevice, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  const auto t1 = clock::now();
 
-    auto c0 = read_cost(0);
-    initial_cost = c0.first;
-    double n_front = c0.second;
-    cost = initial_cost;
+  int cur = 0;
+  launch_cost(L, W.poses[0], W.cams[0], W.pts[0], loss, scale, W.cost_part, W.cost, 0, s);
+  HIP_CHECK(hipMemcpyAsync(h, W.cost, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  const double initial_cost = h[0];
+  double n_front = h[1];
+  double cost = initial_cost;
+  size_t iterations_run = 0;
+
+  if (L.D != 0) {
     double lambda = prm->initial_lambda;
     double nu = 2.0;
-
-    auto factor_and_solve = [&](double lam, bool& ok) {
+    // One damping try: reduced system, factorization, step, trial state and
+    // its cost, with one host synchronization.  Returns false when neither
+    // Cholesky nor LU gives a step.
+    auto try_step = [&](double lam, double& trial) {
       prof.begin(1);
       launch_point_prepare<T>(L, W, lam, s);
       launch_schur<T>(L, W, lam, s);
       prof.end();
-      if (W.S_copy) {
+      if (refine) {
         dense_mirror_copy(W.S, W.S_copy, L.Dp, L.ld, s);
         dense_extract_row(W.S, L.Dp, L.ld, L.Dp, W.rhs, s);
       }
@@ -524,41 +572,52 @@ static void run_lm(const sfmba_input* in, const sfmba_params* prm, const sfmba_o
       HIP_CHECK(hipMemsetAsync(W.flags, 0, 4 * sizeof(int), s));
       dense_cholesky(W.S, W.Linv, L.Dp, L.ld, L.D, W.flags, s);
       prof.end();
+      prof.begin(3);
+      dense_extract_row(W.S, L.Dp, L.ld, L.Dp, W.vec, s);
+      dense_backsolve(W.S, W.Linv, L.Dp, L.ld, W.vec, W.delta, s);
+      for (int it = 0; refine && it < opt->refine_steps; ++it) {
+        // FP64 iterative refinement of the step: r = rhs - S delta,
+        // delta += S^{-1} r with the same factor.
+        dense_symv_residual(W.S_copy, L.Dp, L.ld, W.delta, W.rhs, W.vec, s);
+        dense_forwardsolve(W.S, W.Linv, L.Dp, L.ld, W.vec, H.vec2, s);
+        dense_backsolve(W.S, W.Linv, L.Dp, L.ld, H.vec2, H.vec3, s);
+        dense_vec_add(W.delta, H.vec3, L.Dp, s);
+      }
+      prof.end();
+      prof.begin(4);
+      launch_backsub_update<T>(L, W, cur, s);
+      launch_cost(L, W.poses[1 - cur], W.cams[1 - cur], W.pts[1 - cur], loss, scale, W.cost_part, W.cost, 0, s);
+      prof.end();
+      HIP_CHECK(hipMemcpyAsync(h, W.cost, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
       HIP_CHECK(hipMemcpyAsync(h + 4, W.flags, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
       HIP_CHECK(hipStreamSynchronize(s));
       const int* fl = reinterpret_cast<const int*>(h + 4);
       if (fl[0] == 0) {
-        prof.begin(3);
-        dense_extract_row(W.S, L.Dp, L.ld, L.Dp, W.vec, s);
-        dense_backsolve(W.S, W.Linv, L.Dp, L.ld, W.vec, W.delta, s);
-        for (int it = 0; W.S_copy && it < opt->refine_steps; ++it) {
-          // FP64 iterative refinement of the step: r = rhs - S delta,
-          // delta += S^{-1} r with the same factor.
-          dense_symv_residual(W.S_copy, L.Dp, L.ld, W.delta, W.rhs, W.vec, s);
-          dense_forwardsolve(W.S, W.Linv, L.Dp, L.ld, W.vec, vec2, s);
-          dense_backsolve(W.S, W.Linv, L.Dp, L.ld, vec2, vec3, s);
-          dense_vec_add(W.delta, vec3, L.Dp, s);
-        }
-        prof.end();
-        ok = true;
-        return;
+        trial = h[0];
+        return true;
       }
-      if (st) st->num_cholesky_failures++;
       // Not positive definite: rebuild S and solve by LU (lib.rs:1083-1093).
+      if (st) st->num_cholesky_failures++;
       launch_point_prepare<T>(L, W, lam, s);
       launch_schur<T>(L, W, lam, s);
-      dense_lu_solve(W.S, L.Dp, L.ld, L.D, W.delta, piv, W.flags, s);
+      dense_lu_solve(W.S, L.Dp, L.ld, L.D, W.delta, H.piv, W.flags, s);
       HIP_CHECK(hipMemcpyAsync(h + 4, W.flags, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
       HIP_CHECK(hipStreamSynchronize(s));
       if (st) st->num_lu_solves++;
-      ok = fl[1] == 0;
+      if (fl[1] != 0) return false;
+      launch_backsub_update<T>(L, W, cur, s);
+      launch_cost(L, W.poses[1 - cur], W.cams[1 - cur], W.pts[1 - cur], loss, scale, W.cost_part, W.cost, 0, s);
+      HIP_CHECK(hipMemcpyAsync(h, W.cost, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+      HIP_CHECK(hipStreamSynchronize(s));
+      trial = h[0];
+      return true;
     };
 
     const double FUNCTION_TOLERANCE = 1e-6;
     for (uint64_t outer = 0; outer < prm->max_iterations; ++outer) {
       ++iterations_run;
       const double cost_before = cost;
-      if (n_front == 0.0) break;  // no observation in front of its camera
+      if (n_front == 0.0) break;  // no observation in front of its camera (lib.rs:855-857)
       prof.begin(0);
       launch_linearize<T>(L, W, cur, loss, scale, s);
       launch_point_accumulate<T>(L, W, s);
@@ -566,21 +625,14 @@ static void run_lm(const sfmba_input* in, const sfmba_params* prm, const sfmba_o
       if (st) st->num_linearizations++;
       bool accepted = false;
       for (int inner = 0; inner < 8; ++inner) {
-        bool ok = false;
-        factor_and_solve(lambda, ok);
+        double trial = 0.0;
+        const bool ok = try_step(lambda, trial);
         if (st) st->num_factorizations++;
         if (!ok) {
           lambda *= nu;
           nu *= 2.0;
           continue;
         }
-        prof.begin(4);
-        launch_backsub_update<T>(L, W, cur, s);
-        launch_cost(L, W.poses[1 - cur], W.cams[1 - cur], W.pts[1 - cur], loss, scale, W.cost_part, W.cost, 0, s);
-        prof.end();
-        HIP_CHECK(hipMemcpyAsync(h, W.cost, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
-        HIP_CHECK(hipStreamSynchronize(s));
-        const double trial = h[0];
         if (st) push_stat(st->trial_costs, st->trial_costs_capacity, &st->trial_costs_len, trial);
         if (std::isfinite(trial) && trial < cost) {
           cur = 1 - cur;
@@ -600,10 +652,10 @@ static void run_lm(const sfmba_input* in, const sfmba_params* prm, const sfmba_o
       const double rel = (cost_before - cost) / std::max(cost_before, 1e-12);
       if (rel < FUNCTION_TOLERANCE) break;
     }
-    HIP_CHECK(hipStreamSynchronize(s));
-    if (st) st->time_total_s = std::chrono::duration<double>(clock::now() - t1).count();
-    prof.collect(st);
   }
+  HIP_CHECK(hipStreamSynchronize(s));
+  if (st) st->time_total_s = std::chrono::duration<double>(clock::now() - t1).count();
+  prof.collect(st);
 
   // Results back to the caller's arrays.
   std::vector<double> hp(7 * (size_t)L.n_img), hc(8 * (size_t)L.n_cam);
@@ -619,13 +671,51 @@ static void run_lm(const sfmba_input* in, const sfmba_params* prm, const sfmba_o
     q.tx = v[4]; q.ty = v[5]; q.tz = v[6];
   }
   for (uint32_t c = 0; c < L.n_cam; ++c) {
-    out->cameras[c] = in->cameras[c];
-    const int k = model_param_count_host(in->cameras[c].model_id);
+    out->cameras[c] = H.cams_in[c];
+    const int k = model_param_count_host(H.cams_in[c].model_id);
     for (int j = 0; j < k; ++j) out->cameras[c].params[j] = hc[8 * (size_t)c + j];
   }
   out->initial_cost = initial_cost;
   out->final_cost = cost;
   out->iterations_run = iterations_run;
+}
+
+static void solve_problem(sfmba_problem* H, const sfmba_params* params, const sfmba_options* options,
+                          sfmba_output* out, sfmba_stats* stats) {
+  if (!H || !params || !out) throw Error(SFMBA_ERR_INVALID_ARGUMENT, "NULL argument");
+  const Layout& L = H->pr.L;
+  if ((L.n_img && !out->poses) || (L.n_pt && !out->points) || (L.n_cam && !out->cameras))
+    throw Error(SFMBA_ERR_INVALID_ARGUMENT, "output arrays are NULL");
+  sfmba_options o;
+  sfmba_options_default(&o);
+  if (options) o = *options;
+  if (stats) {
+    // keep the caller's array pointers and capacities, reset the rest
+    sfmba_stats z{};
+    z.outer_costs = stats->outer_costs;
+    z.outer_costs_capacity = stats->outer_costs_capacity;
+    z.trial_costs = stats->trial_costs;
+    z.trial_costs_capacity = stats->trial_costs_capacity;
+    *stats = z;
+  }
+  if (o.solver != SFMBA_SOLVER_CHOLESKY && o.solver != SFMBA_SOLVER_PCG)
+    throw Error(SFMBA_ERR_INVALID_ARGUMENT, "unknown solver");
+  if (o.solver == SFMBA_SOLVER_PCG) throw Error(SFMBA_ERR_UNSUPPORTED, "PCG solver not built");
+  if (o.precision == SFMBA_PRECISION_MIXED) {
+    if (!H->have32) {
+      alloc_work<float>(*H, H->w32);
+      H->have32 = true;
+    }
+    solve<float>(*H, H->w32, params, &o, out, stats);
+  } else if (o.precision == SFMBA_PRECISION_FP64) {
+    if (!H->have64) {
+      alloc_work<double>(*H, H->w64);
+      H->have64 = true;
+    }
+    solve<double>(*H, H->w64, params, &o, out, stats);
+  } else {
+    throw Error(SFMBA_ERR_INVALID_ARGUMENT, "unknown precision");
+  }
 }
 
 // Layout without pair lists for the cost / Jacobian entry points.
@@ -702,33 +792,32 @@ int sfmba_device_count(void) {
 int sfmba_bundle_adjust_ex(const sfmba_input* input, const sfmba_params* params,
                            const sfmba_options* options, sfmba_output* out, sfmba_stats* stats) {
   return guarded([&] {
-    validate(input);
     if (!params || !out) throw Error(SFMBA_ERR_INVALID_ARGUMENT, "params/output is NULL");
-    if ((input->num_images && !out->poses) || (input->num_points && !out->points) ||
-        (input->num_cameras && !out->cameras))
-      throw Error(SFMBA_ERR_INVALID_ARGUMENT, "output arrays are NULL");
     sfmba_options o;
     sfmba_options_default(&o);
     if (options) o = *options;
-    if (stats) {
-      // keep the caller's array pointers and capacities, reset the rest
-      sfmba_stats z{};
-      z.outer_costs = stats->outer_costs;
-      z.outer_costs_capacity = stats->outer_costs_capacity;
-      z.trial_costs = stats->trial_costs;
-      z.trial_costs_capacity = stats->trial_costs_capacity;
-      *stats = z;
-    }
-    if (o.solver != SFMBA_SOLVER_CHOLESKY && o.solver != SFMBA_SOLVER_PCG)
-      throw Error(SFMBA_ERR_INVALID_ARGUMENT, "unknown solver");
-    if (o.solver == SFMBA_SOLVER_PCG) throw Error(SFMBA_ERR_UNSUPPORTED, "PCG solver not built");
-    if (o.precision == SFMBA_PRECISION_MIXED)
-      run_lm<float>(input, params, &o, out, stats);
-    else if (o.precision == SFMBA_PRECISION_FP64)
-      run_lm<double>(input, params, &o, out, stats);
-    else
-      throw Error(SFMBA_ERR_INVALID_ARGUMENT, "unknown precision");
+    std::unique_ptr<sfmba_problem> H(create_problem(input, o.device));
+    solve_problem(H.get(), params, &o, out, stats);
   });
+}
+
+int sfmba_problem_create(const sfmba_input* input, int32_t device, sfmba_problem** problem) {
+  return guarded([&] {
+    if (!problem) throw Error(SFMBA_ERR_INVALID_ARGUMENT, "problem is NULL");
+    *problem = nullptr;
+    *problem = create_problem(input, device);
+  });
+}
+
+int sfmba_problem_solve(sfmba_problem* problem, const sfmba_params* params, const sfmba_options* options,
+                        sfmba_output* out, sfmba_stats* stats) {
+  return guarded([&] { solve_problem(problem, params, options, out, stats); });
+}
+
+void sfmba_problem_destroy(sfmba_problem* problem) {
+  if (!problem) return;
+  (void)hipSetDevice(problem->device);
+  delete problem;
 }
 
 int sfmba_bundle_adjust(const sfmba_input* input, const sfmba_params* params, sfmba_output* out) {

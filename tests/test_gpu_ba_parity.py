@@ -155,3 +155,24 @@ def test_input_not_modified(c0_scene):
     ba.bundle_adjust(c0_scene.inp, ba.BaParams(max_iterations=3))
     for b, a in zip(before, (c0_scene.inp.poses, c0_scene.inp.points, c0_scene.inp.cameras)):
         assert np.array_equal(b, np.asarray(a))
+
+
+def test_failed_factorization_does_not_poison_later_solves():
+    # Regression: a Cholesky failure (NaN factor) must not leave NaN in the
+    # padding of the reduced system; a later solve on the same resident
+    # problem has to reproduce a fresh solve exactly.
+    # A negative damping makes the first pivot B00 (1 + lambda) < 0: the
+    # factorization fails with NaN (sqrt of a negative) on every try.
+    sc = scenes.generate("c0", num_images=20, num_points=300, seed=14)
+    inp = sc.inp
+    prob = ba.Problem(inp)
+    bad = prob.solve(ba.BaParams(max_iterations=1, initial_lambda=-2.0))
+    assert bad.stats["num_cholesky_failures"] >= 1
+    again = prob.solve(ba.BaParams(max_iterations=10))
+    fresh = ba.bundle_adjust(inp, ba.BaParams(max_iterations=10))
+    assert np.array_equal(again.stats["outer_costs"], fresh.stats["outer_costs"])
+    assert np.array_equal(again.points, fresh.points)
+    for precision in ("mixed", "fp64"):
+        a = prob.solve(ba.BaParams(max_iterations=5), precision=precision)
+        b = prob.solve(ba.BaParams(max_iterations=5), precision=precision)
+        assert np.array_equal(a.stats["outer_costs"], b.stats["outer_costs"])
