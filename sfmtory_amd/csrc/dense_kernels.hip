@@ -25,29 +25,50 @@ namespace sfmba {
 
 typedef double double4_t __attribute__((ext_vector_type(4)));
 
-constexpr int KC = 16;  // K chunk staged through LDS
+constexpr int KC = 16;   // K chunk staged through LDS
+constexpr int PW = 16;   // panel width inside a diagonal block
+constexpr int SMEM_DOUBLES = NB * NB + NB * PW + (NB - PW) * PW;  // block, its X11 blocks, trtri scratch
 
 // acc(128x128 tile, 4 waves of 64x64) += sign * A(128 x K) * B(128 x K)^T,
 // A and B column-major (rows contiguous) with leading dims lda, ldb.
 // The v_mfma_f64_16x16x4_f64 operand layout: lane l supplies A[l%16][l/16]
 // and B^T[l/16][l%16]; result register v of lane l is D[l/16 + 4v][l%16]
 // (checked on gfx950 with a one-wavefront probe of the builtin).
+// Software-pipelined: the next K chunk's global loads are in flight while
+// the current chunk's MFMAs run.
 __device__ inline void tile_mma(double4_t acc[4][4], const double* __restrict__ A, size_t lda,
                                 const double* __restrict__ B, size_t ldb, int K, double sign,
                                 double* As, double* Bs) {
   const int t = threadIdx.x;
   const int lane = t & 63, wave = t >> 6;
   const int wr = wave >> 1, wc = wave & 1;
+  constexpr int PER = (KC * 128) / 256;
+  double ra[PER], rb[PER];
+#pragma unroll
+  for (int it = 0; it < PER; ++it) {
+    const int idx = t + 256 * it;
+    const int k = idx >> 7, row = idx & 127;
+    ra[it] = A[(size_t)k * lda + row];
+    rb[it] = B[(size_t)k * ldb + row];
+  }
   for (int k0 = 0; k0 < K; k0 += KC) {
     __syncthreads();
 #pragma unroll
-    for (int it = 0; it < (KC * 128) / 256; ++it) {
+    for (int it = 0; it < PER; ++it) {
       const int idx = t + 256 * it;
-      const int k = idx >> 7, row = idx & 127;
-      As[k * 128 + row] = A[(size_t)(k0 + k) * lda + row];
-      Bs[k * 128 + row] = B[(size_t)(k0 + k) * ldb + row];
+      As[idx] = ra[it];
+      Bs[idx] = rb[it];
     }
     __syncthreads();
+    if (k0 + KC < K) {
+#pragma unroll
+      for (int it = 0; it < PER; ++it) {
+        const int idx = t + 256 * it;
+        const int k = idx >> 7, row = idx & 127;
+        ra[it] = A[(size_t)(k0 + KC + k) * lda + row];
+        rb[it] = B[(size_t)(k0 + KC + k) * ldb + row];
+      }
+    }
 #pragma unroll
     for (int kk = 0; kk < KC; kk += 4) {
       double a[4], b[4];
@@ -65,7 +86,7 @@ __device__ inline void tile_mma(double4_t acc[4][4], const double* __restrict__ 
   }
 }
 
-// Load / store a 128x128 tile into the MFMA accumulator layout.
+// Load a 128x128 tile into the MFMA accumulator layout.
 __device__ inline void tile_load(double4_t acc[4][4], const double* C, size_t ldc) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wr = wave >> 1, wc = wave & 1;
@@ -104,78 +125,283 @@ __device__ inline void tile_store(const double4_t acc[4][4], double* C, size_t l
     }
 }
 
-// ---------------------------------------------------------------- potrf
-// Packed lower-triangular index inside an NB block.
-__device__ inline int pk(int r, int c) { return c * NB - (c * (c - 1)) / 2 + (r - c); }
-constexpr int PACKED = NB * (NB + 1) / 2;
+// The accumulator tile into a column-major NB x NB LDS block, with the
+// identity on padding rows/columns (global index >= D).
+__device__ inline void tile_to_lds(const double4_t acc[4][4], double* L, uint32_t base, uint32_t D) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wr = wave >> 1, wc = wave & 1;
+#pragma unroll
+  for (int m = 0; m < 4; ++m)
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+      const int col = wc * 64 + n * 16 + (lane & 15);
+      const int row = wr * 64 + m * 16 + (lane >> 4);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = row + 4 * v;
+        const bool pad = base + r >= D || base + col >= D;
+        L[col * NB + r] = pad ? (r == col ? 1.0 : 0.0) : acc[m][n][v];
+      }
+    }
+}
+
+// ------------------------------------------------------- diagonal block
+// Broadcast lane l's double to the whole wavefront (v_readlane).
+__device__ __attribute__((always_inline)) inline double readlane_f64(double v, int l) {
+  int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+  int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+  return __hiloint2double(hi, lo);
+}
+
+// 1/sqrt(d): the hardware estimate refined by one Newton step (a few ulp).
+__device__ __attribute__((always_inline)) inline double rsqrt_f64(double d) {
+  const double y = __builtin_amdgcn_rsq(d);
+  return y * (1.5 - 0.5 * d * y * y);
+}
+
+// Cholesky of a PW x PW block and its inverse, on wave 0 of the workgroup:
+// lane r < PW holds row r in registers; pivots and multipliers travel by
+// v_readlane, and each pivot costs one reciprocal square root (no divide),
+// which keeps the 16-step dependent chain short.  L11 is written back to A
+// (column-major, stride NB) and X11 = L11^{-1} (column-major, stride PW)
+// to Xd.  A pivot that is not > 0 (or NaN) on a real row sets flags[0].
+__device__ __attribute__((always_inline)) inline void wave_potrf16(double* A, int j0, double* Xd,
+                                                                    uint32_t base, uint32_t D,
+                                                                    int* flags) {
+  const int lane = threadIdx.x & 63;
+  const int r = lane & (PW - 1);
+  double row[PW];
+  double ipiv[PW];  // 1 / L[j][j]
+#pragma unroll
+  for (int c = 0; c < PW; ++c) row[c] = (c <= r) ? A[(j0 + c) * NB + j0 + r] : 0.0;
+#pragma unroll
+  for (int j = 0; j < PW; ++j) {
+    const double d = readlane_f64(row[j], j);
+    if (lane == 0 && !(d > 0.0) && base + j0 + j < D) atomicOr(&flags[0], 1);
+    const double ip = rsqrt_f64(d);
+    ipiv[j] = ip;
+    row[j] = (r == j) ? d * ip : (r > j ? row[j] * ip : row[j]);
+#pragma unroll
+    for (int c = j + 1; c < PW; ++c) {
+      const double lcj = readlane_f64(row[j], c);
+      if (r >= c) row[c] -= row[j] * lcj;
+    }
+  }
+  if (lane < PW) {
+#pragma unroll
+    for (int c = 0; c < PW; ++c)
+      if (c <= r) A[(j0 + c) * NB + j0 + r] = row[c];
+  }
+  // X11, lane c computes column c: x_i = (delta_ic - sum_{k<i} L[i][k] x_k) / L[i][i],
+  // L entries broadcast from the owning lane.
+  const int c = lane & (PW - 1);
+  double x[PW];
+#pragma unroll
+  for (int i = 0; i < PW; ++i) {
+    double s = (i == c) ? 1.0 : 0.0;
+#pragma unroll
+    for (int k = 0; k < i; ++k) s -= readlane_f64(row[k], i) * x[k];
+    x[i] = s * ipiv[i];
+  }
+  if (lane < PW) {
+#pragma unroll
+    for (int i = 0; i < PW; ++i) Xd[c * PW + i] = x[i];
+  }
+}
+
+// Cholesky of the lower NB x NB block in LDS (column-major, A[c*NB + r]),
+// in PW-wide panels: the register factorization above, the panel below it
+// as L21 = A21 X11^T (independent loads, no dependent chain), and a
+// 4x4-register-tiled rank-PW update of the trailing lower triangle.  The
+// inverses X11 of the diagonal sub-blocks are kept in Xd (NB/PW blocks of
+// PW x PW) for the inverse pass.  A pivot that is not > 0 (or NaN) on a
+// real row (base + j < D) sets flags[0] (nalgebra's Cholesky::new failure
+// test).
+__device__ __attribute__((always_inline)) inline void lds_potrf(double* A, double* Xd, uint32_t base,
+                                                                 uint32_t D, int* flags) {
+  const int t = threadIdx.x, wave = t >> 6;
+  for (int j0 = 0; j0 < NB; j0 += PW) {
+    double* X11 = Xd + (j0 / PW) * PW * PW;
+    if (wave == 0) wave_potrf16(A, j0, X11, base, D, flags);
+    __syncthreads();
+    const int h = NB - j0 - PW;
+    if (h == 0) break;
+    // L21 = A21 X11^T: four threads per row, each producing outputs
+    // c = q, q+4, q+8, q+12 (x_c = sum_{k <= c} a_k X11[c][k]).
+    // The trip count is uniform across the workgroup (barrier inside).
+    for (int e0 = 0; e0 < 4 * h; e0 += blockDim.x) {
+      const int e = e0 + t;
+      const bool valid = e < 4 * h;
+      const int rr = j0 + PW + (valid ? (e >> 2) : 0), q = e & 3;
+      double o[4];
+      if (valid) {
+        double a[PW];
+#pragma unroll
+        for (int k = 0; k < PW; ++k) a[k] = A[(j0 + k) * NB + rr];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+          const int cc = q + 4 * m;
+          double sum = 0.0;
+#pragma unroll
+          for (int k = 0; k < PW; ++k)
+            if (k <= cc) sum += a[k] * X11[k * PW + cc];
+          o[m] = sum;
+        }
+      }
+      __syncthreads();  // every thread of the row has read it before it is overwritten
+      if (valid) {
+#pragma unroll
+        for (int m = 0; m < 4; ++m) A[(j0 + q + 4 * m) * NB + rr] = o[m];
+      }
+    }
+    __syncthreads();
+    const int T = h / 4;
+    const int ntiles = T * (T + 1) / 2;
+    for (int e = t; e < ntiles; e += blockDim.x) {
+      int ti = (int)((sqrtf(8.0f * e + 1.0f) - 1.0f) * 0.5f);
+      while (ti * (ti + 1) / 2 > e) --ti;
+      while ((ti + 1) * (ti + 2) / 2 <= e) ++ti;
+      const int tj = e - ti * (ti + 1) / 2;
+      const int r0 = j0 + PW + 4 * ti, c0 = j0 + PW + 4 * tj;
+      double acc[4][4];
+#pragma unroll
+      for (int p = 0; p < 4; ++p)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[p][q] = A[(c0 + q) * NB + r0 + p];
+#pragma unroll 4
+      for (int k = 0; k < PW; ++k) {
+        double lr[4], lc[4];
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+          lr[p] = A[(j0 + k) * NB + r0 + p];
+          lc[p] = A[(j0 + k) * NB + c0 + p];
+        }
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) acc[p][q] -= lr[p] * lc[q];
+      }
+#pragma unroll
+      for (int p = 0; p < 4; ++p)
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          if (ti != tj || p >= q) A[(c0 + q) * NB + r0 + p] = acc[p][q];
+    }
+    __syncthreads();
+  }
+}
+
+// In-place inverse of the lower triangle L held in LDS, block columns right
+// to left: A21 <- -(X22 A21) X11 with X22 the already inverted trailing
+// triangle and X11 kept by lds_potrf, then the diagonal block <- X11.
+// tmp holds (NB - PW) x PW doubles.
+__device__ __attribute__((always_inline)) inline void lds_trtri(double* A, const double* Xd, double* tmp) {
+  const int t = threadIdx.x;
+  for (int j0 = NB - PW; j0 >= 0; j0 -= PW) {
+    const double* X11 = Xd + (j0 / PW) * PW * PW;
+    const int h = NB - j0 - PW;
+    if (h > 0) {
+      for (int e = t; e < h * PW; e += blockDim.x) {
+        const int c = e / h, rr = e - c * h;
+        tmp[c * h + rr] = A[(j0 + c) * NB + j0 + PW + rr];
+      }
+      __syncthreads();
+      // A21 <- X22 * A21, 4x4 output tiles (X22 lower, already inverted)
+      const int nt = (h / 4) * (PW / 4);
+      for (int e = t; e < nt; e += blockDim.x) {
+        const int ti = e / (PW / 4), tj = e - ti * (PW / 4);
+        const int r0 = 4 * ti, c0 = 4 * tj;
+        double acc[4][4];
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) acc[p][q] = 0.0;
+#pragma unroll 4
+        for (int k = 0; k < r0 + 4; ++k) {
+          double xr[4], tc[4];
+#pragma unroll
+          for (int p = 0; p < 4; ++p) xr[p] = k <= r0 + p ? A[(j0 + PW + k) * NB + j0 + PW + r0 + p] : 0.0;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) tc[q] = tmp[(c0 + q) * h + k];
+#pragma unroll
+          for (int p = 0; p < 4; ++p)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc[p][q] += xr[p] * tc[q];
+        }
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) A[(j0 + c0 + q) * NB + j0 + PW + r0 + p] = acc[p][q];
+      }
+      __syncthreads();
+      // A21 <- -A21 X11: y_c = -sum_{k >= c} a_k X11[k][c], in place from the first column
+      if (t < h) {
+        const int rr = j0 + PW + t;
+        double a[PW];
+#pragma unroll
+        for (int k = 0; k < PW; ++k) a[k] = A[(j0 + k) * NB + rr];
+#pragma unroll
+        for (int c = 0; c < PW; ++c) {
+          double s = 0.0;
+#pragma unroll
+          for (int k = c; k < PW; ++k) s += a[k] * X11[c * PW + k];
+          a[c] = -s;
+        }
+#pragma unroll
+        for (int k = 0; k < PW; ++k) A[(j0 + k) * NB + rr] = a[k];
+      }
+    }
+    for (int e = t; e < PW * PW; e += blockDim.x) {
+      const int c = e / PW, rr = e - c * PW;
+      if (rr >= c) A[(j0 + c) * NB + j0 + rr] = X11[c * PW + rr];
+    }
+    __syncthreads();
+  }
+}
+
+// Factor the diagonal block held in LDS, store L (real rows) into S and
+// L^{-1} (identity on padding) into Linv.
+__device__ __attribute__((always_inline)) inline void factor_diag_block(double* smem, double* __restrict__ S,
+                                                                         size_t ld, int k, uint32_t D,
+                                                                         double* __restrict__ Linv,
+                                                                         int* __restrict__ flags) {
+  const int t = threadIdx.x;
+  const uint32_t base = (uint32_t)k * NB;
+  double* A = smem;
+  double* Xd = smem + NB * NB;
+  double* tmp = Xd + NB * PW;
+  __syncthreads();
+  lds_potrf(A, Xd, base, D, flags);
+  double* G = S + (size_t)k * NB * ld + (size_t)k * NB;
+  for (int idx = t; idx < NB * NB; idx += blockDim.x) {
+    const int c = idx >> 7, r = idx & 127;
+    if (r >= c && base + r < D) G[(size_t)c * ld + r] = A[c * NB + r];
+  }
+  __syncthreads();
+  lds_trtri(A, Xd, tmp);
+  double* X = Linv + (size_t)k * NB * NB;
+  for (int idx = t; idx < NB * NB; idx += blockDim.x) {
+    const int c = idx >> 7, r = idx & 127;
+    const bool pad = base + r >= D || base + c >= D;
+    X[idx] = pad ? (r == c ? 1.0 : 0.0) : (r >= c ? A[c * NB + r] : 0.0);
+  }
+}
 
 __global__ __launch_bounds__(256) void k_potrf_block(double* __restrict__ S, size_t ld, int k,
                                                      uint32_t D, double* __restrict__ Linv,
                                                      int* __restrict__ flags) {
-  __shared__ double Lp[PACKED];
-  __shared__ double Xp[PACKED];
-  __shared__ double piv;
-  const int t = threadIdx.x;
-  double* A = S + (size_t)k * NB * ld + (size_t)k * NB;
-  for (int idx = t; idx < NB * NB; idx += 256) {
+  __shared__ double smem[SMEM_DOUBLES];
+  const uint32_t base = (uint32_t)k * NB;
+  const double* G = S + (size_t)k * NB * ld + (size_t)k * NB;
+  for (int idx = threadIdx.x; idx < NB * NB; idx += blockDim.x) {
     const int c = idx >> 7, r = idx & 127;
-    if (r >= c) Lp[pk(r, c)] = A[(size_t)c * ld + r];
+    const bool pad = base + r >= D || base + c >= D;
+    smem[c * NB + r] = pad ? (r == c ? 1.0 : 0.0) : (r >= c ? G[(size_t)c * ld + r] : 0.0);
   }
-  __syncthreads();
-  // Right-looking unblocked Cholesky in LDS.
-  for (int j = 0; j < NB; ++j) {
-    if (t == 0) {
-      const double d = Lp[pk(j, j)];
-      if (!(d > 0.0) && (uint32_t)(k * NB + j) < D) atomicOr(&flags[0], 1);
-      piv = sqrt(d);
-      Lp[pk(j, j)] = piv;
-    }
-    __syncthreads();
-    const double p = piv;
-    for (int r = j + 1 + t; r < NB; r += 256) Lp[pk(r, j)] = Lp[pk(r, j)] / p;
-    __syncthreads();
-    const int m = NB - 1 - j;  // trailing size
-    const int cnt = m * (m + 1) / 2;
-    for (int e = t; e < cnt; e += 256) {
-      // e -> (rr >= cc) in the trailing triangle, column-major packed order
-      int cc = (int)((2 * m + 1 - sqrt((double)(2 * m + 1) * (2 * m + 1) - 8.0 * e)) * 0.5);
-      if (cc < 0) cc = 0;
-      while (cc > 0 && cc * m - (cc * (cc - 1)) / 2 > e) --cc;
-      while ((cc + 1) * m - ((cc + 1) * cc) / 2 <= e) ++cc;
-      const int rr = cc + (e - (cc * m - (cc * (cc - 1)) / 2));
-      const int r = j + 1 + rr, c = j + 1 + cc;
-      Lp[pk(r, c)] -= Lp[pk(r, j)] * Lp[pk(c, j)];
-    }
-    __syncthreads();
-  }
-  for (int idx = t; idx < NB * NB; idx += 256) {
-    const int c = idx >> 7, r = idx & 127;
-    if (r >= c && (uint32_t)(k * NB + r) < D) A[(size_t)c * ld + r] = Lp[pk(r, c)];
-  }
-  // Inverse: X = L^{-1}, row by row; column c of row i handled by thread
-  // pair (2c, 2c+1) splitting the inner sum.
-  for (int i = 0; i < NB; ++i) {
-    const int c = t >> 1, half = t & 1;
-    double s = 0.0;
-    if (c < i) {
-      for (int kk = c + half; kk < i; kk += 2) s += Lp[pk(i, kk)] * Xp[pk(kk, c)];
-    }
-    s += __shfl_xor(s, 1);
-    if (half == 0 && c <= i) {
-      const double lii = Lp[pk(i, i)];
-      Xp[pk(i, c)] = (c == i) ? 1.0 / lii : -s / lii;
-    }
-    __syncthreads();
-  }
-  double* X = Linv + (size_t)k * NB * NB;
-  for (int idx = t; idx < NB * NB; idx += 256) {
-    const int c = idx >> 7, r = idx & 127;
-    const bool pad = (uint32_t)(k * NB + r) >= D || (uint32_t)(k * NB + c) >= D;
-    X[idx] = pad ? (r == c ? 1.0 : 0.0) : (r >= c ? Xp[pk(r, c)] : 0.0);
-  }
+  factor_diag_block(smem, S, ld, k, D, Linv, flags);
 }
 
-// L_ik = A_ik Linv^T for row blocks i = k+1 .. nrb-1 (nrb includes the rhs block).
+// L_ik = A_ik Linv_k^T for row blocks i = k+1 .. nblk (nblk: the rhs block).
 __global__ __launch_bounds__(256) void k_trsm_panel(double* __restrict__ S, size_t ld, int k,
                                                     const double* __restrict__ Linv, uint32_t D,
                                                     uint32_t Dp) {
@@ -194,11 +420,16 @@ __global__ __launch_bounds__(256) void k_trsm_panel(double* __restrict__ S, size
   tile_store(acc, A, ld, false, (uint32_t)i * NB, (uint32_t)k * NB, D, Dp);
 }
 
-// A_ij -= L_ik L_jk^T for k < j <= i, j < nblk, i <= nblk (i == nblk: rhs block).
+// A_ij -= L_ik L_jk^T for k < j <= i, j < nblk, i <= nblk (i == nblk: rhs
+// block).  The workgroup of the next diagonal tile (k+1, k+1) factors it
+// right away (lookahead), so the next step's panel solve can start.
 __global__ __launch_bounds__(256) void k_syrk_update(double* __restrict__ S, size_t ld, int k,
-                                                     int nblk, uint32_t D, uint32_t Dp) {
-  __shared__ double As[KC * 128];
-  __shared__ double Bs[KC * 128];
+                                                     int nblk, uint32_t D, uint32_t Dp,
+                                                     double* __restrict__ Linv,
+                                                     int* __restrict__ flags) {
+  __shared__ double smem[SMEM_DOUBLES];
+  // Workgroup (0, 0) is the next diagonal tile (k+1, k+1): dispatched first,
+  // so its factorization overlaps the other tiles' updates.
   const int j = k + 1 + blockIdx.x;
   const int i = k + 1 + blockIdx.y;
   if (j > i || j >= nblk) return;
@@ -207,36 +438,43 @@ __global__ __launch_bounds__(256) void k_syrk_update(double* __restrict__ S, siz
   const double* Aj = S + (size_t)k * NB * ld + (size_t)j * NB;
   double4_t acc[4][4];
   tile_load(acc, C, ld);
-  tile_mma(acc, Ai, ld, Aj, ld, NB, -1.0, As, Bs);
+  tile_mma(acc, Ai, ld, Aj, ld, NB, -1.0, smem, smem + KC * 128);
+  if (i == k + 1 && j == k + 1) {
+    __syncthreads();
+    tile_to_lds(acc, smem, (uint32_t)j * NB, D);
+    factor_diag_block(smem, S, ld, k + 1, D, Linv, flags);
+    return;
+  }
   tile_store(acc, C, ld, i == j, (uint32_t)i * NB, (uint32_t)j * NB, D, Dp);
 }
 
 // ------------------------------------------------------ triangular solves
-// Backward solve L^T x = y, column-block k (from the last): x_k = Linv_k^T y_k
-// (every workgroup recomputes it), workgroup 0 stores x_k, and all
-// workgroups apply y_c -= L[block k, c] . x_k for c < k*NB.
-__global__ __launch_bounds__(256) void k_backsolve_step(const double* __restrict__ S, size_t ld,
-                                                        const double* __restrict__ Linv, int k,
-                                                        double* __restrict__ y,
-                                                        double* __restrict__ x) {
-  __shared__ double xs[NB];
+// Backward solve L^T x = y, block k (last to first), in two launches:
+// x_k = Linv_k^T y_k (one workgroup), then y_c -= L[block k, c] . x_k for
+// every column c < k*NB.
+__global__ __launch_bounds__(128) void k_backsolve_block(const double* __restrict__ Linv, int k,
+                                                         const double* __restrict__ y,
+                                                         double* __restrict__ x) {
   __shared__ double ys[NB];
   const int t = threadIdx.x;
-  if (t < NB) ys[t] = y[(size_t)k * NB + t];
+  ys[t] = y[(size_t)k * NB + t];
   __syncthreads();
   const double* X = Linv + (size_t)k * NB * NB;
-  if (t < NB) {
-    // x[t] = sum_r X[r][t] y[r], r >= t
-    double s = 0.0;
-    for (int r = t; r < NB; ++r) s += X[(size_t)t * NB + r] * ys[r];
-    xs[t] = s;
-  }
+  double s = 0.0;
+  for (int r = t; r < NB; ++r) s += X[(size_t)t * NB + r] * ys[r];
+  x[(size_t)k * NB + t] = s;
+}
+
+__global__ __launch_bounds__(256) void k_backsolve_update(const double* __restrict__ S, size_t ld, int k,
+                                                          const double* __restrict__ x,
+                                                          double* __restrict__ y) {
+  __shared__ double xs[NB];
+  const int t = threadIdx.x;
+  if (t < NB) xs[t] = x[(size_t)k * NB + t];
   __syncthreads();
-  if (blockIdx.x == 0 && t < NB) x[(size_t)k * NB + t] = xs[t];
-  // update: columns c in [0, k*NB), 4 waves, each lane group handles a column
   const int ncols = k * NB;
   const int lane = t & 63, wave = t >> 6;
-  for (int c = (blockIdx.x * 4 + wave); c < ncols; c += gridDim.x * 4) {
+  for (int c = blockIdx.x * 4 + wave; c < ncols; c += gridDim.x * 4) {
     const double* col = S + (size_t)c * ld + (size_t)k * NB;
     double s = col[lane] * xs[lane] + col[lane + 64] * xs[lane + 64];
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
@@ -244,33 +482,33 @@ __global__ __launch_bounds__(256) void k_backsolve_step(const double* __restrict
   }
 }
 
-// Forward solve L y = b in place: y_k = Linv_k b_k, then b_r -= L[r, block k] y_k
-// for rows r below.
-__global__ __launch_bounds__(256) void k_forwardsolve_step(const double* __restrict__ S, size_t ld,
-                                                           const double* __restrict__ Linv, int k,
-                                                           uint32_t Dp, double* __restrict__ b,
-                                                           double* __restrict__ y) {
+// Forward solve L y = b: y_k = Linv_k b_k (one workgroup), then
+// b_r -= L[r, block k] . y_k for the rows below.
+__global__ __launch_bounds__(128) void k_forward_block(const double* __restrict__ Linv, int k,
+                                                       const double* __restrict__ b,
+                                                       double* __restrict__ y) {
   __shared__ double bs[NB];
-  __shared__ double ys[NB];
   const int t = threadIdx.x;
-  if (t < NB) bs[t] = b[(size_t)k * NB + t];
+  bs[t] = b[(size_t)k * NB + t];
   __syncthreads();
   const double* X = Linv + (size_t)k * NB * NB;
-  if (t < NB) {
-    double s = 0.0;
-    for (int c = 0; c <= t; ++c) s += X[(size_t)c * NB + t] * bs[c];
-    ys[t] = s;
-  }
+  double s = 0.0;
+  for (int c = 0; c <= t; ++c) s += X[(size_t)c * NB + t] * bs[c];
+  y[(size_t)k * NB + t] = s;
+}
+
+__global__ __launch_bounds__(256) void k_forward_update(const double* __restrict__ S, size_t ld, int k,
+                                                        uint32_t Dp, const double* __restrict__ y,
+                                                        double* __restrict__ b) {
+  __shared__ double ys[NB];
+  const int t = threadIdx.x;
+  if (t < NB) ys[t] = y[(size_t)k * NB + t];
   __syncthreads();
-  const uint32_t r0 = (uint32_t)(k + 1) * NB;
-  // Every block re-derives y_k; block 0 stores it.  Only rows r >= r0 of b
-  // are written, disjoint from block k, which every block reads.
-  if (blockIdx.x == 0 && t < NB) y[(size_t)k * NB + t] = ys[t];
-  for (uint32_t r = r0 + blockIdx.x * 256 + t; r < Dp; r += gridDim.x * 256) {
-    double s = 0.0;
-    for (int c = 0; c < NB; ++c) s += S[(size_t)(k * NB + c) * ld + r] * ys[c];
-    b[r] -= s;
-  }
+  const uint32_t r = (uint32_t)(k + 1) * NB + blockIdx.x * 256 + t;
+  if (r >= Dp) return;
+  double s = 0.0;
+  for (int c = 0; c < NB; ++c) s += S[(size_t)(k * NB + c) * ld + r] * ys[c];
+  b[r] -= s;
 }
 
 // r = b - A x, A symmetric stored full (column-major, ld).
@@ -406,14 +644,14 @@ __global__ __launch_bounds__(1024) void k_lu_solve(const double* __restrict__ A,
 void dense_cholesky(double* S, double* Linv, uint32_t Dp, uint32_t ld, uint32_t D, int* flags,
                     hipStream_t s) {
   const int nblk = (int)(Dp / NB);
+  hipLaunchKernelGGL(k_potrf_block, dim3(1), dim3(256), 0, s, S, (size_t)ld, 0, D, Linv, flags);
   for (int k = 0; k < nblk; ++k) {
-    hipLaunchKernelGGL(k_potrf_block, dim3(1), dim3(256), 0, s, S, (size_t)ld, k, D, Linv, flags);
     const int nbelow = nblk - k;  // row blocks k+1 .. nblk (rhs block included)
     hipLaunchKernelGGL(k_trsm_panel, dim3(nbelow), dim3(256), 0, s, S, (size_t)ld, k, Linv, D, Dp);
     const int ntrail = nblk - 1 - k;
     if (ntrail > 0)
-      hipLaunchKernelGGL(k_syrk_update, dim3(ntrail, ntrail + 1), dim3(256), 0, s, S, (size_t)ld,
-                         k, nblk, D, Dp);
+      hipLaunchKernelGGL(k_syrk_update, dim3(ntrail, ntrail + 1), dim3(256), 0, s, S, (size_t)ld, k,
+                         nblk, D, Dp, Linv, flags);
   }
   HIP_CHECK(hipGetLastError());
 }
@@ -428,15 +666,16 @@ void dense_extract_row(const double* S, uint32_t Dp, uint32_t ld, uint32_t row, 
 void dense_backsolve(const double* S, const double* Linv, uint32_t Dp, uint32_t ld, double* y,
                      double* x, hipStream_t s) {
   // y is consumed as scratch (its blocks are reduced in place); x receives
-  // the solution.  Separate buffers: a workgroup writing x_k must not race
-  // with another one still reading y_k.
+  // the solution.
   const int nblk = (int)(Dp / NB);
   for (int k = nblk - 1; k >= 0; --k) {
+    hipLaunchKernelGGL(k_backsolve_block, dim3(1), dim3(NB), 0, s, Linv, k, y, x);
     const int ncols = k * NB;
-    int grid = (ncols + 4 * 16 - 1) / (4 * 16);
-    if (grid < 1) grid = 1;
-    if (grid > 1024) grid = 1024;
-    hipLaunchKernelGGL(k_backsolve_step, dim3(grid), dim3(256), 0, s, S, (size_t)ld, Linv, k, y, x);
+    if (ncols > 0) {
+      int grid = (ncols + 15) / 16;
+      if (grid > 2048) grid = 2048;
+      hipLaunchKernelGGL(k_backsolve_update, dim3(grid), dim3(256), 0, s, S, (size_t)ld, k, x, y);
+    }
   }
   HIP_CHECK(hipGetLastError());
 }
@@ -447,11 +686,11 @@ void dense_forwardsolve(const double* S, const double* Linv, uint32_t Dp, uint32
   // the solution.
   const int nblk = (int)(Dp / NB);
   for (int k = 0; k < nblk; ++k) {
+    hipLaunchKernelGGL(k_forward_block, dim3(1), dim3(NB), 0, s, Linv, k, b, y);
     const uint32_t rows = Dp - (uint32_t)(k + 1) * NB;
-    int grid = (int)((rows + 255) / 256);
-    if (grid < 1) grid = 1;
-    hipLaunchKernelGGL(k_forwardsolve_step, dim3(grid), dim3(256), 0, s, S, (size_t)ld, Linv, k, Dp,
-                       b, y);
+    if (rows > 0)
+      hipLaunchKernelGGL(k_forward_update, dim3((rows + 255) / 256), dim3(256), 0, s, S, (size_t)ld, k, Dp, y,
+                         b);
   }
   HIP_CHECK(hipGetLastError());
 }

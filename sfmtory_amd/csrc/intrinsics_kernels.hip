@@ -15,7 +15,7 @@ namespace sfmba {
 
 constexpr int RI_BLOCK = 256;
 
-__device__ inline double block_sum(double v, double* red) {
+__device__ __attribute__((always_inline)) inline double block_sum(double v, double* red) {
   const int t = threadIdx.x;
   __syncthreads();
   red[t] = v;
