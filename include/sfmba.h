@@ -152,7 +152,7 @@ typedef struct sfmba_options {
   int32_t solver;     /* SFMBA_SOLVER_* */
   int32_t device;     /* HIP device ordinal */
   int32_t profile;    /* 1: time each phase with HIP events into stats */
-  int32_t refine_steps; /* mixed precision: max FP64 refinement sweeps (default 3) */
+  int32_t refine_steps; /* mixed precision: FP64 iterative-refinement sweeps of the step (default 0) */
   int32_t pcg_max_iterations; /* PCG: iteration cap (default 500) */
   double pcg_tolerance;       /* PCG: relative residual target (default 1e-10) */
 } sfmba_options;

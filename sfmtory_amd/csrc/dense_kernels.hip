@@ -379,11 +379,14 @@ __device__ __attribute__((always_inline)) inline void factor_diag_block(double* 
   }
   __syncthreads();
   lds_trtri(A, Xd, tmp);
-  double* X = Linv + (size_t)k * NB * NB;
+  // Linv holds, per block, L^{-1} (column-major) followed by its transpose.
+  double* X = Linv + (size_t)k * 2 * NB * NB;
   for (int idx = t; idx < NB * NB; idx += blockDim.x) {
     const int c = idx >> 7, r = idx & 127;
     const bool pad = base + r >= D || base + c >= D;
     X[idx] = pad ? (r == c ? 1.0 : 0.0) : (r >= c ? A[c * NB + r] : 0.0);
+    const bool padT = base + c >= D || base + r >= D;
+    X[NB * NB + idx] = padT ? (r == c ? 1.0 : 0.0) : (c >= r ? A[r * NB + c] : 0.0);
   }
 }
 
@@ -409,7 +412,7 @@ __global__ __launch_bounds__(256) void k_trsm_panel(double* __restrict__ S, size
   __shared__ double Bs[KC * 128];
   const int i = k + 1 + blockIdx.x;
   double* A = S + (size_t)k * NB * ld + (size_t)i * NB;
-  const double* X = Linv + (size_t)k * NB * NB;
+  const double* X = Linv + (size_t)k * 2 * NB * NB;
   double4_t acc[4][4];
 #pragma unroll
   for (int m = 0; m < 4; ++m)
@@ -459,9 +462,12 @@ __global__ __launch_bounds__(128) void k_backsolve_block(const double* __restric
   const int t = threadIdx.x;
   ys[t] = y[(size_t)k * NB + t];
   __syncthreads();
-  const double* X = Linv + (size_t)k * NB * NB;
+  // x[t] = sum_r Linv[r][t] y[r] = sum_r LinvT[t][r] y[r]; LinvT column-major
+  // makes LinvT[t][r] = XT[r*NB + t] contiguous across threads.
+  const double* XT = Linv + (size_t)k * 2 * NB * NB + NB * NB;
   double s = 0.0;
-  for (int r = t; r < NB; ++r) s += X[(size_t)t * NB + r] * ys[r];
+#pragma unroll 8
+  for (int r = 0; r < NB; ++r) s += XT[(size_t)r * NB + t] * ys[r];
   x[(size_t)k * NB + t] = s;
 }
 
@@ -491,9 +497,10 @@ __global__ __launch_bounds__(128) void k_forward_block(const double* __restrict_
   const int t = threadIdx.x;
   bs[t] = b[(size_t)k * NB + t];
   __syncthreads();
-  const double* X = Linv + (size_t)k * NB * NB;
+  const double* X = Linv + (size_t)k * 2 * NB * NB;
   double s = 0.0;
-  for (int c = 0; c <= t; ++c) s += X[(size_t)c * NB + t] * bs[c];
+#pragma unroll 8
+  for (int c = 0; c < NB; ++c) s += X[(size_t)c * NB + t] * bs[c];
   y[(size_t)k * NB + t] = s;
 }
 

@@ -117,7 +117,7 @@ struct Work {
   double* dpt = nullptr;      // 3 per point
   double* S = nullptr;        // ld x Dp
   double* S_copy = nullptr;   // refinement copy (mixed precision)
-  double* Linv = nullptr;     // Dp/NB blocks of NB x NB
+  double* Linv = nullptr;     // per NB block: L^{-1} then its transpose (2 NB x NB)
   double* delta = nullptr;    // Dp
   double* vec = nullptr;      // Dp scratch
   double* rhs = nullptr;      // Dp

@@ -500,7 +500,7 @@ static void alloc_work(sfmba_problem& H, Work<T>& W) {
   W.pt_ok = A.alloc<uint8_t>(L.n_pt);
   const size_t s_elems = (size_t)L.ld * L.Dp;
   W.S = A.alloc<double>(s_elems);
-  W.Linv = A.alloc<double>((size_t)L.Dp * NB);
+  W.Linv = A.alloc<double>((size_t)L.Dp * NB * 2);
   W.delta = A.alloc<double>(L.Dp);
   W.vec = A.alloc<double>(L.Dp);
   W.rhs = A.alloc<double>(L.Dp);
@@ -559,7 +559,10 @@ static void solve(sfmba_problem& H, Work<T>& W, const sfmba_params* prm, const s
     // One damping try: reduced system, factorization, step, trial state and
     // its cost, with one host synchronization.  Returns false when neither
     // Cholesky nor LU gives a step.
-    auto try_step = [&](double lam, double& trial) {
+    // One damping try: reduced system at damping lam, factorization, step,
+    // trial state and its cost, with one host synchronization.  Returns 0
+    // with the trial cost, 1 when the factorization failed (flags[0]).
+    auto assemble_factor_step = [&](double lam, double& trial) {
       prof.begin(1);
       launch_point_prepare<T>(L, W, lam, s);
       launch_schur<T>(L, W, lam, s);
@@ -591,20 +594,40 @@ static void solve(sfmba_problem& H, Work<T>& W, const sfmba_params* prm, const s
       HIP_CHECK(hipMemcpyAsync(h, W.cost, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
       HIP_CHECK(hipMemcpyAsync(h + 4, W.flags, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
       HIP_CHECK(hipStreamSynchronize(s));
-      const int* fl = reinterpret_cast<const int*>(h + 4);
-      if (fl[0] == 0) {
-        trial = h[0];
-        return true;
+      trial = h[0];
+      return reinterpret_cast<const int*>(h + 4)[0] != 0;
+    };
+    // Returns false when no step could be formed (the try counts as rejected).
+    // Mixed form only: FP32 assembly resolves the reduced system to ~1e-7 of
+    // its diagonal, so a tiny Marquardt damping can leave it indefinite by
+    // rounding alone (the free scale gauge leaves an eigenvalue of order
+    // lambda).  After a failed factorization the mixed form refactors with a
+    // growing damping floor, which it then keeps for the rest of the solve;
+    // the step stays an LM step for that damping and acceptance still uses
+    // the FP64 cost.  An LU solve of an indefinite FP32-assembled system
+    // would carry no information, so the mixed form does not fall back to it.
+    double mixed_floor = 0.0;
+    auto try_step = [&](double lam_lm, double& trial) {
+      const double lam = mixed ? std::max(lam_lm, mixed_floor) : lam_lm;
+      if (!assemble_factor_step(lam, trial)) return true;
+      if (st) st->num_cholesky_failures++;
+      if (mixed) {
+        for (double shift : {1e-6, 1e-5, 1e-4, 1e-3, 1e-2}) {
+          if (shift <= lam) continue;
+          mixed_floor = shift;
+          if (!assemble_factor_step(shift, trial)) return true;
+          if (st) st->num_cholesky_failures++;
+        }
+        return false;
       }
       // Not positive definite: rebuild S and solve by LU (lib.rs:1083-1093).
-      if (st) st->num_cholesky_failures++;
       launch_point_prepare<T>(L, W, lam, s);
       launch_schur<T>(L, W, lam, s);
       dense_lu_solve(W.S, L.Dp, L.ld, L.D, W.delta, H.piv, W.flags, s);
       HIP_CHECK(hipMemcpyAsync(h + 4, W.flags, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
       HIP_CHECK(hipStreamSynchronize(s));
       if (st) st->num_lu_solves++;
-      if (fl[1] != 0) return false;
+      if (reinterpret_cast<const int*>(h + 4)[1] != 0) return false;
       launch_backsub_update<T>(L, W, cur, s);
       launch_cost(L, W.poses[1 - cur], W.cams[1 - cur], W.pts[1 - cur], loss, scale, W.cost_part, W.cost, 0, s);
       HIP_CHECK(hipMemcpyAsync(h, W.cost, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -767,7 +790,7 @@ void sfmba_options_default(sfmba_options* o) {
   o->solver = SFMBA_SOLVER_CHOLESKY;
   o->device = 0;
   o->profile = 0;
-  o->refine_steps = 1;
+  o->refine_steps = 0;
   o->pcg_max_iterations = 500;
   o->pcg_tolerance = 1e-10;
 }
@@ -922,7 +945,7 @@ int sfmba_dense_solve(const double* A, size_t n, const double* b, int32_t method
     }
     for (uint32_t d = D; d < Dp; ++d) h[(size_t)d * ld + d] = 1.0;
     double* S = ar.upload(h, sg.s);
-    double* Linv = ar.alloc<double>((size_t)Dp * NB);
+    double* Linv = ar.alloc<double>((size_t)Dp * NB * 2);
     double* y = ar.alloc<double>(Dp);
     double* xd = ar.alloc<double>(Dp);
     int* flags = ar.alloc<int>(4);
