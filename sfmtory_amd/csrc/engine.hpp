@@ -10,7 +10,7 @@
 //   block) in the row-indexed Jacobian stores:
 //     J4[row] = (J0, J1, w*J0, w*J1)   residual Jacobian column, plain and weighted
 //     E4[row] = (E0, E1, E2, g)        E = w J^T Jx (camera-point coupling), g = w J^T r
-//     F4[row] = (F0, F1, F2, 0)        F = E * Cinv(point), per damping try;
+//     F4[row] = (F0, F1, F2, F.g)      F = E * Cinv(point), g the point gradient, per try;
 //   the reduced camera system S is dense, column-major, lower triangle,
 //   (Dp + NB) x Dp with Dp = D rounded up to NB; padding is the identity and
 //   row Dp carries the right-hand side, so the blocked Cholesky produces
@@ -88,7 +88,7 @@ struct Layout {
   uint32_t* grp_off = nullptr;
   uint32_t* grp_dim = nullptr;
   // Schur pair lists (destination-sorted)
-  uint32_t* pairs = nullptr;      // 2 per pair (incidence a, incidence b)
+  uint32_t* pairs = nullptr;      // 2 per pair: first rows of incidences a, b (bit 31 flags)
   uint4* chunks = nullptr;        // (dest, begin, end, part or NONE)
   uint32_t* multi_dest = nullptr; // destinations reduced from partials
   uint32_t* dest_part0 = nullptr; // first partial of each multi destination

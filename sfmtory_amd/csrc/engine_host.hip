@@ -251,6 +251,7 @@ static void build_pack(const sfmba_input* in, Pack& P, bool need_pairs) {
   }
   for (uint32_t p = 0; p < P.n_pt; ++p) P.pt_inc_ptr[p + 1] = std::max(P.pt_inc_ptr[p + 1], P.pt_inc_ptr[p]);
   P.n_rows = rows;
+  if (rows >= 0x80000000u) throw Error(SFMBA_ERR_UNSUPPORTED, "too many Jacobian rows for 31-bit offsets");
   if (!need_pairs || D == 0) return;
 
   // Destination-sorted point pairs.  Every lower-triangular block (ga >= gb)
@@ -281,8 +282,9 @@ static void build_pack(const sfmba_input* in, Pack& P, bool need_pairs) {
           const uint64_t ga = P.inc_grp[a], gb = P.inc_grp[b];
           if (ga < gb) continue;
           const uint32_t q = pos[ga * (ga + 1) / 2 + gb]++;
-          P.pairs[2 * (size_t)q] = a;
-          P.pairs[2 * (size_t)q + 1] = b;
+          // first rows of the two incidences; bit 31: same observation / a == b
+          P.pairs[2 * (size_t)q] = P.inc_roff[a] | (P.inc_obs[a] == P.inc_obs[b] ? 0x80000000u : 0u);
+          P.pairs[2 * (size_t)q + 1] = P.inc_roff[b] | (a == b ? 0x80000000u : 0u);
         }
     }
   }

@@ -233,27 +233,33 @@ __global__ __launch_bounds__(256) void k_point_prepare(uint32_t n_pt, const doub
   pt_ok[p] = ok ? 1 : 0;
 }
 
-// F = E Cinv, one thread per incidence row.
+// F = E Cinv and F . g (the incidence's rhs share), one thread per incidence.
 template <typename T>
 __global__ __launch_bounds__(256) void k_compute_F(uint32_t n_inc, const uint32_t* __restrict__ inc_pt,
                                                    const uint32_t* __restrict__ inc_roff,
                                                    const uint32_t* __restrict__ inc_grp,
                                                    const uint32_t* __restrict__ grp_dim,
                                                    const double* __restrict__ pt_cinv,
+                                                   const double* __restrict__ pt_g,
                                                    const typename Vec4<T>::type* __restrict__ E4,
                                                    typename Vec4<T>::type* __restrict__ F4) {
   using V4 = typename Vec4<T>::type;
   const uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
   if (a >= n_inc) return;
-  const double* ci = pt_cinv + 9 * (size_t)inc_pt[a];
+  const uint32_t pt = inc_pt[a];
+  const double* ci = pt_cinv + 9 * (size_t)pt;
   T c[9];
 #pragma unroll
   for (int k = 0; k < 9; ++k) c[k] = (T)ci[k];
+  const T g0 = (T)pt_g[3 * (size_t)pt], g1 = (T)pt_g[3 * (size_t)pt + 1], g2 = (T)pt_g[3 * (size_t)pt + 2];
   const uint32_t r0 = inc_roff[a], n = grp_dim[inc_grp[a]];
   for (uint32_t r = 0; r < n; ++r) {
     const V4 e = E4[r0 + r];
-    F4[r0 + r] = V4{e.x * c[0] + e.y * c[3] + e.z * c[6], e.x * c[1] + e.y * c[4] + e.z * c[7],
-                    e.x * c[2] + e.y * c[5] + e.z * c[8], T(0)};
+    const T f0 = e.x * c[0] + e.y * c[3] + e.z * c[6];
+    const T f1 = e.x * c[1] + e.y * c[4] + e.z * c[7];
+    const T f2 = e.x * c[2] + e.y * c[5] + e.z * c[8];
+    // w: this row's share of rhs, F . g_point = E Cinv g_point
+    F4[r0 + r] = V4{f0, f1, f2, f0 * g0 + f1 * g1 + f2 * g2};
   }
 }
 
@@ -290,18 +296,22 @@ constexpr int PART_STRIDE = 2 * 196 + 16;
 // system.  Sums, over the chunk's point pairs (a, b), B = w J_a^T J_b for
 // pairs of the same observation and Schur = F_a E_b^T = E_a Cinv E_b^T; on
 // diagonal blocks also the right-hand side F_a g_point - w J_a^T r over
-// pairs a == b.  Single-chunk destinations are written straight into S
-// (with the Marquardt damping of the diagonal); others leave partials.
+// pairs a == b (F4.w - E4.w).  Each pair is one 8-byte word pair: the two
+// incidences' first rows, with bit 31 flagging "same observation" (first
+// word) and "a == b" (second word); four pairs are processed per step with
+// their loads issued together.  Single-chunk destinations are written
+// straight into S (with the Marquardt damping of the diagonal); others
+// leave partials.
 template <typename T>
 __global__ __launch_bounds__(256) void k_schur(
-    uint32_t n_chunks, const uint4* __restrict__ chunks, const uint32_t* __restrict__ pairs,
-    const uint32_t* __restrict__ inc_obs, const uint32_t* __restrict__ inc_pt,
-    const uint32_t* __restrict__ inc_roff, const uint32_t* __restrict__ grp_off,
-    const uint32_t* __restrict__ grp_dim, const typename Vec4<T>::type* __restrict__ J4,
-    const typename Vec4<T>::type* __restrict__ E4, const typename Vec4<T>::type* __restrict__ F4,
-    const double* __restrict__ pt_g, double lambda, double* __restrict__ S, uint32_t ld,
+    uint32_t n_chunks, const uint4* __restrict__ chunks, const uint2* __restrict__ pairs,
+    const uint32_t* __restrict__ grp_off, const uint32_t* __restrict__ grp_dim,
+    const typename Vec4<T>::type* __restrict__ J4, const typename Vec4<T>::type* __restrict__ E4,
+    const typename Vec4<T>::type* __restrict__ F4, double lambda, double* __restrict__ S, uint32_t ld,
     uint32_t Dp, double* __restrict__ parts) {
   using V4 = typename Vec4<T>::type;
+  constexpr uint32_t HI = 0x80000000u, LO = 0x7FFFFFFFu;
+  constexpr int U = 4;
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t ci = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
   if (ci >= n_chunks) return;
@@ -313,41 +323,45 @@ __global__ __launch_bounds__(256) void k_schur(
   const bool diag = ga == gb;
   Acc<T> accS[4], accB[4], accR;
   uint32_t er[4], ec[4];
+  bool on[4];
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     const uint32_t e = lane + 64 * q;
-    er[q] = e % da;
-    ec[q] = e / da;
+    on[q] = e < ne;
+    er[q] = on[q] ? e % da : 0;
+    ec[q] = on[q] ? e / da : 0;
   }
-  for (uint32_t p = ch.y; p < ch.z; ++p) {
-    const uint32_t a = pairs[2 * (size_t)p], b = pairs[2 * (size_t)p + 1];
-    const uint32_t ra = inc_roff[a], rb = inc_roff[b];
-    const bool same = inc_obs[a] == inc_obs[b];
+  const bool rlane = diag && lane < da;
+  for (uint32_t p = ch.y; p < ch.z; p += U) {
+    uint2 pr[U];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      if (lane + 64 * q < ne) {
-        const V4 f = F4[ra + er[q]];
-        const V4 e = E4[rb + ec[q]];
-        accS[q].add(f.x * e.x + f.y * e.y + f.z * e.z);
-        if (same) {
-          const V4 ja = J4[ra + er[q]];
-          const V4 jb = J4[rb + ec[q]];
-          accB[q].add(ja.z * jb.x + ja.w * jb.y);
+    for (int u = 0; u < U; ++u) pr[u] = (p + u < ch.z) ? pairs[p + u] : make_uint2(HI, 0u);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (p + u >= ch.z) continue;
+      const uint32_t ra = pr[u].x & LO, rb = pr[u].y & LO;
+      const bool same = (pr[u].x & HI) != 0u;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (on[q]) {
+          const V4 f = F4[ra + er[q]];
+          const V4 e = E4[rb + ec[q]];
+          accS[q].add(f.x * e.x + f.y * e.y + f.z * e.z);
+          if (same) {
+            const V4 ja = J4[ra + er[q]];
+            const V4 jb = J4[rb + ec[q]];
+            accB[q].add(ja.z * jb.x + ja.w * jb.y);
+          }
         }
       }
-    }
-    if (diag && a == b && lane < da) {
-      const double* g = pt_g + 3 * (size_t)inc_pt[a];
-      const V4 f = F4[ra + lane];
-      const V4 e = E4[ra + lane];
-      accR.add(f.x * (T)g[0] + f.y * (T)g[1] + f.z * (T)g[2] - e.w);
+      if (rlane && (pr[u].y & HI)) accR.add(F4[ra + lane].w - E4[ra + lane].w);
     }
   }
   const uint32_t oa = grp_off[ga], ob = grp_off[gb];
   if (ch.w == NONE) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      if (lane + 64 * q < ne) {
+      if (on[q]) {
         const uint32_t r = er[q], c = ec[q];
         if (diag && r < c) continue;
         const double bv = accB[q].value();
@@ -356,17 +370,17 @@ __global__ __launch_bounds__(256) void k_schur(
         S[(size_t)(ob + c) * ld + oa + r] = v;
       }
     }
-    if (diag && lane < da) S[(size_t)(oa + lane) * ld + Dp] = accR.value();
+    if (rlane) S[(size_t)(oa + lane) * ld + Dp] = accR.value();
   } else {
     double* pp = parts + (size_t)ch.w * PART_STRIDE;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      if (lane + 64 * q < ne) {
+      if (on[q]) {
         pp[lane + 64 * q] = accB[q].value();
         pp[196 + lane + 64 * q] = accS[q].value();
       }
     }
-    if (diag && lane < da) pp[392 + lane] = accR.value();
+    if (rlane) pp[392 + lane] = accR.value();
   }
 }
 
@@ -658,7 +672,7 @@ void launch_point_prepare(const Layout& L, Work<T>& W, double lambda, hipStream_
   HIP_CHECK(hipGetLastError());
   if (L.n_inc == 0) return;
   hipLaunchKernelGGL(k_compute_F<T>, dim3(blocks_for(L.n_inc, 256)), dim3(256), 0, s, L.n_inc,
-                     L.inc_pt, L.inc_roff, L.inc_grp, L.grp_dim, W.pt_cinv,
+                     L.inc_pt, L.inc_roff, L.inc_grp, L.grp_dim, W.pt_cinv, W.pt_g,
                      reinterpret_cast<const V4*>(W.E4), reinterpret_cast<V4*>(W.F4));
   HIP_CHECK(hipGetLastError());
 }
@@ -668,9 +682,9 @@ void launch_schur(const Layout& L, Work<T>& W, double lambda, hipStream_t s) {
   using V4 = typename Vec4<T>::type;
   if (L.n_chunks) {
     hipLaunchKernelGGL(k_schur<T>, dim3(blocks_for(L.n_chunks, 4)), dim3(256), 0, s, L.n_chunks,
-                       L.chunks, L.pairs, L.inc_obs, L.inc_pt, L.inc_roff, L.grp_off, L.grp_dim,
+                       L.chunks, reinterpret_cast<const uint2*>(L.pairs), L.grp_off, L.grp_dim,
                        reinterpret_cast<const V4*>(W.J4), reinterpret_cast<const V4*>(W.E4),
-                       reinterpret_cast<const V4*>(W.F4), W.pt_g, lambda, W.S, L.ld, L.Dp, W.parts);
+                       reinterpret_cast<const V4*>(W.F4), lambda, W.S, L.ld, L.Dp, W.parts);
     HIP_CHECK(hipGetLastError());
   }
   if (L.n_multi) {
