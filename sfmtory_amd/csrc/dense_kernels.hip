@@ -10,13 +10,19 @@
 // rides along with the trailing updates).
 //
 // Blocked right-looking Cholesky, per diagonal block k (NB = 128):
-//   k_potrf_block   one workgroup: L_kk = chol(A_kk) in LDS and its
-//                   inverse Linv_kk; a pivot that is not > 0 (or NaN) sets
-//                   flags[0] (nalgebra's failure test: diag <= 0 or NaN);
-//   k_trsm_panel    L_ik = A_ik Linv_kk^T for every row block i > k (and
-//                   the rhs block), 128x128x128 FP64 MFMA tiles;
+//   diagonal block  one workgroup factors L_kk in LDS: 16-column panels on
+//                   one wavefront in registers, MFMA trailing updates, then
+//                   the inverses X11 of its 16x16 diagonal blocks; a pivot
+//                   that is not > 0 (or NaN) sets flags[0] (nalgebra's
+//                   failure test: diag <= 0 or NaN);
+//   k_trsm_panel    L_ik = A_ik L_kk^{-T} for every row block i > k (and
+//                   the rhs row) by blocked substitution with the X11, one
+//                   wavefront per 16-row strip, all in MFMA registers;
 //   k_syrk_update   A_ij -= L_ik L_jk^T on the trailing lower triangle,
-//                   128x128 tiles, v_mfma_f64_16x16x4_f64.
+//                   128x128 tiles, v_mfma_f64_16x16x4_f64; its first
+//                   workgroup updates and factors the next diagonal block.
+// L_kk^{-1} (for the triangular solves) is formed for all blocks at once
+// after the factorization, off the critical path.
 #include <hip/hip_runtime.h>
 
 #include "engine.hpp"
@@ -25,9 +31,28 @@ namespace sfmba {
 
 typedef double double4_t __attribute__((ext_vector_type(4)));
 
-constexpr int KC = 16;   // K chunk staged through LDS
+constexpr int KC = 8;    // K chunk staged through LDS
+constexpr int KS = 144;  // LDS row stride of a staged K chunk (2 KS mod 64 = 32 banks)
 constexpr int PW = 16;   // panel width inside a diagonal block
-constexpr int SMEM_DOUBLES = NB * NB + NB * PW + (NB - PW) * PW;  // block, its X11 blocks, trtri scratch
+// The diagonal block in LDS is stored as its 36 lower 16x16 tiles, tile
+// (ti, tj), ti >= tj, at (ti (ti + 1) / 2 + tj) * 256, each column-major
+// with stride 16: every MFMA operand or accumulator access (16 rows of 4
+// columns of one tile) is then 64 consecutive doubles.  128 doubles after
+// the tiles hold 1/L[j][j], then four 16-double broadcast buffers.  75 KB:
+// two workgroups fit in a CU's LDS.
+constexpr int DIAG_TILES = 36;
+constexpr int DIAG_IP = DIAG_TILES * 256;
+constexpr int DIAG_COL = DIAG_IP + NB;  // per-wavefront 16-double broadcast buffers
+constexpr int SMEM_DIAG = DIAG_COL + 4 * PW;
+constexpr int SMEM_INV = NB * NB + NB * PW + (NB - PW) * PW;  // block, X11 blocks, trtri scratch
+
+__device__ __attribute__((always_inline)) inline int tile_index(int ti, int tj) {
+  return ti * (ti + 1) / 2 + tj;
+}
+// LDS offset of element (r, c), r >= c (tile-wise), of the diagonal block.
+__device__ __attribute__((always_inline)) inline int doff(int r, int c) {
+  return tile_index(r >> 4, c >> 4) * 256 + (c & 15) * 16 + (r & 15);
+}
 
 // acc(128x128 tile, 4 waves of 64x64) += sign * A(128 x K) * B(128 x K)^T,
 // A and B column-major (rows contiguous) with leading dims lda, ldb.
@@ -56,8 +81,9 @@ __device__ inline void tile_mma(double4_t acc[4][4], const double* __restrict__ 
 #pragma unroll
     for (int it = 0; it < PER; ++it) {
       const int idx = t + 256 * it;
-      As[idx] = ra[it];
-      Bs[idx] = rb[it];
+      const int k = idx >> 7, row = idx & 127;
+      As[k * KS + row] = ra[it];
+      Bs[k * KS + row] = rb[it];
     }
     __syncthreads();
     if (k0 + KC < K) {
@@ -74,9 +100,9 @@ __device__ inline void tile_mma(double4_t acc[4][4], const double* __restrict__ 
       double a[4], b[4];
       const int kr = kk + (lane >> 4);
 #pragma unroll
-      for (int m = 0; m < 4; ++m) a[m] = sign * As[kr * 128 + wr * 64 + m * 16 + (lane & 15)];
+      for (int m = 0; m < 4; ++m) a[m] = sign * As[kr * KS + wr * 64 + m * 16 + (lane & 15)];
 #pragma unroll
-      for (int n = 0; n < 4; ++n) b[n] = Bs[kr * 128 + wc * 64 + n * 16 + (lane & 15)];
+      for (int n = 0; n < 4; ++n) b[n] = Bs[kr * KS + wc * 64 + n * 16 + (lane & 15)];
 #pragma unroll
       for (int m = 0; m < 4; ++m)
 #pragma unroll
@@ -86,19 +112,26 @@ __device__ inline void tile_mma(double4_t acc[4][4], const double* __restrict__ 
   }
 }
 
-// Load a 128x128 tile into the MFMA accumulator layout.
-__device__ inline void tile_load(double4_t acc[4][4], const double* C, size_t ldc) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int wr = wave >> 1, wc = wave & 1;
+// The trailing-update tiles are computed transposed, acc = C^T: tile_mma
+// is given the column panel as A and the row panel as B, so element
+// (m, n, v) of lane l is C[r][c] with r = wc*64 + n*16 + l%16 and
+// c = wr*64 + m*16 + l/16 + 4v.  Every global access of C then reads or
+// writes 16 consecutive rows (128 bytes) of 4 columns.
+__device__ __attribute__((always_inline)) inline int acc_row(int n, int lane) {
+  return (((threadIdx.x >> 6) & 1) * 64) + n * 16 + (lane & 15);
+}
+__device__ __attribute__((always_inline)) inline int acc_col(int m, int v, int lane) {
+  return ((threadIdx.x >> 7) * 64) + m * 16 + (lane >> 4) + 4 * v;
+}
+
+__device__ inline void tile_load_t(double4_t acc[4][4], const double* C, size_t ldc) {
+  const int lane = threadIdx.x & 63;
 #pragma unroll
   for (int m = 0; m < 4; ++m)
 #pragma unroll
-    for (int n = 0; n < 4; ++n) {
-      const int col = wc * 64 + n * 16 + (lane & 15);
-      const int row = wr * 64 + m * 16 + (lane >> 4);
+    for (int n = 0; n < 4; ++n)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) acc[m][n][v] = C[(size_t)col * ldc + row + 4 * v];
-    }
+      for (int v = 0; v < 4; ++v) acc[m][n][v] = C[(size_t)acc_col(m, v, lane) * ldc + acc_row(n, lane)];
 }
 
 // Stores only entries that carry data: global column < D, and global row
@@ -106,43 +139,38 @@ __device__ inline void tile_load(double4_t acc[4][4], const double* C, size_t ld
 // rows of the rhs block keep their initial values, so a failed (NaN)
 // factorization cannot leak into storage the next assembly does not rewrite
 // (NaN * 0 is NaN).
-__device__ inline void tile_store(const double4_t acc[4][4], double* C, size_t ldc, bool lower_only,
-                                  uint32_t row0, uint32_t col0, uint32_t D, uint32_t Dp) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int wr = wave >> 1, wc = wave & 1;
+__device__ inline void tile_store_t(const double4_t acc[4][4], double* C, size_t ldc, bool lower_only,
+                                    uint32_t row0, uint32_t col0, uint32_t D, uint32_t Dp) {
+  const int lane = threadIdx.x & 63;
 #pragma unroll
   for (int m = 0; m < 4; ++m)
 #pragma unroll
-    for (int n = 0; n < 4; ++n) {
-      const int col = wc * 64 + n * 16 + (lane & 15);
-      const int row = wr * 64 + m * 16 + (lane >> 4);
+    for (int n = 0; n < 4; ++n)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const uint32_t g = row0 + row + 4 * v;
-        if ((!lower_only || row + 4 * v >= col) && (g < D || g == Dp) && col0 + col < D)
-          C[(size_t)col * ldc + row + 4 * v] = acc[m][n][v];
+        const int r = acc_row(n, lane), c = acc_col(m, v, lane);
+        const uint32_t g = row0 + r;
+        if ((!lower_only || r >= c) && (g < D || g == Dp) && col0 + c < D)
+          C[(size_t)c * ldc + r] = acc[m][n][v];
       }
-    }
 }
 
-// The accumulator tile into a column-major NB x NB LDS block, with the
-// identity on padding rows/columns (global index >= D).
+// The (transposed) accumulator of a diagonal tile into the tiled LDS block,
+// lower triangle, with the identity on padding rows/columns (global index
+// >= D).
 __device__ inline void tile_to_lds(const double4_t acc[4][4], double* L, uint32_t base, uint32_t D) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int wr = wave >> 1, wc = wave & 1;
+  const int lane = threadIdx.x & 63;
 #pragma unroll
   for (int m = 0; m < 4; ++m)
 #pragma unroll
-    for (int n = 0; n < 4; ++n) {
-      const int col = wc * 64 + n * 16 + (lane & 15);
-      const int row = wr * 64 + m * 16 + (lane >> 4);
+    for (int n = 0; n < 4; ++n)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int r = row + 4 * v;
-        const bool pad = base + r >= D || base + col >= D;
-        L[col * NB + r] = pad ? (r == col ? 1.0 : 0.0) : acc[m][n][v];
+        const int r = acc_row(n, lane), c = acc_col(m, v, lane);
+        if ((r >> 4) < (c >> 4)) continue;
+        const bool pad = base + r >= D || base + c >= D;
+        L[doff(r, c)] = pad ? (r == c ? 1.0 : 0.0) : acc[m][n][v];
       }
-    }
 }
 
 // ------------------------------------------------------- diagonal block
@@ -159,141 +187,172 @@ __device__ __attribute__((always_inline)) inline double rsqrt_f64(double d) {
   return y * (1.5 - 0.5 * d * y * y);
 }
 
-// Cholesky of a PW x PW block and its inverse, on wave 0 of the workgroup:
-// lane r < PW holds row r in registers; pivots and multipliers travel by
-// v_readlane, and each pivot costs one reciprocal square root (no divide),
-// which keeps the 16-step dependent chain short.  L11 is written back to A
-// (column-major, stride NB) and X11 = L11^{-1} (column-major, stride PW)
-// to Xd.  A pivot that is not > 0 (or NaN) on a real row sets flags[0].
-__device__ __attribute__((always_inline)) inline void wave_potrf16(double* A, int j0, double* Xd,
-                                                                    uint32_t base, uint32_t D,
-                                                                    int* flags) {
-  const int lane = threadIdx.x & 63;
-  const int r = lane & (PW - 1);
-  double row[PW];
-  double ipiv[PW];  // 1 / L[j][j]
+// Right-looking factorization of the PW-column panel j0 of the LDS block,
+// all rows j0 .. NB-1 at once, one row per lane on all four wavefronts:
+// lanes 0..15 of every wavefront hold the panel's diagonal tile (each
+// wavefront factors its own copy, so the multipliers never cross
+// wavefronts), lanes 16..63 of wavefront w hold rows j0 + 16 + 48 w + ...
+// The pivot travels by v_readlane, the column of multipliers through the
+// wavefront's 16-double LDS buffer (written by the owning lanes, read as a
+// broadcast: LDS operations of one wavefront complete in order), and each
+// pivot costs one reciprocal square root; 1/L[j][j] is kept after the
+// tiles.  A pivot that is not > 0 (or NaN) on a real row (base + j < D)
+// sets flags[0], nalgebra's Cholesky::new failure test.
+__device__ __attribute__((always_inline)) inline void panel_factor(double* A, int j0, uint32_t base,
+                                                                    uint32_t D, int* flags) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int r = lane < PW ? j0 + lane : j0 + PW + 48 * wave + (lane - PW);
+  const bool has = r < NB;
+  const bool writer = has && (lane >= PW || wave == 0);
+  // element (r, j0 + c) of the panel sits at the row's base + 16 c
+  const int o = has ? doff(r, j0) : 0;
+  double* colbuf = A + DIAG_COL + PW * wave;
+  double a[PW];
 #pragma unroll
-  for (int c = 0; c < PW; ++c) row[c] = (c <= r) ? A[(j0 + c) * NB + j0 + r] : 0.0;
+  for (int c = 0; c < PW; ++c) a[c] = has ? A[o + 16 * c] : 0.0;
+  double my_d = 0.0, my_ip = 0.0;  // lane j < PW: pivot j and 1/L[j][j]
+  double d = readlane_f64(a[0], 0);
 #pragma unroll
   for (int j = 0; j < PW; ++j) {
-    const double d = readlane_f64(row[j], j);
-    if (lane == 0 && !(d > 0.0) && base + j0 + j < D) atomicOr(&flags[0], 1);
     const double ip = rsqrt_f64(d);
-    ipiv[j] = ip;
-    row[j] = (r == j) ? d * ip : (r > j ? row[j] * ip : row[j]);
+    if (lane == j) {
+      my_d = d;
+      my_ip = ip;
+    }
+    a[j] *= ip;  // lane j holds the pivot itself: d * ip = L[j][j]
+    if (j + 1 < PW) {
+      // The next pivot, a[j+1] - L[j+1][j]^2 on lane j+1, needs no other
+      // lane; the multiplier L[j+1][j] reaches the other lanes by
+      // v_readlane and the rest of the column through the LDS buffer.
+      d = readlane_f64(a[j + 1] - a[j] * a[j], j + 1);
+      const double l1 = readlane_f64(a[j], j + 1);
+      a[j + 1] -= a[j] * l1;
+      if (j + 2 < PW) {
+        // compiler barriers keep each step's buffer accesses in program order
+        __asm__ volatile("" ::: "memory");
+        if (lane < PW) colbuf[lane] = a[j];
+        __asm__ volatile("" ::: "memory");
 #pragma unroll
-    for (int c = j + 1; c < PW; ++c) {
-      const double lcj = readlane_f64(row[j], c);
-      if (r >= c) row[c] -= row[j] * lcj;
+        for (int c = j + 2; c < PW; ++c) a[c] -= a[j] * colbuf[c];
+        __asm__ volatile("" ::: "memory");
+      }
     }
   }
-  if (lane < PW) {
-#pragma unroll
-    for (int c = 0; c < PW; ++c)
-      if (c <= r) A[(j0 + c) * NB + j0 + r] = row[c];
+  if (wave == 0 && lane < PW) {
+    if (!(my_d > 0.0) && base + j0 + lane < D) atomicOr(&flags[0], 1);
+    A[DIAG_IP + j0 + lane] = my_ip;
   }
-  // X11, lane c computes column c: x_i = (delta_ic - sum_{k<i} L[i][k] x_k) / L[i][i],
-  // L entries broadcast from the owning lane.
-  const int c = lane & (PW - 1);
-  double x[PW];
+  if (writer) {
 #pragma unroll
-  for (int i = 0; i < PW; ++i) {
-    double s = (i == c) ? 1.0 : 0.0;
-#pragma unroll
-    for (int k = 0; k < i; ++k) s -= readlane_f64(row[k], i) * x[k];
-    x[i] = s * ipiv[i];
-  }
-  if (lane < PW) {
-#pragma unroll
-    for (int i = 0; i < PW; ++i) Xd[c * PW + i] = x[i];
+    for (int c = 0; c < PW; ++c) A[o + 16 * c] = a[c];
   }
 }
 
-// Cholesky of the lower NB x NB block in LDS (column-major, A[c*NB + r]),
-// in PW-wide panels: the register factorization above, the panel below it
-// as L21 = A21 X11^T (independent loads, no dependent chain), and a
-// 4x4-register-tiled rank-PW update of the trailing lower triangle.  The
-// inverses X11 of the diagonal sub-blocks are kept in Xd (NB/PW blocks of
-// PW x PW) for the inverse pass.  A pivot that is not > 0 (or NaN) on a
-// real row (base + j < D) sets flags[0] (nalgebra's Cholesky::new failure
-// test).
-__device__ __attribute__((always_inline)) inline void lds_potrf(double* A, double* Xd, uint32_t base,
-                                                                 uint32_t D, int* flags) {
-  const int t = threadIdx.x, wave = t >> 6;
-  for (int j0 = 0; j0 < NB; j0 += PW) {
-    double* X11 = Xd + (j0 / PW) * PW * PW;
-    if (wave == 0) wave_potrf16(A, j0, X11, base, D, flags);
-    __syncthreads();
-    const int h = NB - j0 - PW;
-    if (h == 0) break;
-    // L21 = A21 X11^T: four threads per row, each producing outputs
-    // c = q, q+4, q+8, q+12 (x_c = sum_{k <= c} a_k X11[c][k]).
-    // The trip count is uniform across the workgroup (barrier inside).
-    for (int e0 = 0; e0 < 4 * h; e0 += blockDim.x) {
-      const int e = e0 + t;
-      const bool valid = e < 4 * h;
-      const int rr = j0 + PW + (valid ? (e >> 2) : 0), q = e & 3;
-      double o[4];
-      if (valid) {
-        double a[PW];
+// Rank-PW update of the trailing lower triangle after panel p, one 16x16
+// tile per wave at a time on v_mfma_f64_16x16x4f64, computed transposed
+// (D'[n][i] = C[r0+i][c0+n]) so that operands and accumulators are read and
+// written as 64 consecutive doubles of a tile.
+__device__ __attribute__((always_inline)) inline void lds_trailing_update(double* A, int p) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int T = NB / PW - 1 - p;
+  const int ntiles = T * (T + 1) / 2;
+  // two tiles per wave at a time, their MFMA chains interleaved
+  for (int e0 = wave; e0 < ntiles; e0 += 8) {
+    const int e1 = e0 + 4;
+    const bool two = e1 < ntiles;
+    double* C[2];
+    const double* Li[2];
+    const double* Lj[2];
 #pragma unroll
-        for (int k = 0; k < PW; ++k) a[k] = A[(j0 + k) * NB + rr];
-#pragma unroll
-        for (int m = 0; m < 4; ++m) {
-          const int cc = q + 4 * m;
-          double sum = 0.0;
-#pragma unroll
-          for (int k = 0; k < PW; ++k)
-            if (k <= cc) sum += a[k] * X11[k * PW + cc];
-          o[m] = sum;
-        }
-      }
-      __syncthreads();  // every thread of the row has read it before it is overwritten
-      if (valid) {
-#pragma unroll
-        for (int m = 0; m < 4; ++m) A[(j0 + q + 4 * m) * NB + rr] = o[m];
-      }
-    }
-    __syncthreads();
-    const int T = h / 4;
-    const int ntiles = T * (T + 1) / 2;
-    for (int e = t; e < ntiles; e += blockDim.x) {
-      int ti = (int)((sqrtf(8.0f * e + 1.0f) - 1.0f) * 0.5f);
-      while (ti * (ti + 1) / 2 > e) --ti;
+    for (int u = 0; u < 2; ++u) {
+      const int e = (u == 0 || !two) ? e0 : e1;
+      int ti = 0;
       while ((ti + 1) * (ti + 2) / 2 <= e) ++ti;
       const int tj = e - ti * (ti + 1) / 2;
-      const int r0 = j0 + PW + 4 * ti, c0 = j0 + PW + 4 * tj;
-      double acc[4][4];
-#pragma unroll
-      for (int p = 0; p < 4; ++p)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) acc[p][q] = A[(c0 + q) * NB + r0 + p];
-#pragma unroll 4
-      for (int k = 0; k < PW; ++k) {
-        double lr[4], lc[4];
-#pragma unroll
-        for (int p = 0; p < 4; ++p) {
-          lr[p] = A[(j0 + k) * NB + r0 + p];
-          lc[p] = A[(j0 + k) * NB + c0 + p];
-        }
-#pragma unroll
-        for (int p = 0; p < 4; ++p)
-#pragma unroll
-          for (int q = 0; q < 4; ++q) acc[p][q] -= lr[p] * lc[q];
-      }
-#pragma unroll
-      for (int p = 0; p < 4; ++p)
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-          if (ti != tj || p >= q) A[(c0 + q) * NB + r0 + p] = acc[p][q];
+      const int I = p + 1 + ti, J = p + 1 + tj;
+      C[u] = A + tile_index(I, J) * 256 + lane;
+      Li[u] = A + tile_index(I, p) * 256 + lane;
+      Lj[u] = A + tile_index(J, p) * 256 + lane;
     }
-    __syncthreads();
+    double4_t acc[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc[u][v] = C[u][64 * v];
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+      for (int u = 0; u < 2; ++u)
+        acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(-Lj[u][64 * s], Li[u][64 * s], acc[u], 0, 0, 0);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) C[0][64 * v] = acc[0][v];
+    if (two) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) C[1][64 * v] = acc[1][v];
+    }
   }
 }
 
-// In-place inverse of the lower triangle L held in LDS, block columns right
-// to left: A21 <- -(X22 A21) X11 with X22 the already inverted trailing
-// triangle and X11 kept by lds_potrf, then the diagonal block <- X11.
+// Factor the diagonal block k held in LDS (tiled, lower; padding rows and
+// columns hold the identity), store L (lower, real rows) into S, and write
+// the panel solve's operands for this block to Top (TOP_BLOCK doubles):
+//   block pq = p(p-1)/2 + q, q < p:  -L tile (p, q) (lane order = tile order:
+//                                    operand s, lane l -> -L[16p + l%16][16q + 4s + l/16])
+//   block 28 + p:                    X11_p = L_pp^{-1}, column-major 16x16
+//                                    (operand s, lane l -> X11_p[l%16][4s + l/16])
+__device__ __attribute__((always_inline)) inline void diag_factor(double* A, double* __restrict__ S,
+                                                                   size_t ld, int k, uint32_t D,
+                                                                   double* __restrict__ Top,
+                                                                   int* __restrict__ flags) {
+  const int t = threadIdx.x;
+  const uint32_t base = (uint32_t)k * NB;
+  double* T = Top + (size_t)k * TOP_BLOCK;
+  __syncthreads();
+  double* G = S + (size_t)k * NB * ld + (size_t)k * NB;
+#pragma unroll 1
+  for (int p = 0; p < NB / PW; ++p) {
+    panel_factor(A, PW * p, base, D, flags);
+    __syncthreads();
+    // Column p of tiles is final: its global stores (L into S, -L tiles
+    // into Top) drain while the trailing update runs.
+    for (int idx = t; idx < (NB / PW - p) * 256; idx += blockDim.x) {
+      const int ti = p + (idx >> 8), e = idx & 255;
+      const int rr = e & 15, cc = e >> 4;
+      const double v = A[tile_index(ti, p) * 256 + e];
+      if (ti > p) T[(ti * (ti - 1) / 2 + p) * 256 + e] = -v;
+      const int r = PW * ti + rr;
+      if ((ti > p || rr >= cc) && base + r < D) G[(size_t)(PW * p + cc) * ld + r] = v;
+    }
+    if (p + 1 < NB / PW) {
+      lds_trailing_update(A, p);
+      __syncthreads();
+    }
+  }
+  // X11_p by forward substitution, one thread per (p, column c), straight
+  // to Top.
+  if (t < 128) {
+    const int p = t >> 4, c = t & 15;
+    const double* Lpp = A + tile_index(p, p) * 256;
+    const double* ip = A + DIAG_IP + PW * p;
+    double x[PW];
+#pragma unroll
+    for (int i = 0; i < PW; ++i) {
+      double s = (i == c) ? 1.0 : 0.0;
+#pragma unroll
+      for (int m = 0; m < i; ++m) s -= Lpp[m * 16 + i] * x[m];
+      x[i] = s * ip[i];
+      __asm__ volatile("" ::: "memory");  // bounds the loads kept in flight
+    }
+    double* X = T + (28 + p) * 256 + c * 16;
+#pragma unroll
+    for (int i = 0; i < PW; ++i) X[i] = x[i];
+  }
+}
+
+// In-place inverse of the lower triangle L held in LDS (stride NB), block
+// columns right to left: A21 <- -(X22 A21) X11 with X22 the already
+// inverted trailing triangle and X11 the diagonal blocks' inverses (Xd,
+// column-major PW x PW each), then the diagonal block <- X11.
 // tmp holds (NB - PW) x PW doubles.
 __device__ __attribute__((always_inline)) inline void lds_trtri(double* A, const double* Xd, double* tmp) {
   const int t = threadIdx.x;
@@ -359,78 +418,74 @@ __device__ __attribute__((always_inline)) inline void lds_trtri(double* A, const
   }
 }
 
-// Factor the diagonal block held in LDS, store L (real rows) into S and
-// L^{-1} (identity on padding) into Linv.
-__device__ __attribute__((always_inline)) inline void factor_diag_block(double* smem, double* __restrict__ S,
-                                                                         size_t ld, int k, uint32_t D,
-                                                                         double* __restrict__ Linv,
-                                                                         int* __restrict__ flags) {
-  const int t = threadIdx.x;
-  const uint32_t base = (uint32_t)k * NB;
-  double* A = smem;
-  double* Xd = smem + NB * NB;
-  double* tmp = Xd + NB * PW;
-  __syncthreads();
-  lds_potrf(A, Xd, base, D, flags);
-  double* G = S + (size_t)k * NB * ld + (size_t)k * NB;
-  for (int idx = t; idx < NB * NB; idx += blockDim.x) {
-    const int c = idx >> 7, r = idx & 127;
-    if (r >= c && base + r < D) G[(size_t)c * ld + r] = A[c * NB + r];
-  }
-  __syncthreads();
-  lds_trtri(A, Xd, tmp);
-  // Linv holds, per block, L^{-1} (column-major) followed by its transpose.
-  double* X = Linv + (size_t)k * 2 * NB * NB;
-  for (int idx = t; idx < NB * NB; idx += blockDim.x) {
-    const int c = idx >> 7, r = idx & 127;
-    const bool pad = base + r >= D || base + c >= D;
-    X[idx] = pad ? (r == c ? 1.0 : 0.0) : (r >= c ? A[c * NB + r] : 0.0);
-    const bool padT = base + c >= D || base + r >= D;
-    X[NB * NB + idx] = padT ? (r == c ? 1.0 : 0.0) : (c >= r ? A[r * NB + c] : 0.0);
-  }
-}
-
 __global__ __launch_bounds__(256) void k_potrf_block(double* __restrict__ S, size_t ld, int k,
-                                                     uint32_t D, double* __restrict__ Linv,
+                                                     uint32_t D, double* __restrict__ Top,
                                                      int* __restrict__ flags) {
-  __shared__ double smem[SMEM_DOUBLES];
+  __shared__ double smem[SMEM_DIAG];
   const uint32_t base = (uint32_t)k * NB;
   const double* G = S + (size_t)k * NB * ld + (size_t)k * NB;
   for (int idx = threadIdx.x; idx < NB * NB; idx += blockDim.x) {
     const int c = idx >> 7, r = idx & 127;
+    if ((r >> 4) < (c >> 4)) continue;
     const bool pad = base + r >= D || base + c >= D;
-    smem[c * NB + r] = pad ? (r == c ? 1.0 : 0.0) : (r >= c ? G[(size_t)c * ld + r] : 0.0);
+    smem[doff(r, c)] = pad ? (r == c ? 1.0 : 0.0) : (r >= c ? G[(size_t)c * ld + r] : 0.0);
   }
-  factor_diag_block(smem, S, ld, k, D, Linv, flags);
+  diag_factor(smem, S, ld, k, D, Top, flags);
 }
 
-// L_ik = A_ik Linv_k^T for row blocks i = k+1 .. nblk (nblk: the rhs block).
-__global__ __launch_bounds__(256) void k_trsm_panel(double* __restrict__ S, size_t ld, int k,
-                                                    const double* __restrict__ Linv, uint32_t D,
-                                                    uint32_t Dp) {
-  __shared__ double As[KC * 128];
-  __shared__ double Bs[KC * 128];
-  const int i = k + 1 + blockIdx.x;
-  double* A = S + (size_t)k * NB * ld + (size_t)i * NB;
-  const double* X = Linv + (size_t)k * 2 * NB * NB;
-  double4_t acc[4][4];
+// L_ik = A_ik L_kk^{-T} for row blocks i = k+1 .. nblk-1 and the rhs row
+// (block nblk, first 16 rows), one wavefront per 16-row strip, entirely in
+// registers: for each 16-column sub-panel p,
+//   R_p = A_p - sum_{q<p} X_q L_pq^T,   X_p = R_p X11_p^T,
+// computed transposed on v_mfma_f64_16x16x4f64, whose accumulator layout
+// for X_p^T is exactly the B-operand layout X_p^T needs in the later
+// sub-panels (lane l, register v: X_p[l%16][l/16 + 4v]).  The L_pq and
+// X11_p operands come from Top in lane order (one coalesced 512-byte load
+// per operand).
+__global__ __launch_bounds__(64) void k_trsm_panel(double* __restrict__ S, size_t ld, int k,
+                                                   const double* __restrict__ Top, uint32_t D,
+                                                   uint32_t Dp) {
+  const int lane = threadIdx.x;
+  const int i = k + 1 + (int)(blockIdx.x >> 3), rt = blockIdx.x & 7;
+  const uint32_t row0 = (uint32_t)i * NB + 16 * rt;
+  const uint32_t grow = row0 + (lane & 15);
+  const bool row_ok = grow < D || grow == Dp;
+  const double* T = Top + (size_t)k * TOP_BLOCK + lane;
+  double* Acol = S + (size_t)k * NB * ld + grow;
+  const int lk = lane >> 4;
+  double xb[8][4];
 #pragma unroll
-  for (int m = 0; m < 4; ++m)
+  for (int p = 0; p < 8; ++p) {
+    double4_t r;
 #pragma unroll
-    for (int n = 0; n < 4; ++n) acc[m][n] = double4_t{0.0, 0.0, 0.0, 0.0};
-  tile_mma(acc, A, ld, X, NB, NB, 1.0, As, Bs);
-  __syncthreads();
-  tile_store(acc, A, ld, false, (uint32_t)i * NB, (uint32_t)k * NB, D, Dp);
+    for (int v = 0; v < 4; ++v) r[v] = Acol[(size_t)(PW * p + lk + 4 * v) * ld];
+#pragma unroll
+    for (int q = 0; q < p; ++q)
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+        r = __builtin_amdgcn_mfma_f64_16x16x4f64(T[((p * (p - 1) / 2 + q) * 4 + s) * 64], xb[q][s], r,
+                                                  0, 0, 0);
+    double4_t x = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+      x = __builtin_amdgcn_mfma_f64_16x16x4f64(T[((28 + p) * 4 + s) * 64], r[s], x, 0, 0, 0);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      xb[p][v] = x[v];
+      const uint32_t col = (uint32_t)k * NB + PW * p + lk + 4 * v;
+      if (row_ok && col < D) Acol[(size_t)(PW * p + lk + 4 * v) * ld] = x[v];
+    }
+  }
 }
 
 // A_ij -= L_ik L_jk^T for k < j <= i, j < nblk, i <= nblk (i == nblk: rhs
 // block).  The workgroup of the next diagonal tile (k+1, k+1) factors it
 // right away (lookahead), so the next step's panel solve can start.
-__global__ __launch_bounds__(256) void k_syrk_update(double* __restrict__ S, size_t ld, int k,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_syrk_update(double* __restrict__ S, size_t ld, int k,
                                                      int nblk, uint32_t D, uint32_t Dp,
-                                                     double* __restrict__ Linv,
+                                                     double* __restrict__ Top,
                                                      int* __restrict__ flags) {
-  __shared__ double smem[SMEM_DOUBLES];
+  __shared__ double smem[SMEM_DIAG];
   // Workgroup (0, 0) is the next diagonal tile (k+1, k+1): dispatched first,
   // so its factorization overlaps the other tiles' updates.
   const int j = k + 1 + blockIdx.x;
@@ -440,15 +495,48 @@ __global__ __launch_bounds__(256) void k_syrk_update(double* __restrict__ S, siz
   const double* Ai = S + (size_t)k * NB * ld + (size_t)i * NB;
   const double* Aj = S + (size_t)k * NB * ld + (size_t)j * NB;
   double4_t acc[4][4];
-  tile_load(acc, C, ld);
-  tile_mma(acc, Ai, ld, Aj, ld, NB, -1.0, smem, smem + KC * 128);
+  tile_load_t(acc, C, ld);
+  tile_mma(acc, Aj, ld, Ai, ld, NB, -1.0, smem, smem + KC * KS);
   if (i == k + 1 && j == k + 1) {
     __syncthreads();
     tile_to_lds(acc, smem, (uint32_t)j * NB, D);
-    factor_diag_block(smem, S, ld, k + 1, D, Linv, flags);
+    diag_factor(smem, S, ld, k + 1, D, Top, flags);
     return;
   }
-  tile_store(acc, C, ld, i == j, (uint32_t)i * NB, (uint32_t)j * NB, D, Dp);
+  tile_store_t(acc, C, ld, i == j, (uint32_t)i * NB, (uint32_t)j * NB, D, Dp);
+}
+
+// L^{-1} of every diagonal block, all blocks at once after the
+// factorization (only the triangular solves use it): Linv holds, per block,
+// L^{-1} (column-major, identity on padding) followed by its transpose.
+__global__ __launch_bounds__(256) void k_block_inverse(const double* __restrict__ S, size_t ld,
+                                                       uint32_t D, const double* __restrict__ Top,
+                                                       double* __restrict__ Linv) {
+  __shared__ double smem[SMEM_INV];
+  const int t = threadIdx.x;
+  const int k = blockIdx.x;
+  const uint32_t base = (uint32_t)k * NB;
+  double* A = smem;
+  double* Xd = smem + NB * NB;
+  double* tmp = Xd + NB * PW;
+  const double* G = S + (size_t)k * NB * ld + (size_t)k * NB;
+  for (int idx = t; idx < NB * NB; idx += blockDim.x) {
+    const int c = idx >> 7, r = idx & 127;
+    const bool pad = base + r >= D || base + c >= D;
+    A[c * NB + r] = pad ? (r == c ? 1.0 : 0.0) : (r >= c ? G[(size_t)c * ld + r] : 0.0);
+  }
+  // X11_p (column-major 16x16) from the panel-solve operands.
+  const double* T = Top + (size_t)k * TOP_BLOCK + 28 * 256;
+  for (int idx = t; idx < NB * PW; idx += blockDim.x) Xd[idx] = T[idx];
+  __syncthreads();
+  lds_trtri(A, Xd, tmp);
+  double* X = Linv + (size_t)k * LINV_BLOCK;
+  for (int idx = t; idx < NB * NB; idx += blockDim.x) {
+    const int c = idx >> 7, r = idx & 127;
+    const bool pad = base + r >= D || base + c >= D;
+    X[idx] = pad ? (r == c ? 1.0 : 0.0) : (r >= c ? A[c * NB + r] : 0.0);
+    X[NB * NB + idx] = pad ? (r == c ? 1.0 : 0.0) : (c >= r ? A[r * NB + c] : 0.0);
+  }
 }
 
 // ------------------------------------------------------ triangular solves
@@ -464,7 +552,7 @@ __global__ __launch_bounds__(128) void k_backsolve_block(const double* __restric
   __syncthreads();
   // x[t] = sum_r Linv[r][t] y[r] = sum_r LinvT[t][r] y[r]; LinvT column-major
   // makes LinvT[t][r] = XT[r*NB + t] contiguous across threads.
-  const double* XT = Linv + (size_t)k * 2 * NB * NB + NB * NB;
+  const double* XT = Linv + (size_t)k * LINV_BLOCK + NB * NB;
   double s = 0.0;
 #pragma unroll 8
   for (int r = 0; r < NB; ++r) s += XT[(size_t)r * NB + t] * ys[r];
@@ -497,7 +585,7 @@ __global__ __launch_bounds__(128) void k_forward_block(const double* __restrict_
   const int t = threadIdx.x;
   bs[t] = b[(size_t)k * NB + t];
   __syncthreads();
-  const double* X = Linv + (size_t)k * 2 * NB * NB;
+  const double* X = Linv + (size_t)k * LINV_BLOCK;
   double s = 0.0;
 #pragma unroll 8
   for (int c = 0; c < NB; ++c) s += X[(size_t)c * NB + t] * bs[c];
@@ -651,15 +739,18 @@ __global__ __launch_bounds__(1024) void k_lu_solve(const double* __restrict__ A,
 void dense_cholesky(double* S, double* Linv, uint32_t Dp, uint32_t ld, uint32_t D, int* flags,
                     hipStream_t s) {
   const int nblk = (int)(Dp / NB);
-  hipLaunchKernelGGL(k_potrf_block, dim3(1), dim3(256), 0, s, S, (size_t)ld, 0, D, Linv, flags);
+  double* Top = Linv + (size_t)nblk * LINV_BLOCK;
+  hipLaunchKernelGGL(k_potrf_block, dim3(1), dim3(256), 0, s, S, (size_t)ld, 0, D, Top, flags);
   for (int k = 0; k < nblk; ++k) {
-    const int nbelow = nblk - k;  // row blocks k+1 .. nblk (rhs block included)
-    hipLaunchKernelGGL(k_trsm_panel, dim3(nbelow), dim3(256), 0, s, S, (size_t)ld, k, Linv, D, Dp);
+    // 8 strips per row block k+1 .. nblk-1, one for the rhs row
+    const int strips = 8 * (nblk - 1 - k) + 1;
+    hipLaunchKernelGGL(k_trsm_panel, dim3(strips), dim3(64), 0, s, S, (size_t)ld, k, Top, D, Dp);
     const int ntrail = nblk - 1 - k;
     if (ntrail > 0)
       hipLaunchKernelGGL(k_syrk_update, dim3(ntrail, ntrail + 1), dim3(256), 0, s, S, (size_t)ld, k,
-                         nblk, D, Dp, Linv, flags);
+                         nblk, D, Dp, Top, flags);
   }
+  hipLaunchKernelGGL(k_block_inverse, dim3(nblk), dim3(256), 0, s, S, (size_t)ld, D, Top, Linv);
   HIP_CHECK(hipGetLastError());
 }
 

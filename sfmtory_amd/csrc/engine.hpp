@@ -30,6 +30,14 @@ namespace sfmba {
 
 constexpr uint32_t NONE = 0xFFFFFFFFu;
 constexpr int NB = 128;           // dense factorization block
+// Dense factor workspace per NB block: L^{-1} and its transpose (2 NB x NB),
+// and, after all blocks, the panel-solve operands (36 16x16x4 MFMA operand
+// sets of 4 x 64 doubles per block).
+constexpr size_t LINV_BLOCK = 2 * (size_t)NB * NB;
+constexpr size_t TOP_BLOCK = 36 * 4 * 64;
+inline size_t dense_workspace_doubles(uint32_t Dp) {
+  return (size_t)(Dp / NB) * (LINV_BLOCK + TOP_BLOCK);
+}
 constexpr int PAIR_CHUNK = 128;   // max point pairs reduced by one wavefront
 
 struct Error : std::runtime_error {
@@ -117,7 +125,7 @@ struct Work {
   double* dpt = nullptr;      // 3 per point
   double* S = nullptr;        // ld x Dp
   double* S_copy = nullptr;   // refinement copy (mixed precision)
-  double* Linv = nullptr;     // per NB block: L^{-1} then its transpose (2 NB x NB)
+  double* Linv = nullptr;     // dense factor workspace (dense_workspace_doubles)
   double* delta = nullptr;    // Dp
   double* vec = nullptr;      // Dp scratch
   double* rhs = nullptr;      // Dp

@@ -502,7 +502,7 @@ static void alloc_work(sfmba_problem& H, Work<T>& W) {
   W.pt_ok = A.alloc<uint8_t>(L.n_pt);
   const size_t s_elems = (size_t)L.ld * L.Dp;
   W.S = A.alloc<double>(s_elems);
-  W.Linv = A.alloc<double>((size_t)L.Dp * NB * 2);
+  W.Linv = A.alloc<double>(dense_workspace_doubles(L.Dp));
   W.delta = A.alloc<double>(L.Dp);
   W.vec = A.alloc<double>(L.Dp);
   W.rhs = A.alloc<double>(L.Dp);
@@ -947,7 +947,7 @@ int sfmba_dense_solve(const double* A, size_t n, const double* b, int32_t method
     }
     for (uint32_t d = D; d < Dp; ++d) h[(size_t)d * ld + d] = 1.0;
     double* S = ar.upload(h, sg.s);
-    double* Linv = ar.alloc<double>((size_t)Dp * NB * 2);
+    double* Linv = ar.alloc<double>(dense_workspace_doubles(Dp));
     double* y = ar.alloc<double>(Dp);
     double* xd = ar.alloc<double>(Dp);
     int* flags = ar.alloc<int>(4);
