@@ -73,6 +73,7 @@ struct Layout {
   uint32_t D = 0, Dp = 0, ld = 0;
   uint64_t n_pairs = 0;
   uint32_t n_chunks = 0, n_dests = 0, n_multi = 0, n_parts = 0;
+  uint32_t max_grp_dim = 0;
   // observations (point-major)
   uint32_t* obs_img = nullptr;
   uint32_t* obs_pt = nullptr;
@@ -83,7 +84,8 @@ struct Layout {
   uint32_t* inc_grp = nullptr;
   uint32_t* inc_pt = nullptr;
   uint32_t* inc_obs = nullptr;
-  uint32_t* inc_roff = nullptr;  // first row
+  uint32_t* inc_roff = nullptr;  // first row (point-major J, E)
+  uint32_t* inc_froff = nullptr; // first row of F (group-major)
   uint32_t* inc_row0 = nullptr;  // first entry of the observation's side vector
   uint32_t* pt_inc_ptr = nullptr;  // incidences of each point (CSR, n_pt + 1)
   // cameras and images
@@ -96,8 +98,8 @@ struct Layout {
   uint32_t* grp_off = nullptr;
   uint32_t* grp_dim = nullptr;
   // Schur pair lists (destination-sorted)
-  uint32_t* pairs = nullptr;      // 2 per pair: first rows of incidences a, b (bit 31 flags)
-  uint4* chunks = nullptr;        // (dest, begin, end, part or NONE)
+  uint32_t* pairs = nullptr;      // 2 per pair: a's first F row, b's first row (flag bits)
+  uint4* chunks = nullptr;        // ((ga << 16) | gb, begin, end, part or NONE)
   uint32_t* multi_dest = nullptr; // destinations reduced from partials
   uint32_t* dest_part0 = nullptr; // first partial of each multi destination
   uint32_t* dest_nparts = nullptr;

@@ -107,7 +107,7 @@ struct Pack {
   uint32_t D = 0, Dp = 0, ld = 0, n_grp = 0, n_rows = 0;
   std::vector<uint32_t> obs_img, obs_pt, obs_inc, pt_ptr;
   std::vector<double2> obs_xy;
-  std::vector<uint32_t> inc_grp, inc_pt, inc_obs, inc_roff, inc_row0, pt_inc_ptr;
+  std::vector<uint32_t> inc_grp, inc_pt, inc_obs, inc_roff, inc_froff, inc_row0, pt_inc_ptr;
   std::vector<uint32_t> cam_of_img, img_slot, cam_slot, cam_nfree, grp_off, grp_dim;
   std::vector<int32_t> cam_model;
   std::vector<uint8_t> cam_free;
@@ -251,8 +251,28 @@ static void build_pack(const sfmba_input* in, Pack& P, bool need_pairs) {
   }
   for (uint32_t p = 0; p < P.n_pt; ++p) P.pt_inc_ptr[p + 1] = std::max(P.pt_inc_ptr[p + 1], P.pt_inc_ptr[p]);
   P.n_rows = rows;
-  if (rows >= 0x80000000u) throw Error(SFMBA_ERR_UNSUPPORTED, "too many Jacobian rows for 31-bit offsets");
+  if (rows >= 0x40000000u) throw Error(SFMBA_ERR_UNSUPPORTED, "too many Jacobian rows for 30-bit offsets");
+  // F rows (the Schur assembly's row-side operand) are stored group-major:
+  // each group's incidences contiguous, in point order, so the destination
+  // blocks of one block row read a compact range that stays in L2.
+  {
+    std::vector<uint32_t> gstart(P.n_grp, 0);
+    for (uint32_t g : P.inc_grp) gstart[g] += P.grp_dim[g];
+    uint32_t acc = 0;
+    for (uint32_t g = 0; g < P.n_grp; ++g) {
+      const uint32_t n = gstart[g];
+      gstart[g] = acc;
+      acc += n;
+    }
+    P.inc_froff.resize(P.inc_grp.size());
+    for (size_t a = 0; a < P.inc_grp.size(); ++a) {
+      const uint32_t g = P.inc_grp[a];
+      P.inc_froff[a] = gstart[g];
+      gstart[g] += P.grp_dim[g];
+    }
+  }
   if (!need_pairs || D == 0) return;
+  if (P.n_grp > 0xFFFFu) throw Error(SFMBA_ERR_UNSUPPORTED, "more than 65535 parameter blocks");
 
   // Destination-sorted point pairs.  Every lower-triangular block (ga >= gb)
   // is a destination, so each try rewrites all of S.
@@ -282,25 +302,36 @@ static void build_pack(const sfmba_input* in, Pack& P, bool need_pairs) {
           const uint64_t ga = P.inc_grp[a], gb = P.inc_grp[b];
           if (ga < gb) continue;
           const uint32_t q = pos[ga * (ga + 1) / 2 + gb]++;
-          // first rows of the two incidences; bit 31: same observation / a == b
-          P.pairs[2 * (size_t)q] = P.inc_roff[a] | (P.inc_obs[a] == P.inc_obs[b] ? 0x80000000u : 0u);
+          // First word: a's first F row (group-major), bit 31 "same
+          // observation", bit 30 "a's rows precede b's" (the two incidences
+          // of one observation are adjacent in the point-major layout).
+          // Second word: b's first row (point-major), bit 31 "a == b".
+          const bool same = P.inc_obs[a] == P.inc_obs[b];
+          const bool before = same && a != b && P.inc_roff[a] < P.inc_roff[b];
+          P.pairs[2 * (size_t)q] = P.inc_froff[a] | (same ? 0x80000000u : 0u) | (before ? 0x40000000u : 0u);
           P.pairs[2 * (size_t)q + 1] = P.inc_roff[b] | (a == b ? 0x80000000u : 0u);
         }
     }
   }
-  for (uint64_t d = 0; d < n_dests; ++d) {
+  // Chunks name their destination as (ga << 16) | gb.
+  for (uint64_t d = 0, ga = 0, gb = 0; d < n_dests; ++d) {
+    const uint32_t code = (uint32_t)((ga << 16) | gb);
+    if (++gb > ga) {
+      ++ga;
+      gb = 0;
+    }
     const uint32_t b0 = cnt[d], b1 = cnt[d + 1];
     const uint32_t nch = std::max<uint32_t>(1, (b1 - b0 + PAIR_CHUNK - 1) / PAIR_CHUNK);
     if (nch == 1) {
-      P.chunks.push_back(uint4{(uint32_t)d, b0, b1, NONE});
+      P.chunks.push_back(uint4{code, b0, b1, NONE});
       continue;
     }
-    P.multi_dest.push_back((uint32_t)d);
+    P.multi_dest.push_back(code);
     P.dest_part0.push_back(P.n_parts);
     P.dest_nparts.push_back(nch);
     for (uint32_t k = 0; k < nch; ++k) {
       const uint32_t lo = b0 + k * PAIR_CHUNK, hi = std::min(b1, lo + PAIR_CHUNK);
-      P.chunks.push_back(uint4{(uint32_t)d, lo, hi, P.n_parts++});
+      P.chunks.push_back(uint4{code, lo, hi, P.n_parts++});
     }
   }
 }
@@ -327,6 +358,7 @@ static void upload_layout(const Pack& P, Problem& pr, hipStream_t s) {
   L.n_chunks = (uint32_t)P.chunks.size();
   L.n_multi = (uint32_t)P.multi_dest.size();
   L.n_parts = P.n_parts;
+  for (uint32_t g : P.grp_dim) L.max_grp_dim = std::max(L.max_grp_dim, g);
   L.obs_img = A.upload(P.obs_img, s);
   L.obs_pt = A.upload(P.obs_pt, s);
   L.obs_xy = A.upload(P.obs_xy, s);
@@ -336,6 +368,7 @@ static void upload_layout(const Pack& P, Problem& pr, hipStream_t s) {
   L.inc_pt = A.upload(P.inc_pt, s);
   L.inc_obs = A.upload(P.inc_obs, s);
   L.inc_roff = A.upload(P.inc_roff, s);
+  L.inc_froff = A.upload(P.inc_froff, s);
   L.inc_row0 = A.upload(P.inc_row0, s);
   L.pt_inc_ptr = A.upload(P.pt_inc_ptr, s);
   L.cam_of_img = A.upload(P.cam_of_img, s);

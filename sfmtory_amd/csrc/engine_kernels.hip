@@ -237,6 +237,7 @@ __global__ __launch_bounds__(256) void k_point_prepare(uint32_t n_pt, const doub
 template <typename T>
 __global__ __launch_bounds__(256) void k_compute_F(uint32_t n_inc, const uint32_t* __restrict__ inc_pt,
                                                    const uint32_t* __restrict__ inc_roff,
+                                                   const uint32_t* __restrict__ inc_froff,
                                                    const uint32_t* __restrict__ inc_grp,
                                                    const uint32_t* __restrict__ grp_dim,
                                                    const double* __restrict__ pt_cinv,
@@ -252,14 +253,14 @@ __global__ __launch_bounds__(256) void k_compute_F(uint32_t n_inc, const uint32_
 #pragma unroll
   for (int k = 0; k < 9; ++k) c[k] = (T)ci[k];
   const T g0 = (T)pt_g[3 * (size_t)pt], g1 = (T)pt_g[3 * (size_t)pt + 1], g2 = (T)pt_g[3 * (size_t)pt + 2];
-  const uint32_t r0 = inc_roff[a], n = grp_dim[inc_grp[a]];
+  const uint32_t r0 = inc_roff[a], f0ff = inc_froff[a], n = grp_dim[inc_grp[a]];
   for (uint32_t r = 0; r < n; ++r) {
     const V4 e = E4[r0 + r];
     const T f0 = e.x * c[0] + e.y * c[3] + e.z * c[6];
     const T f1 = e.x * c[1] + e.y * c[4] + e.z * c[7];
     const T f2 = e.x * c[2] + e.y * c[5] + e.z * c[8];
     // w: this row's share of rhs, F . g_point = E Cinv g_point
-    F4[r0 + r] = V4{f0, f1, f2, f0 * g0 + f1 * g1 + f2 * g2};
+    F4[f0ff + r] = V4{f0, f1, f2, f0 * g0 + f1 * g1 + f2 * g2};
   }
 }
 
@@ -282,27 +283,22 @@ struct Acc<double> {
   __device__ inline double value() const { return s; }
 };
 
-__device__ inline void decode_dest(uint32_t d, uint32_t& ga, uint32_t& gb) {
-  uint32_t a = (uint32_t)((sqrt(8.0 * (double)d + 1.0) - 1.0) * 0.5);
-  while ((uint64_t)a * (a + 1) / 2 > d) --a;
-  while ((uint64_t)(a + 1) * (a + 2) / 2 <= d) ++a;
-  ga = a;
-  gb = d - (uint32_t)((uint64_t)a * (a + 1) / 2);
-}
-
 constexpr int PART_STRIDE = 2 * 196 + 16;
 
 // One wavefront per chunk of a destination block (ga >= gb) of the reduced
-// system.  Sums, over the chunk's point pairs (a, b), B = w J_a^T J_b for
+// system; lane l owns entries l, l + 64, ... (NQ of them) of the da x db
+// block.  Sums, over the chunk's point pairs (a, b), B = w J_a^T J_b for
 // pairs of the same observation and Schur = F_a E_b^T = E_a Cinv E_b^T; on
 // diagonal blocks also the right-hand side F_a g_point - w J_a^T r over
-// pairs a == b (F4.w - E4.w).  Each pair is one 8-byte word pair: the two
-// incidences' first rows, with bit 31 flagging "same observation" (first
-// word) and "a == b" (second word); four pairs are processed per step with
-// their loads issued together.  Single-chunk destinations are written
-// straight into S (with the Marquardt damping of the diagonal); others
-// leave partials.
-template <typename T>
+// pairs a == b (F4.w - E4.w).  A pair is two words: a's first F row (F is
+// group-major) with bit 31 "same observation" and bit 30 "a's rows precede
+// b's", and b's first row with bit 31 "a == b".  U pairs are processed per
+// step with all their loads issued together.  Chunks run in destination
+// order, so the workgroups in flight on all XCDs share one block row's F
+// rows (a compact group-major range, held by L2 and the MALL).  Single-chunk
+// destinations are written straight into S (with the Marquardt damping of
+// the diagonal); others leave partials.
+template <typename T, int NQ>
 __global__ __launch_bounds__(256) void k_schur(
     uint32_t n_chunks, const uint4* __restrict__ chunks, const uint2* __restrict__ pairs,
     const uint32_t* __restrict__ grp_off, const uint32_t* __restrict__ grp_dim,
@@ -310,22 +306,22 @@ __global__ __launch_bounds__(256) void k_schur(
     const typename Vec4<T>::type* __restrict__ F4, double lambda, double* __restrict__ S, uint32_t ld,
     uint32_t Dp, double* __restrict__ parts) {
   using V4 = typename Vec4<T>::type;
-  constexpr uint32_t HI = 0x80000000u, LO = 0x7FFFFFFFu;
-  constexpr int U = 4;
+  constexpr uint32_t HI = 0x80000000u, BEFORE = 0x40000000u, OFF = 0x3FFFFFFFu;
+  constexpr int UMAX = sizeof(T) == 4 ? 4 : 2;  // measured best on gfx950
+  constexpr int U = UMAX / NQ > 1 ? UMAX / NQ : 1;
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t ci = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
   if (ci >= n_chunks) return;
   const uint4 ch = chunks[ci];
-  uint32_t ga, gb;
-  decode_dest(ch.x, ga, gb);
+  const uint32_t ga = ch.x >> 16, gb = ch.x & 0xFFFFu;
   const uint32_t da = grp_dim[ga], db = grp_dim[gb];
   const uint32_t ne = da * db;
   const bool diag = ga == gb;
-  Acc<T> accS[4], accB[4], accR;
-  uint32_t er[4], ec[4];
-  bool on[4];
+  Acc<T> accS[NQ], accB[NQ], accR;
+  uint32_t er[NQ], ec[NQ];
+  bool on[NQ];
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
+  for (int q = 0; q < NQ; ++q) {
     const uint32_t e = lane + 64 * q;
     on[q] = e < ne;
     er[q] = on[q] ? e % da : 0;
@@ -334,33 +330,51 @@ __global__ __launch_bounds__(256) void k_schur(
   const bool rlane = diag && lane < da;
   for (uint32_t p = ch.y; p < ch.z; p += U) {
     uint2 pr[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) pr[u] = (p + u < ch.z) ? pairs[p + u] : make_uint2(HI, 0u);
+    bool ok[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      if (p + u >= ch.z) continue;
-      const uint32_t ra = pr[u].x & LO, rb = pr[u].y & LO;
-      const bool same = (pr[u].x & HI) != 0u;
+      ok[u] = p + u < ch.z;
+      pr[u] = ok[u] ? pairs[p + u] : make_uint2(0u, 0u);
+    }
+    V4 f[U][NQ], e[U][NQ];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        if (on[q]) {
-          const V4 f = F4[ra + er[q]];
-          const V4 e = E4[rb + ec[q]];
-          accS[q].add(f.x * e.x + f.y * e.y + f.z * e.z);
-          if (same) {
-            const V4 ja = J4[ra + er[q]];
+    for (int u = 0; u < U; ++u) {
+      const uint32_t ra = pr[u].x & OFF, rb = pr[u].y & OFF;
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        if (ok[u] && on[q]) {
+          f[u][q] = F4[ra + er[q]];
+          e[u][q] = E4[rb + ec[q]];
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (!ok[u]) continue;
+      const uint32_t ra = pr[u].x & OFF, rb = pr[u].y & OFF;
+#pragma unroll
+      for (int q = 0; q < NQ; ++q)
+        if (on[q]) accS[q].add(f[u][q].x * e[u][q].x + f[u][q].y * e[u][q].y + f[u][q].z * e[u][q].z);
+      if (pr[u].x & HI) {
+        // same observation: a's point-major first row from b's
+        const bool self = (pr[u].y & HI) != 0u;
+        const uint32_t raj = self ? rb : ((pr[u].x & BEFORE) ? rb - da : rb + db);
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+          if (on[q]) {
+            const V4 ja = J4[raj + er[q]];
             const V4 jb = J4[rb + ec[q]];
             accB[q].add(ja.z * jb.x + ja.w * jb.y);
           }
         }
       }
-      if (rlane && (pr[u].y & HI)) accR.add(F4[ra + lane].w - E4[ra + lane].w);
+      if (rlane && (pr[u].y & HI)) accR.add(F4[ra + lane].w - E4[rb + lane].w);
     }
   }
   const uint32_t oa = grp_off[ga], ob = grp_off[gb];
   if (ch.w == NONE) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < NQ; ++q) {
       if (on[q]) {
         const uint32_t r = er[q], c = ec[q];
         if (diag && r < c) continue;
@@ -374,7 +388,7 @@ __global__ __launch_bounds__(256) void k_schur(
   } else {
     double* pp = parts + (size_t)ch.w * PART_STRIDE;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < NQ; ++q) {
       if (on[q]) {
         pp[lane + 64 * q] = accB[q].value();
         pp[196 + lane + 64 * q] = accS[q].value();
@@ -394,8 +408,7 @@ __global__ __launch_bounds__(256) void k_schur_reduce(
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t m = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
   if (m >= n_multi) return;
-  uint32_t ga, gb;
-  decode_dest(multi_dest[m], ga, gb);
+  const uint32_t ga = multi_dest[m] >> 16, gb = multi_dest[m] & 0xFFFFu;
   const uint32_t da = grp_dim[ga], db = grp_dim[gb], ne = da * db;
   const bool diag = ga == gb;
   const uint32_t p0 = dest_part0[m], np = dest_nparts[m];
@@ -672,7 +685,7 @@ void launch_point_prepare(const Layout& L, Work<T>& W, double lambda, hipStream_
   HIP_CHECK(hipGetLastError());
   if (L.n_inc == 0) return;
   hipLaunchKernelGGL(k_compute_F<T>, dim3(blocks_for(L.n_inc, 256)), dim3(256), 0, s, L.n_inc,
-                     L.inc_pt, L.inc_roff, L.inc_grp, L.grp_dim, W.pt_cinv, W.pt_g,
+                     L.inc_pt, L.inc_roff, L.inc_froff, L.inc_grp, L.grp_dim, W.pt_cinv, W.pt_g,
                      reinterpret_cast<const V4*>(W.E4), reinterpret_cast<V4*>(W.F4));
   HIP_CHECK(hipGetLastError());
 }
@@ -681,10 +694,18 @@ template <typename T>
 void launch_schur(const Layout& L, Work<T>& W, double lambda, hipStream_t s) {
   using V4 = typename Vec4<T>::type;
   if (L.n_chunks) {
-    hipLaunchKernelGGL(k_schur<T>, dim3(blocks_for(L.n_chunks, 4)), dim3(256), 0, s, L.n_chunks,
-                       L.chunks, reinterpret_cast<const uint2*>(L.pairs), L.grp_off, L.grp_dim,
-                       reinterpret_cast<const V4*>(W.J4), reinterpret_cast<const V4*>(W.E4),
-                       reinterpret_cast<const V4*>(W.F4), lambda, W.S, L.ld, L.Dp, W.parts);
+    const unsigned grid = blocks_for(L.n_chunks, 4);
+    const uint32_t ne = L.max_grp_dim * L.max_grp_dim;
+    auto go = [&](auto kern) {
+      hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, s, L.n_chunks, L.chunks,
+                         reinterpret_cast<const uint2*>(L.pairs), L.grp_off, L.grp_dim,
+                         reinterpret_cast<const V4*>(W.J4), reinterpret_cast<const V4*>(W.E4),
+                         reinterpret_cast<const V4*>(W.F4), lambda, W.S, L.ld, L.Dp, W.parts);
+    };
+    if (ne <= 64) go(k_schur<T, 1>);
+    else if (ne <= 128) go(k_schur<T, 2>);
+    else if (ne <= 192) go(k_schur<T, 3>);
+    else go(k_schur<T, 4>);
     HIP_CHECK(hipGetLastError());
   }
   if (L.n_multi) {
