@@ -141,12 +141,23 @@ def main() -> int:
                 "cholesky_failures": st["num_cholesky_failures"]}
 
     r_mixed, r_fp64 = rates(prof_mixed), rates(prof_fp64)
-    # Dominant kernel group: the dense factorization (D^3/3 + 2 D^2 FP64 FLOP).
+    # Dominant kernel: the factorization's trailing update (k_syrk_update),
+    # timed per launch with HIP events on the solve stream; its algorithmic
+    # FLOP per launch (lower-triangle update + rhs row + next diagonal block)
+    # come from the library beside the times.
+    n_tu = prof_fp64["count_trailing_update"]
+    t_tu = prof_fp64["time_trailing_update_s"] / max(1, n_tu)
+    f_tu = prof_fp64["flop_trailing_update"] / max(1, n_tu)
+    roof = {"bound": "mfma", "kernel": "k_syrk_update (FP64 trailing update + lookahead diagonal factorization)",
+            "achieved": f_tu / t_tu / 1e12, "peak": PEAK_FP64_MFMA / 1e12, "unit": "TFLOP/s",
+            "frac": f_tu / t_tu / PEAK_FP64_MFMA, "traffic": None,
+            "flop_per_launch": f_tu, "avg_launch_us": t_tu * 1e6, "launches": n_tu}
+    # The whole factorization (D^3/3 + 2 D^2 FP64 FLOP) as a kernel group.
     flop = D ** 3 / 3.0 + 2.0 * D ** 2
     t_fact = r_fp64["factor_ms"] * 1e-3
-    roof = {"bound": "mfma", "kernel": "dense FP64 Cholesky (potrf + trsm + syrk, D=%d)" % D,
-            "achieved": flop / t_fact / 1e12, "peak": PEAK_FP64_MFMA / 1e12, "unit": "TFLOP/s",
-            "frac": flop / t_fact / PEAK_FP64_MFMA, "traffic": None}
+    roof_fact = {"bound": "mfma", "kernel": "dense FP64 Cholesky, all launches (D=%d)" % D,
+                 "achieved": flop / t_fact / 1e12, "peak": PEAK_FP64_MFMA / 1e12, "unit": "TFLOP/s",
+                 "frac": flop / t_fact / PEAK_FP64_MFMA}
     # Schur assembly, mixed: 12 N_obs + 88 N_pts + 4 D(D+1) + 8 D algorithmic bytes per try.
     bytes_schur = 12 * n_obs + 88 * n_pts + 4 * D * (D + 1) + 8 * D
     t_as = r_mixed["schur_assembly_ms"] * 1e-3
@@ -187,6 +198,7 @@ def main() -> int:
         "mixed": r_mixed,
         "fp64": r_fp64,
         "roofline": roof,
+        "roofline_factorization": roof_fact,
         "roofline_schur": roof_schur,
         "cpu_baseline": cpu,
     }

@@ -186,6 +186,12 @@ typedef struct sfmba_stats {
   uint64_t count_linearize;
   uint64_t count_schur;
   uint64_t count_factor;
+  /* profile=1: the factorization's trailing-update kernel, timed per launch
+   * with HIP events on the solve stream, and the algorithmic FP64 FLOP of
+   * those launches (lower-triangle update, rhs row, next diagonal block). */
+  double time_trailing_update_s;
+  uint64_t count_trailing_update;
+  double flop_trailing_update;
 } sfmba_stats;
 
 /* bundle_adjust with engine options and diagnostics (stats may be NULL). */
@@ -207,9 +213,8 @@ int sfmba_problem_solve(sfmba_problem* problem, const sfmba_params* params,
                         const sfmba_options* options, sfmba_output* out, sfmba_stats* stats);
 void sfmba_problem_destroy(sfmba_problem* problem);
 
-/* Many independent problems (local-BA windows) solved by one persistent
- * batched launch.  inputs/outputs/stats are arrays of `count` entries
- * (stats may be NULL).  No counterpart in the reference, whose callers loop
+/* Many independent problems (local-BA windows) solved together.
+ * inputs/outputs/stats are arrays of `count` entries (stats may be NULL).  No counterpart in the reference, whose callers loop
  * over bundle_adjust (crates/sfm-reconstruction/src/lib.rs:708-719); each
  * problem gets exactly the bundle_adjust semantics. */
 int sfmba_bundle_adjust_batch(const sfmba_input* inputs, size_t count,

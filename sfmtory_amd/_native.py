@@ -88,7 +88,8 @@ class Stats(C.Structure):
         ("time_total_s", C.c_double), ("time_upload_s", C.c_double), ("time_linearize_s", C.c_double),
         ("time_schur_s", C.c_double), ("time_factor_s", C.c_double), ("time_solve_s", C.c_double),
         ("time_update_s", C.c_double), ("count_linearize", C.c_uint64), ("count_schur", C.c_uint64),
-        ("count_factor", C.c_uint64),
+        ("count_factor", C.c_uint64), ("time_trailing_update_s", C.c_double),
+        ("count_trailing_update", C.c_uint64), ("flop_trailing_update", C.c_double),
     ]
 
 

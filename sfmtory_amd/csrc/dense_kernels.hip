@@ -25,6 +25,8 @@
 // after the factorization, off the critical path.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "engine.hpp"
 
 namespace sfmba {
@@ -737,7 +739,7 @@ __global__ __launch_bounds__(1024) void k_lu_solve(const double* __restrict__ A,
 
 // ------------------------------------------------------------- launchers
 void dense_cholesky(double* S, double* Linv, uint32_t Dp, uint32_t ld, uint32_t D, int* flags,
-                    hipStream_t s) {
+                    hipStream_t s, KernelTimer* kt) {
   const int nblk = (int)(Dp / NB);
   double* Top = Linv + (size_t)nblk * LINV_BLOCK;
   hipLaunchKernelGGL(k_potrf_block, dim3(1), dim3(256), 0, s, S, (size_t)ld, 0, D, Top, flags);
@@ -746,9 +748,19 @@ void dense_cholesky(double* S, double* Linv, uint32_t Dp, uint32_t ld, uint32_t 
     const int strips = 8 * (nblk - 1 - k) + 1;
     hipLaunchKernelGGL(k_trsm_panel, dim3(strips), dim3(64), 0, s, S, (size_t)ld, k, Top, D, Dp);
     const int ntrail = nblk - 1 - k;
-    if (ntrail > 0)
+    if (ntrail > 0) {
+      hipEvent_t e0 = kt ? kt->before(s) : nullptr;
       hipLaunchKernelGGL(k_syrk_update, dim3(ntrail, ntrail + 1), dim3(256), 0, s, S, (size_t)ld, k,
                          nblk, D, Dp, Top, flags);
+      if (kt) {
+        // algorithmic FLOP: lower-triangle update of the m trailing rows with
+        // the panel's nk columns, the rhs row, and the next diagonal block
+        const double nk = std::min<double>(NB, (double)D - (double)k * NB);
+        const double m = std::max(0.0, (double)D - (double)(k + 1) * NB);
+        const double nb = std::min<double>(NB, m);
+        kt->after(e0, s, nk * m * (m + 1.0) + 2.0 * nk * m + nb * nb * nb / 3.0);
+      }
+    }
   }
   hipLaunchKernelGGL(k_block_inverse, dim3(nblk), dim3(256), 0, s, S, (size_t)ld, D, Top, Linv);
   HIP_CHECK(hipGetLastError());

@@ -163,11 +163,52 @@ void launch_refine_intrinsics(uint32_t n_cams_refined, const uint32_t* cam_list,
                               const double* pts, const int32_t* cam_model, double* cams,
                               const sfmba_intrinsics_params* p, hipStream_t s);
 
+// Per-launch timing of one kernel with HIP events on its stream (profiling).
+struct KernelTimer {
+  struct Rec {
+    hipEvent_t a, b;
+    double flop;
+  };
+  std::vector<Rec> recs;
+  KernelTimer() = default;
+  KernelTimer(const KernelTimer&) = delete;
+  KernelTimer& operator=(const KernelTimer&) = delete;
+  ~KernelTimer() {
+    for (auto& r : recs) {
+      (void)hipEventDestroy(r.a);
+      (void)hipEventDestroy(r.b);
+    }
+  }
+  hipEvent_t before(hipStream_t s) {
+    hipEvent_t e;
+    HIP_CHECK(hipEventCreate(&e));
+    HIP_CHECK(hipEventRecord(e, s));
+    return e;
+  }
+  void after(hipEvent_t a, hipStream_t s, double flop) {
+    hipEvent_t e;
+    HIP_CHECK(hipEventCreate(&e));
+    HIP_CHECK(hipEventRecord(e, s));
+    recs.push_back(Rec{a, e, flop});
+  }
+  // Adds seconds, launches and FLOP of the recorded launches (stream synced by the caller).
+  void collect(double& sec, uint64_t& n, double& flop) const {
+    for (const auto& r : recs) {
+      float ms = 0.f;
+      HIP_CHECK(hipEventElapsedTime(&ms, r.a, r.b));
+      sec += ms * 1e-3;
+      ++n;
+      flop += r.flop;
+    }
+  }
+};
+
 // ---- dense reduced-system solver (dense_kernels.hip) ----
 // Cholesky of the lower triangle of S (ld x Dp) with the rhs row at Dp; on
 // return row Dp holds y = L^{-1} rhs; flags[0] != 0 on a non-positive pivot.
+// kt (optional) times every trailing-update launch.
 void dense_cholesky(double* S, double* Linv, uint32_t Dp, uint32_t ld, uint32_t D, int* flags,
-                    hipStream_t s);
+                    hipStream_t s, KernelTimer* kt = nullptr);
 // Solve L^T x = y; y (Dp) is consumed as scratch, x (Dp) receives the result.
 void dense_backsolve(const double* S, const double* Linv, uint32_t Dp, uint32_t ld, double* y,
                      double* x, hipStream_t s);

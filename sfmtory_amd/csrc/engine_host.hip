@@ -403,6 +403,7 @@ struct Profiler {
   std::vector<Rec> recs;
   int open_phase = -1;
   hipEvent_t open_ev = nullptr;
+  KernelTimer trailing;  // the factorization's trailing-update launches
   ~Profiler() {
     for (auto& r : recs) {
       (void)hipEventDestroy(r.a);
@@ -439,6 +440,7 @@ struct Profiler {
         case 4: st->time_update_s += sec; break;
       }
     }
+    trailing.collect(st->time_trailing_update_s, st->count_trailing_update, st->flop_trailing_update);
   }
 };
 
@@ -608,7 +610,7 @@ static void solve(sfmba_problem& H, Work<T>& W, const sfmba_params* prm, const s
       }
       prof.begin(2);
       HIP_CHECK(hipMemsetAsync(W.flags, 0, 4 * sizeof(int), s));
-      dense_cholesky(W.S, W.Linv, L.Dp, L.ld, L.D, W.flags, s);
+      dense_cholesky(W.S, W.Linv, L.Dp, L.ld, L.D, W.flags, s, prof.on ? &prof.trailing : nullptr);
       prof.end();
       prof.begin(3);
       dense_extract_row(W.S, L.Dp, L.ld, L.Dp, W.vec, s);
