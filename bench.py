@@ -36,10 +36,49 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def rms_plain(ba, inp, out):
-    """sqrt(mean |r|^2) over in-front observations at the solved state."""
+def quat_to_R(q):
+    """Rotation matrices of poses given as rows (qw, qx, qy, qz, tx, ty, tz)."""
+    w, x, y, z = q[:, 0], q[:, 1], q[:, 2], q[:, 3]
+    n = np.sqrt(w * w + x * x + y * y + z * z)
+    w, x, y, z = w / n, x / n, y / n, z / n
+    return np.stack([
+        np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)], -1),
+        np.stack([2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)], -1),
+        np.stack([2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)], -1)], -2)
+
+
+def pose_error(poses, poses_true):
+    """Camera centres aligned to ground truth by a similarity (Umeyama; scale
+    is a free gauge): RMS centre error and mean rotation error (deg)."""
+    def centres(p):
+        p = np.asarray(p, dtype=np.float64).reshape(-1, 7)
+        R = quat_to_R(p)
+        t = p[:, 4:7]
+        return R, -np.einsum("nji,nj->ni", R, t)
+    R_e, C_e = centres(poses)
+    R_t, C_t = centres(poses_true)
+    me, mt = C_e.mean(0), C_t.mean(0)
+    Xe, Xt = C_e - me, C_t - mt
+    U, S, Vt = np.linalg.svd(Xt.T @ Xe / len(Xe))
+    Dg = np.diag([1.0, 1.0, np.sign(np.linalg.det(U @ Vt))])
+    R0 = U @ Dg @ Vt
+    s = np.trace(np.diag(S) @ Dg) / (Xe ** 2).sum(1).mean()
+    C_al = s * (R0 @ Xe.T).T + mt
+    rel = np.einsum("nji,njk->nik", R_t, R_e @ R0.T)
+    ang = np.degrees(np.arccos(np.clip((np.trace(rel, axis1=1, axis2=2) - 1) / 2, -1, 1)))
+    return {"centre_rms": float(np.sqrt(((C_al - C_t) ** 2).sum(1).mean())),
+            "rotation_mean_deg": float(ang.mean()), "scale": float(s)}
+
+
+def residual_stats(ba, inp, out, inlier_px=10.0):
+    """Plain residual norms at the solved state: RMS, mean (the reference's
+    mean_reprojection_error) and the RMS over |r| < inlier_px."""
     st = ba.BaInput(inp.camera_of_image, out.cameras, out.poses, out.points, inp.observations)
-    return float(np.sqrt(ba.total_cost(st, ba.BaParams(robust_loss=ba.RobustLoss.NONE)) / len(inp.observations)))
+    j = ba.observation_jacobians(st)
+    n = np.linalg.norm(j["r"], axis=1)[j["valid"]]
+    inl = n[n < inlier_px]
+    return {"rms_px": float(np.sqrt((n ** 2).mean())), "mean_px": float(n.mean()),
+            "inlier_rms_px": float(np.sqrt((inl ** 2).mean())), "inlier_fraction": float(len(inl) / len(n))}
 
 
 def per_unit(stats, key, count_key):
@@ -125,9 +164,29 @@ def main() -> int:
 
     mixed_t = float(np.mean(mixed_walls))
     fp64_t = float(np.mean(fp64_walls))
+    # Like-for-like: the mixed solve stopped at the iteration where it first
+    # reaches the FP64 solve's final cost (same trajectory, cut short).
+    fp64_final = fp64_outs[-1].final_cost
+    mixed_costs = prof_mixed["outer_costs"]
+    reach = next((i + 1 for i, c in enumerate(mixed_costs) if c <= fp64_final), None)
+    reach_t = None
+    if reach is not None:
+        p_reach = ba.BaParams(max_iterations=reach)
+        prob.solve(p_reach, precision="mixed")
+        tr = []
+        for _ in range(max(1, args.steps)):
+            t = time.perf_counter()
+            prob.solve(p_reach, precision="mixed")
+            tr.append(time.perf_counter() - t)
+        reach_t = float(np.mean(tr))
+        log(f"mixed reaches the FP64 final cost at iteration {reach}: {reach_t:.3f}s")
     D = prof_fp64["dim"]
-    rms_mixed = rms_plain(ba, inp, mixed_outs[-1])
-    rms_fp64 = rms_plain(ba, inp, fp64_outs[-1])
+    q_mixed = residual_stats(ba, inp, mixed_outs[-1])
+    q_fp64 = residual_stats(ba, inp, fp64_outs[-1])
+    rms_mixed, rms_fp64 = q_mixed["rms_px"], q_fp64["rms_px"]
+    pe = {"mixed": pose_error(mixed_outs[-1].poses, sc.poses_true),
+          "fp64": pose_error(fp64_outs[-1].poses, sc.poses_true),
+          "initial": pose_error(inp.poses, sc.poses_true)}
 
     def rates(st):
         t_lin = per_unit(st, "time_linearize_s", "count_linearize")
@@ -190,10 +249,15 @@ def main() -> int:
                    "scene": args.scene, "max_iterations": 50},
         "fp64_time_to_converge_s": fp64_t,
         "mixed_speedup_vs_fp64": fp64_t / mixed_t,
+        "mixed_time_to_fp64_final_cost_s": reach_t,
+        "mixed_iterations_to_fp64_final_cost": reach,
+        "iterations": {"mixed": mixed_outs[-1].iterations_run, "fp64": fp64_outs[-1].iterations_run},
         "device_resident_solve_s": {"mixed": float(np.mean([o.stats["time_total_s"] for o in mixed_outs])),
                                     "fp64": float(np.mean([o.stats["time_total_s"] for o in fp64_outs]))},
         "problem_create_s": create_s,
         "rms_px": {"mixed": rms_mixed, "fp64": rms_fp64, "diff": abs(rms_mixed - rms_fp64)},
+        "residuals": {"mixed": q_mixed, "fp64": q_fp64},
+        "pose_error": pe,
         "final_cost": {"mixed": mixed_outs[-1].final_cost, "fp64": fp64_outs[-1].final_cost},
         "mixed": r_mixed,
         "fp64": r_fp64,
