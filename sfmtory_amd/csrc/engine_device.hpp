@@ -1,0 +1,516 @@
+// Device-side view of one problem and the per-element work of an LM try.
+//
+// The grid-wide kernels (engine_kernels.hip) call these functions with one
+// thread per element; the persistent window solver (window_kernels.hip)
+// calls the same functions from one workgroup per problem.  Reference
+// correspondence (crates/sfm-ba/src/lib.rs):
+//   linearize_obs      hot loop A, 832-853 (analytic_jacobians + robust_weight)
+//                      and the per-observation terms of hot loop B, 880-909
+//   point_accumulate   c_diag / c_rhs sums, 885-886; the points_to_obs test, 960
+//   point_prepare      damped point-block inverse, 958-965
+//   compute_F / schur_chunk
+//                      damped diagonal and cross blocks, 923-956, and the
+//                      point elimination S -= E C^-1 E^T, rhs += E C^-1 g of
+//                      accumulate_point_schur, 112-211 / 967-1043
+//   backsub_point      delta_point = -C^-1 (g + E^T delta), 1096-1120
+//   update_pose / update_cam
+//                      trial state, 1122-1164 (perturb_pose 263-269)
+//   obs_cost           one term of total_cost 727-745 / mean_reprojection_error 708-725
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "camera_models.hpp"
+#include "engine.hpp"
+
+namespace sfmba {
+
+template <typename T>
+struct Vec4;
+template <>
+struct Vec4<float> {
+  using type = float4;
+};
+template <>
+struct Vec4<double> {
+  using type = double4;
+};
+
+// Pointers and sizes of one problem on the device (layout + work arrays).
+template <typename T>
+struct View {
+  using V4 = typename Vec4<T>::type;
+  uint32_t n_img, n_cam, n_pt, n_obs, n_inc, n_chunks;
+  uint32_t D, Dp, ld;
+  const uint32_t* obs_img;
+  const uint32_t* obs_pt;
+  const double2* obs_xy;
+  const uint32_t* obs_inc;
+  const uint32_t* pt_ptr;
+  const uint32_t* inc_grp;
+  const uint32_t* inc_pt;
+  const uint32_t* inc_roff;
+  const uint32_t* inc_froff;
+  const uint32_t* pt_inc_ptr;
+  const uint32_t* cam_of_img;
+  const int32_t* cam_model;
+  const uint32_t* img_slot;
+  const uint32_t* cam_slot;
+  const uint8_t* cam_free;
+  const uint32_t* cam_nfree;
+  const uint32_t* grp_off;
+  const uint32_t* grp_dim;
+  const uint2* pairs;
+  const uint4* chunks;
+  V4* J4;
+  V4* E4;
+  V4* F4;
+  T* obs_pc;
+  uint8_t* obs_valid;
+  double* pt_C;
+  double* pt_g;
+  uint8_t* pt_active;
+  double* pt_cinv;
+  uint8_t* pt_ok;
+  double* S;
+  double* delta;
+  double* poses[2];
+  double* cams[2];
+  double* pts[2];
+};
+
+// Compensated (Kahan) accumulator for the FP32 path; plain sum in FP64.
+template <typename T>
+struct Acc {
+  T s = T(0), c = T(0);
+  __device__ inline void add(T x) {
+    const T y = x - c;
+    const T t = s + y;
+    c = (t - s) - y;
+    s = t;
+  }
+  __device__ inline double value() const { return (double)s - (double)c; }
+};
+template <>
+struct Acc<double> {
+  double s = 0.0;
+  __device__ inline void add(double x) { s += x; }
+  __device__ inline double value() const { return s; }
+};
+
+// ---------------------------------------------------------------- linearize
+// Observation o (point-major order): for each of its incidences one J4 and
+// one E4 row per unknown, and its contribution to its point's 3x3 block and
+// gradient (obs_pc).
+template <typename T>
+__device__ inline void linearize_obs(const View<T>& v, uint32_t o, int cur, int loss, double scale) {
+  using V4 = typename Vec4<T>::type;
+  const uint32_t img = v.obs_img[o];
+  const uint32_t pt = v.obs_pt[o];
+  const uint32_t cam = v.cam_of_img[img];
+  const int model = v.cam_model[cam];
+  const double* pose = v.poses[cur] + 7 * (size_t)img;
+  const double* P = v.cams[cur] + 8 * (size_t)cam;
+  const double* Xp = v.pts[cur] + 3 * (size_t)pt;
+  double c0, c1, c2;
+  transform_point(pose, Xp[0], Xp[1], Xp[2], c0, c1, c2);
+
+  const bool pose_free = v.img_slot[img] != NONE;
+  const bool cam_is_free = v.cam_slot[cam] != NONE;
+  const uint32_t kf = cam_is_free ? v.cam_nfree[cam] : 0u;
+  const uint32_t ns = (pose_free ? 6u : 0u) + kf;
+  const uint32_t inc0 = v.obs_inc[2 * o], inc1 = v.obs_inc[2 * o + 1];
+  const uint32_t roff0 = inc0 != NONE ? v.inc_roff[inc0] : 0u;
+  const bool two = inc1 != NONE;
+  const uint32_t roff1 = two ? v.inc_roff[inc1] : 0u;
+  T* pc_out = v.obs_pc + 9 * (size_t)o;
+
+  if (c2 <= 1e-9) {
+    // Behind the camera: dropped for this iteration (lib.rs:339-341).
+    v.obs_valid[o] = 0;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) pc_out[k] = T(0);
+    const V4 z = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+    for (uint32_t s = 0; s < 14; ++s) {
+      if (s >= ns) break;
+      const uint32_t row = (two && s >= 6) ? roff1 + (s - 6) : roff0 + s;
+      v.J4[row] = z;
+      v.E4[row] = z;
+    }
+    return;
+  }
+  v.obs_valid[o] = 1;
+  const double2 xy = v.obs_xy[o];
+  double r0, r1;
+  linearization_residual(model, P, c0, c1, c2, xy.x, xy.y, r0, r1);
+  const double wd = robust_weight(sqrt(r0 * r0 + r1 * r1), loss, scale);
+  const T w = (T)wd;
+
+  T du[3], dv[3], jc0[8], jc1[8];
+  model_jacobian<T>(model, P, (T)c0, (T)c1, (T)c2, du, dv, jc0, jc1);
+  T R[9];
+  rotation_matrix<T>(pose, R);
+  T jp[12], jx[6];
+  pose_point_jacobians<T>(R, (T)(c0 - pose[4]), (T)(c1 - pose[5]), (T)(c2 - pose[6]), du, dv, jp,
+                          jx);
+  const T tr0 = (T)r0, tr1 = (T)r1;
+
+  // Point terms: w Jx^T Jx (upper 6), w Jx^T r.
+  const T wx0 = w * jx[0], wx1 = w * jx[1], wx2 = w * jx[2];
+  const T wy0 = w * jx[3], wy1 = w * jx[4], wy2 = w * jx[5];
+  pc_out[0] = wx0 * jx[0] + wy0 * jx[3];
+  pc_out[1] = wx0 * jx[1] + wy0 * jx[4];
+  pc_out[2] = wx0 * jx[2] + wy0 * jx[5];
+  pc_out[3] = wx1 * jx[1] + wy1 * jx[4];
+  pc_out[4] = wx1 * jx[2] + wy1 * jx[5];
+  pc_out[5] = wx2 * jx[2] + wy2 * jx[5];
+  pc_out[6] = wx0 * tr0 + wy0 * tr1;
+  pc_out[7] = wx1 * tr0 + wy1 * tr1;
+  pc_out[8] = wx2 * tr0 + wy2 * tr1;
+
+  // Side vector: free pose columns, then the camera's free parameters.
+  T s0[14], s1[14];
+#pragma unroll
+  for (int s = 0; s < 14; ++s) {
+    s0[s] = T(0);
+    s1[s] = T(0);
+  }
+  if (pose_free) {
+#pragma unroll
+    for (int s = 0; s < 6; ++s) {
+      s0[s] = jp[s];
+      s1[s] = jp[6 + s];
+    }
+  }
+  const uint8_t* fi = v.cam_free + 8 * (size_t)cam;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    if ((uint32_t)j < kf) {
+      const int idx = fi[j];
+      T a0 = T(0), a1 = T(0);
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        if (q == idx) {
+          a0 = jc0[q];
+          a1 = jc1[q];
+        }
+      if (pose_free) {
+#pragma unroll
+        for (int s = 6; s < 14; ++s)
+          if (s == 6 + j) {
+            s0[s] = a0;
+            s1[s] = a1;
+          }
+      } else {
+#pragma unroll
+        for (int s = 0; s < 8; ++s)
+          if (s == j) {
+            s0[s] = a0;
+            s1[s] = a1;
+          }
+      }
+    }
+  }
+#pragma unroll
+  for (uint32_t s = 0; s < 14; ++s) {
+    if (s >= ns) break;
+    const uint32_t row = (two && s >= 6) ? roff1 + (s - 6) : roff0 + s;
+    const T w0 = w * s0[s], w1 = w * s1[s];
+    v.J4[row] = V4{s0[s], s1[s], w0, w1};
+    v.E4[row] = V4{w0 * jx[0] + w1 * jx[3], w0 * jx[1] + w1 * jx[4], w0 * jx[2] + w1 * jx[5],
+                   w0 * tr0 + w1 * tr1};
+  }
+}
+
+// ------------------------------------------------------- point accumulation
+template <typename T>
+__device__ inline void point_accumulate(const View<T>& v, uint32_t p) {
+  double a[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) a[k] = 0.0;
+  uint8_t active = 0;
+  for (uint32_t o = v.pt_ptr[p]; o < v.pt_ptr[p + 1]; ++o) {
+    if (!v.obs_valid[o]) continue;
+    if (v.obs_inc[2 * o] != NONE) active = 1;
+    const T* c = v.obs_pc + 9 * (size_t)o;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) a[k] += (double)c[k];
+  }
+#pragma unroll
+  for (int k = 0; k < 6; ++k) v.pt_C[6 * (size_t)p + k] = a[k];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) v.pt_g[3 * (size_t)p + k] = a[6 + k];
+  v.pt_active[p] = active;
+}
+
+// Damped point block (C + lambda diag C)^-1 by the adjugate; a point without
+// free couplings or with det == 0 is left out of the Schur complement and
+// keeps its position (lib.rs:958-965, 1001, 1102).
+template <typename T>
+__device__ inline void point_prepare(const View<T>& v, uint32_t p, double lambda) {
+  double* o = v.pt_cinv + 9 * (size_t)p;
+  const double* C = v.pt_C + 6 * (size_t)p;
+  bool ok = false;
+  if (v.pt_active[p]) {
+    const double m11 = C[0] + C[0] * lambda, m12 = C[1], m13 = C[2];
+    const double m22 = C[3] + C[3] * lambda, m23 = C[4];
+    const double m33 = C[5] + C[5] * lambda;
+    const double m21 = m12, m31 = m13, m32 = m23;
+    const double a = m22 * m33 - m32 * m23;
+    const double b = m21 * m33 - m31 * m23;
+    const double c = m21 * m32 - m31 * m22;
+    const double det = m11 * a - m12 * b + m13 * c;
+    if (det != 0.0) {
+      ok = true;
+      o[0] = a / det;
+      o[1] = (m13 * m32 - m33 * m12) / det;
+      o[2] = (m12 * m23 - m22 * m13) / det;
+      o[3] = -b / det;
+      o[4] = (m11 * m33 - m31 * m13) / det;
+      o[5] = (m13 * m21 - m23 * m11) / det;
+      o[6] = c / det;
+      o[7] = (m12 * m31 - m32 * m11) / det;
+      o[8] = (m11 * m22 - m21 * m12) / det;
+    }
+  }
+  if (!ok)
+#pragma unroll
+    for (int k = 0; k < 9; ++k) o[k] = 0.0;
+  v.pt_ok[p] = ok ? 1 : 0;
+}
+
+// F = E Cinv and F . g (the incidence's rhs share) for incidence a; F is
+// stored group-major.
+template <typename T>
+__device__ inline void compute_F(const View<T>& v, uint32_t a) {
+  using V4 = typename Vec4<T>::type;
+  const uint32_t pt = v.inc_pt[a];
+  const double* ci = v.pt_cinv + 9 * (size_t)pt;
+  T c[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) c[k] = (T)ci[k];
+  const T g0 = (T)v.pt_g[3 * (size_t)pt], g1 = (T)v.pt_g[3 * (size_t)pt + 1],
+          g2 = (T)v.pt_g[3 * (size_t)pt + 2];
+  const uint32_t r0 = v.inc_roff[a], f0ff = v.inc_froff[a], n = v.grp_dim[v.inc_grp[a]];
+  for (uint32_t r = 0; r < n; ++r) {
+    const V4 e = v.E4[r0 + r];
+    const T f0 = e.x * c[0] + e.y * c[3] + e.z * c[6];
+    const T f1 = e.x * c[1] + e.y * c[4] + e.z * c[7];
+    const T f2 = e.x * c[2] + e.y * c[5] + e.z * c[8];
+    // w: this row's share of rhs, F . g_point = E Cinv g_point
+    v.F4[f0ff + r] = V4{f0, f1, f2, f0 * g0 + f1 * g1 + f2 * g2};
+  }
+}
+
+constexpr int PART_STRIDE = 2 * 196 + 16;
+
+// One wavefront reduces one chunk of a destination block (ga >= gb) of the
+// reduced system; lane l owns entries l, l + 64, ... (NQ of them) of the
+// da x db block.  Sums, over the chunk's point pairs (a, b), B = w J_a^T J_b
+// for pairs of the same observation and Schur = F_a E_b^T = E_a Cinv E_b^T;
+// on diagonal blocks also the right-hand side F_a g_point - w J_a^T r over
+// pairs a == b (F4.w - E4.w).  A pair is two words: a's first F row (F is
+// group-major) with bit 31 "same observation" and bit 30 "a's rows precede
+// b's", and b's first row with bit 31 "a == b".  U pairs are processed per
+// step with all their loads issued together.  A chunk (ch.w == NONE) that
+// covers its whole destination is written straight into S (with the
+// Marquardt damping of the diagonal); others leave partials.
+template <typename T, int NQ>
+__device__ inline void schur_chunk(const View<T>& v, const uint4 ch, double lambda, double* parts) {
+  using V4 = typename Vec4<T>::type;
+  constexpr uint32_t HI = 0x80000000u, BEFORE = 0x40000000u, OFF = 0x3FFFFFFFu;
+  constexpr int UMAX = sizeof(T) == 4 ? 4 : 2;  // measured best on gfx950
+  constexpr int U = UMAX / NQ > 1 ? UMAX / NQ : 1;
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t ga = ch.x >> 16, gb = ch.x & 0xFFFFu;
+  const uint32_t da = v.grp_dim[ga], db = v.grp_dim[gb];
+  const uint32_t ne = da * db;
+  const bool diag = ga == gb;
+  Acc<T> accS[NQ], accB[NQ], accR;
+  uint32_t er[NQ], ec[NQ];
+  bool on[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    const uint32_t e = lane + 64 * q;
+    on[q] = e < ne;
+    er[q] = on[q] ? e % da : 0;
+    ec[q] = on[q] ? e / da : 0;
+  }
+  const bool rlane = diag && lane < da;
+  for (uint32_t p = ch.y; p < ch.z; p += U) {
+    uint2 pr[U];
+    bool ok[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      ok[u] = p + u < ch.z;
+      pr[u] = ok[u] ? v.pairs[p + u] : make_uint2(0u, 0u);
+    }
+    V4 f[U][NQ], e[U][NQ];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t ra = pr[u].x & OFF, rb = pr[u].y & OFF;
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        if (ok[u] && on[q]) {
+          f[u][q] = v.F4[ra + er[q]];
+          e[u][q] = v.E4[rb + ec[q]];
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (!ok[u]) continue;
+      const uint32_t ra = pr[u].x & OFF, rb = pr[u].y & OFF;
+#pragma unroll
+      for (int q = 0; q < NQ; ++q)
+        if (on[q]) accS[q].add(f[u][q].x * e[u][q].x + f[u][q].y * e[u][q].y + f[u][q].z * e[u][q].z);
+      if (pr[u].x & HI) {
+        // same observation: a's point-major first row from b's
+        const bool self = (pr[u].y & HI) != 0u;
+        const uint32_t raj = self ? rb : ((pr[u].x & BEFORE) ? rb - da : rb + db);
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+          if (on[q]) {
+            const V4 ja = v.J4[raj + er[q]];
+            const V4 jb = v.J4[rb + ec[q]];
+            accB[q].add(ja.z * jb.x + ja.w * jb.y);
+          }
+        }
+      }
+      if (rlane && (pr[u].y & HI)) accR.add(v.F4[ra + lane].w - v.E4[rb + lane].w);
+    }
+  }
+  const uint32_t oa = v.grp_off[ga], ob = v.grp_off[gb];
+  if (ch.w == NONE) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      if (on[q]) {
+        const uint32_t r = er[q], c = ec[q];
+        if (diag && r < c) continue;
+        const double bv = accB[q].value();
+        double val = bv - accS[q].value();
+        if (diag && r == c) val = (bv + bv * lambda) - accS[q].value();
+        v.S[(size_t)(ob + c) * v.ld + oa + r] = val;
+      }
+    }
+    if (rlane) v.S[(size_t)(oa + lane) * v.ld + v.Dp] = accR.value();
+  } else {
+    double* pp = parts + (size_t)ch.w * PART_STRIDE;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      if (on[q]) {
+        pp[lane + 64 * q] = accB[q].value();
+        pp[196 + lane + 64 * q] = accS[q].value();
+      }
+    }
+    if (rlane) pp[392 + lane] = accR.value();
+  }
+}
+
+// ------------------------------------------------- back-substitution + update
+template <typename T>
+__device__ inline void backsub_point(const View<T>& v, uint32_t p, int cur) {
+  const double* X = v.pts[cur] + 3 * (size_t)p;
+  double* Y = v.pts[1 - cur] + 3 * (size_t)p;
+  if (!v.pt_ok[p]) {
+    Y[0] = X[0];
+    Y[1] = X[1];
+    Y[2] = X[2];
+    return;
+  }
+  double acc0 = v.pt_g[3 * (size_t)p], acc1 = v.pt_g[3 * (size_t)p + 1], acc2 = v.pt_g[3 * (size_t)p + 2];
+  for (uint32_t a = v.pt_inc_ptr[p]; a < v.pt_inc_ptr[p + 1]; ++a) {
+    const uint32_t g = v.inc_grp[a];
+    const uint32_t r0 = v.inc_roff[a], n = v.grp_dim[g], off = v.grp_off[g];
+    double c0 = 0.0, c1 = 0.0, c2 = 0.0;
+    for (uint32_t r = 0; r < n; ++r) {
+      const auto e = v.E4[r0 + r];
+      const double d = v.delta[off + r];
+      c0 += (double)e.x * d;
+      c1 += (double)e.y * d;
+      c2 += (double)e.z * d;
+    }
+    acc0 += c0;
+    acc1 += c1;
+    acc2 += c2;
+  }
+  const double* ci = v.pt_cinv + 9 * (size_t)p;
+  Y[0] = X[0] + -(ci[0] * acc0 + ci[1] * acc1 + ci[2] * acc2);
+  Y[1] = X[1] + -(ci[3] * acc0 + ci[4] * acc1 + ci[5] * acc2);
+  Y[2] = X[2] + -(ci[6] * acc0 + ci[7] * acc1 + ci[8] * acc2);
+}
+
+// perturb_pose: R' = exp(omega) R (nalgebra from_scaled_axis: identity when
+// |omega/2|^2 <= eps^2, no renormalization), t' = t + dt.
+template <typename T>
+__device__ inline void update_pose(const View<T>& v, uint32_t i, int cur) {
+  const double* p = v.poses[cur] + 7 * (size_t)i;
+  double* o = v.poses[1 - cur] + 7 * (size_t)i;
+  const uint32_t s = v.img_slot[i];
+  double d[6] = {0, 0, 0, 0, 0, 0};
+  if (s != NONE)
+    for (int k = 0; k < 6; ++k) d[k] = v.delta[s + k];
+  const double hx = d[0] / 2.0, hy = d[1] / 2.0, hz = d[2] / 2.0;
+  const double nn = hx * hx + hy * hy + hz * hz;
+  const double eps = 2.220446049250313e-16;
+  double dw = 1.0, dx = 0.0, dy = 0.0, dz = 0.0;
+  if (!(nn <= eps * eps)) {
+    const double n = sqrt(nn);
+    const double sn = sin(n) / n;
+    dx = hx * sn;
+    dy = hy * sn;
+    dz = hz * sn;
+    dw = cos(n);
+  }
+  o[0] = dw * p[0] - dx * p[1] - dy * p[2] - dz * p[3];
+  o[1] = dw * p[1] + dx * p[0] + dy * p[3] - dz * p[2];
+  o[2] = dw * p[2] - dx * p[3] + dy * p[0] + dz * p[1];
+  o[3] = dw * p[3] + dx * p[2] - dy * p[1] + dz * p[0];
+  o[4] = p[4] + d[3];
+  o[5] = p[5] + d[4];
+  o[6] = p[6] + d[5];
+}
+
+template <typename T>
+__device__ inline void update_cam(const View<T>& v, uint32_t c, int cur) {
+  double p[8];
+  for (int k = 0; k < 8; ++k) p[k] = v.cams[cur][8 * (size_t)c + k];
+  const uint32_t s = v.cam_slot[c];
+  if (s != NONE) {
+    const uint32_t nf = v.cam_nfree[c];
+    for (uint32_t j = 0; j < nf; ++j) p[v.cam_free[8 * (size_t)c + j]] += v.delta[s + j];
+  }
+  for (int k = 0; k < 8; ++k) v.cams[1 - cur][8 * (size_t)c + k] = p[k];
+}
+
+// ------------------------------------------------------------------- cost
+// Observation o's term: mode 0 w(|r|) |r|^2 (total_cost), mode 1 |r| (the
+// numerator of mean_reprojection_error); in_front is 1 for an observation
+// in front of its camera.
+__device__ inline void obs_cost(const uint32_t* obs_img, const uint32_t* obs_pt, const double2* obs_xy,
+                                const uint32_t* cam_of_img, const int32_t* cam_model,
+                                const double* poses, const double* cams, const double* pts,
+                                uint32_t o, int loss, double scale, int mode, double& term,
+                                double& in_front) {
+  term = 0.0;
+  in_front = 0.0;
+  const uint32_t img = obs_img[o], pt = obs_pt[o], cam = cam_of_img[img];
+  double c0, c1, c2;
+  transform_point(poses + 7 * (size_t)img, pts[3 * (size_t)pt], pts[3 * (size_t)pt + 1],
+                  pts[3 * (size_t)pt + 2], c0, c1, c2);
+  if (c2 <= 1e-9) return;
+  const double2 xy = obs_xy[o];
+  double r0, r1;
+  project_residual(cam_model[cam], cams + 8 * (size_t)cam, c0, c1, c2, xy.x, xy.y, r0, r1);
+  const double n = sqrt(r0 * r0 + r1 * r1);
+  if (mode == 0) {
+    const double w = robust_weight(n, loss, scale);
+    term = w * n * n;
+  } else {
+    term = n;
+  }
+  in_front = 1.0;
+}
+
+}  // namespace sfmba

@@ -7,7 +7,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <exception>
+#include <mutex>
+#include <thread>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -17,6 +21,8 @@
 #include "../../include/sfmba.h"
 #include "camera_models.hpp"
 #include "engine.hpp"
+#include "engine_device.hpp"
+#include "window.hpp"
 
 namespace sfmba {
 
@@ -118,7 +124,10 @@ struct Pack {
   std::vector<double> poses, cams, pts;  // packed FP64 state
 };
 
-static void build_pack(const sfmba_input* in, Pack& P, bool need_pairs) {
+// pair_chunk: most pairs one Schur chunk reduces (0: no limit, one chunk
+// per destination).
+static void build_pack(const sfmba_input* in, Pack& P, bool need_pairs,
+                       uint32_t pair_chunk = PAIR_CHUNK) {
   P.n_img = (uint32_t)in->num_images;
   P.n_cam = (uint32_t)in->num_cameras;
   P.n_pt = (uint32_t)in->num_points;
@@ -321,7 +330,8 @@ static void build_pack(const sfmba_input* in, Pack& P, bool need_pairs) {
       gb = 0;
     }
     const uint32_t b0 = cnt[d], b1 = cnt[d + 1];
-    const uint32_t nch = std::max<uint32_t>(1, (b1 - b0 + PAIR_CHUNK - 1) / PAIR_CHUNK);
+    const uint32_t nch =
+        pair_chunk == 0 ? 1u : std::max<uint32_t>(1, (b1 - b0 + pair_chunk - 1) / pair_chunk);
     if (nch == 1) {
       P.chunks.push_back(uint4{code, b0, b1, NONE});
       continue;
@@ -330,7 +340,7 @@ static void build_pack(const sfmba_input* in, Pack& P, bool need_pairs) {
     P.dest_part0.push_back(P.n_parts);
     P.dest_nparts.push_back(nch);
     for (uint32_t k = 0; k < nch; ++k) {
-      const uint32_t lo = b0 + k * PAIR_CHUNK, hi = std::min(b1, lo + PAIR_CHUNK);
+      const uint32_t lo = b0 + k * pair_chunk, hi = std::min(b1, lo + pair_chunk);
       P.chunks.push_back(uint4{code, lo, hi, P.n_parts++});
     }
   }
@@ -790,6 +800,289 @@ static void upload_state(const sfmba_input* in, Problem& pr, Pack& P, hipStream_
 
 }  // namespace sfmba
 
+// ------------------------------------------------------------ batched solve
+// Many small problems (local-BA windows) in one persistent launch
+// (window_kernels.hip): host packing on worker threads, every window's
+// arrays in one device allocation uploaded by one copy, one kernel launch,
+// one copy of all final states back.  Windows whose reduced system exceeds
+// WINDOW_MAX_DIM go through the single-problem path afterwards.
+namespace sfmba {
+
+// Offsets of one window's arrays inside the batch allocation.
+struct WinPlan {
+  size_t obs_img, obs_pt, obs_xy, obs_inc, pt_ptr, inc_grp, inc_pt, inc_roff, inc_froff, pt_inc_ptr;
+  size_t cam_of_img, cam_model, img_slot, cam_slot, cam_free, cam_nfree, grp_off, grp_dim, pairs, chunks;
+  size_t state;  // poses[0], cams[0], pts[0], poses[1], cams[1], pts[1] (doubles)
+  size_t J4, E4, F4, obs_pc, obs_valid, pt_C, pt_g, pt_active, pt_cinv, pt_ok, S, delta;
+};
+
+struct Planner {
+  size_t n = 0;
+  size_t take(size_t bytes) {
+    const size_t o = n;
+    n += (bytes + 255) & ~size_t(255);
+    return o;
+  }
+};
+
+template <typename T>
+static void batch_solve(const sfmba_input* inputs, size_t count, const sfmba_params* prm,
+                        const sfmba_options& opt, sfmba_output* outs, sfmba_stats* stats) {
+  using clock = std::chrono::steady_clock;
+  using V4 = typename Vec4<T>::type;
+  const auto t0 = clock::now();
+  for (size_t w = 0; w < count; ++w) validate(&inputs[w]);
+  StreamGuard sg(opt.device);
+
+  // Host packing on worker threads.
+  std::vector<Pack> packs(count);
+  {
+    std::atomic<size_t> next{0};
+    std::exception_ptr err;
+    std::mutex err_mu;
+    const unsigned nth = std::max(1u, std::min<unsigned>(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    for (unsigned k = 0; k < std::min<size_t>(nth, count); ++k)
+      th.emplace_back([&] {
+        for (size_t w; (w = next++) < count;) {
+          try {
+            build_pack(&inputs[w], packs[w], true, 0);
+          } catch (...) {
+            std::lock_guard<std::mutex> g(err_mu);
+            if (!err) err = std::current_exception();
+          }
+        }
+      });
+    for (auto& t : th) t.join();
+    if (err) std::rethrow_exception(err);
+  }
+  std::vector<size_t> win;   // solved by the window kernel
+  std::vector<size_t> big;   // too large for it
+  uint32_t max_dim = 1;
+  for (size_t w = 0; w < count; ++w) {
+    if (packs[w].D > WINDOW_MAX_DIM) {
+      big.push_back(w);
+      continue;
+    }
+    win.push_back(w);
+    for (uint32_t g : packs[w].grp_dim) max_dim = std::max(max_dim, g);
+  }
+
+  // One allocation: uploaded structure and initial state first, then the
+  // device-only work arrays.
+  const size_t nw = win.size();
+  std::vector<WinPlan> plan(nw);
+  Planner up;
+  for (size_t k = 0; k < nw; ++k) {
+    const Pack& P = packs[win[k]];
+    WinPlan& q = plan[k];
+    q.obs_img = up.take(4 * (size_t)P.n_obs);
+    q.obs_pt = up.take(4 * (size_t)P.n_obs);
+    q.obs_xy = up.take(16 * (size_t)P.n_obs);
+    q.obs_inc = up.take(8 * (size_t)P.n_obs);
+    q.pt_ptr = up.take(4 * ((size_t)P.n_pt + 1));
+    q.inc_grp = up.take(4 * P.inc_grp.size());
+    q.inc_pt = up.take(4 * P.inc_pt.size());
+    q.inc_roff = up.take(4 * P.inc_roff.size());
+    q.inc_froff = up.take(4 * P.inc_froff.size());
+    q.pt_inc_ptr = up.take(4 * P.pt_inc_ptr.size());
+    q.cam_of_img = up.take(4 * (size_t)P.n_img);
+    q.cam_model = up.take(4 * (size_t)P.n_cam);
+    q.img_slot = up.take(4 * (size_t)P.n_img);
+    q.cam_slot = up.take(4 * (size_t)P.n_cam);
+    q.cam_free = up.take(8 * (size_t)P.n_cam);
+    q.cam_nfree = up.take(4 * (size_t)P.n_cam);
+    q.grp_off = up.take(4 * P.grp_off.size());
+    q.grp_dim = up.take(4 * P.grp_dim.size());
+    q.pairs = up.take(4 * P.pairs.size());
+    q.chunks = up.take(16 * P.chunks.size());
+  }
+  const size_t up_struct = up.n;
+  std::vector<size_t> state_dbl(nw);
+  for (size_t k = 0; k < nw; ++k) {
+    const Pack& P = packs[win[k]];
+    state_dbl[k] = 7 * (size_t)P.n_img + 8 * (size_t)P.n_cam + 3 * (size_t)P.n_pt;
+    plan[k].state = up.take(2 * 8 * state_dbl[k]);
+  }
+  const size_t state_end = up.n;
+  Planner& wk = up;  // work arrays follow in the same allocation
+  for (size_t k = 0; k < nw; ++k) {
+    const Pack& P = packs[win[k]];
+    WinPlan& q = plan[k];
+    q.J4 = wk.take(sizeof(V4) * (size_t)P.n_rows);
+    q.E4 = wk.take(sizeof(V4) * (size_t)P.n_rows);
+    q.F4 = wk.take(sizeof(V4) * (size_t)P.n_rows);
+    q.obs_pc = wk.take(9 * sizeof(T) * (size_t)P.n_obs);
+    q.obs_valid = wk.take(P.n_obs);
+    q.pt_C = wk.take(6 * 8 * (size_t)P.n_pt);
+    q.pt_g = wk.take(3 * 8 * (size_t)P.n_pt);
+    q.pt_active = wk.take(P.n_pt);
+    q.pt_cinv = wk.take(9 * 8 * (size_t)P.n_pt);
+    q.pt_ok = wk.take(P.n_pt);
+    q.S = wk.take(8 * ((size_t)P.D + 1) * P.D);
+    q.delta = wk.take(8 * (size_t)P.D);
+  }
+  DevArena arena;
+  uint8_t* base = arena.alloc<uint8_t>(std::max<size_t>(up.n, 1));
+  std::vector<uint8_t> host(state_end);
+  auto put = [&](size_t off, const void* src, size_t bytes) {
+    if (bytes) std::memcpy(host.data() + off, src, bytes);
+  };
+  std::vector<View<T>> views(nw);
+  for (size_t k = 0; k < nw; ++k) {
+    const Pack& P = packs[win[k]];
+    const WinPlan& q = plan[k];
+    put(q.obs_img, P.obs_img.data(), 4 * (size_t)P.n_obs);
+    put(q.obs_pt, P.obs_pt.data(), 4 * (size_t)P.n_obs);
+    put(q.obs_xy, P.obs_xy.data(), 16 * (size_t)P.n_obs);
+    put(q.obs_inc, P.obs_inc.data(), 8 * (size_t)P.n_obs);
+    put(q.pt_ptr, P.pt_ptr.data(), 4 * P.pt_ptr.size());
+    put(q.inc_grp, P.inc_grp.data(), 4 * P.inc_grp.size());
+    put(q.inc_pt, P.inc_pt.data(), 4 * P.inc_pt.size());
+    put(q.inc_roff, P.inc_roff.data(), 4 * P.inc_roff.size());
+    put(q.inc_froff, P.inc_froff.data(), 4 * P.inc_froff.size());
+    put(q.pt_inc_ptr, P.pt_inc_ptr.data(), 4 * P.pt_inc_ptr.size());
+    put(q.cam_of_img, P.cam_of_img.data(), 4 * (size_t)P.n_img);
+    put(q.cam_model, P.cam_model.data(), 4 * (size_t)P.n_cam);
+    put(q.img_slot, P.img_slot.data(), 4 * (size_t)P.n_img);
+    put(q.cam_slot, P.cam_slot.data(), 4 * (size_t)P.n_cam);
+    put(q.cam_free, P.cam_free.data(), 8 * (size_t)P.n_cam);
+    put(q.cam_nfree, P.cam_nfree.data(), 4 * (size_t)P.n_cam);
+    put(q.grp_off, P.grp_off.data(), 4 * P.grp_off.size());
+    put(q.grp_dim, P.grp_dim.data(), 4 * P.grp_dim.size());
+    put(q.pairs, P.pairs.data(), 4 * P.pairs.size());
+    put(q.chunks, P.chunks.data(), 16 * P.chunks.size());
+    put(q.state, P.poses.data(), 8 * P.poses.size());
+    put(q.state + 8 * P.poses.size(), P.cams.data(), 8 * P.cams.size());
+    put(q.state + 8 * (P.poses.size() + P.cams.size()), P.pts.data(), 8 * P.pts.size());
+
+    View<T>& v = views[k];
+    v = View<T>{};
+    v.n_img = P.n_img;
+    v.n_cam = P.n_cam;
+    v.n_pt = P.n_pt;
+    v.n_obs = P.n_obs;
+    v.n_inc = (uint32_t)P.inc_grp.size();
+    v.n_chunks = (uint32_t)P.chunks.size();
+    v.D = P.D;
+    v.Dp = P.D;  // rhs row right below the system
+    v.ld = P.D + 1;
+    auto at = [&](size_t off) { return base + off; };
+    v.obs_img = reinterpret_cast<const uint32_t*>(at(q.obs_img));
+    v.obs_pt = reinterpret_cast<const uint32_t*>(at(q.obs_pt));
+    v.obs_xy = reinterpret_cast<const double2*>(at(q.obs_xy));
+    v.obs_inc = reinterpret_cast<const uint32_t*>(at(q.obs_inc));
+    v.pt_ptr = reinterpret_cast<const uint32_t*>(at(q.pt_ptr));
+    v.inc_grp = reinterpret_cast<const uint32_t*>(at(q.inc_grp));
+    v.inc_pt = reinterpret_cast<const uint32_t*>(at(q.inc_pt));
+    v.inc_roff = reinterpret_cast<const uint32_t*>(at(q.inc_roff));
+    v.inc_froff = reinterpret_cast<const uint32_t*>(at(q.inc_froff));
+    v.pt_inc_ptr = reinterpret_cast<const uint32_t*>(at(q.pt_inc_ptr));
+    v.cam_of_img = reinterpret_cast<const uint32_t*>(at(q.cam_of_img));
+    v.cam_model = reinterpret_cast<const int32_t*>(at(q.cam_model));
+    v.img_slot = reinterpret_cast<const uint32_t*>(at(q.img_slot));
+    v.cam_slot = reinterpret_cast<const uint32_t*>(at(q.cam_slot));
+    v.cam_free = reinterpret_cast<const uint8_t*>(at(q.cam_free));
+    v.cam_nfree = reinterpret_cast<const uint32_t*>(at(q.cam_nfree));
+    v.grp_off = reinterpret_cast<const uint32_t*>(at(q.grp_off));
+    v.grp_dim = reinterpret_cast<const uint32_t*>(at(q.grp_dim));
+    v.pairs = reinterpret_cast<const uint2*>(at(q.pairs));
+    v.chunks = reinterpret_cast<const uint4*>(at(q.chunks));
+    v.J4 = reinterpret_cast<V4*>(at(q.J4));
+    v.E4 = reinterpret_cast<V4*>(at(q.E4));
+    v.F4 = reinterpret_cast<V4*>(at(q.F4));
+    v.obs_pc = reinterpret_cast<T*>(at(q.obs_pc));
+    v.obs_valid = at(q.obs_valid);
+    v.pt_C = reinterpret_cast<double*>(at(q.pt_C));
+    v.pt_g = reinterpret_cast<double*>(at(q.pt_g));
+    v.pt_active = at(q.pt_active);
+    v.pt_cinv = reinterpret_cast<double*>(at(q.pt_cinv));
+    v.pt_ok = at(q.pt_ok);
+    v.S = reinterpret_cast<double*>(at(q.S));
+    v.delta = reinterpret_cast<double*>(at(q.delta));
+    double* st = reinterpret_cast<double*>(at(q.state));
+    for (int c = 0; c < 2; ++c) {
+      double* sb = st + (size_t)c * state_dbl[k];
+      v.poses[c] = sb;
+      v.cams[c] = sb + 7 * (size_t)P.n_img;
+      v.pts[c] = sb + 7 * (size_t)P.n_img + 8 * (size_t)P.n_cam;
+    }
+  }
+  (void)up_struct;
+  View<T>* d_views = arena.alloc<View<T>>(std::max<size_t>(nw, 1));
+  WindowResult* d_res = arena.alloc<WindowResult>(std::max<size_t>(nw, 1));
+  const uint64_t max_it = std::max<uint64_t>(prm->max_iterations, 1);
+  double* d_oc = arena.alloc<double>(std::max<size_t>(nw, 1) * max_it);
+  const auto t1 = clock::now();
+  if (nw) {
+    HIP_CHECK(hipMemcpyAsync(base, host.data(), state_end, hipMemcpyHostToDevice, sg.s));
+    HIP_CHECK(hipMemcpyAsync(d_views, views.data(), nw * sizeof(View<T>), hipMemcpyHostToDevice, sg.s));
+  }
+  HIP_CHECK(hipStreamSynchronize(sg.s));
+  const auto t2 = clock::now();
+  WindowParams wp{};
+  wp.max_iterations = prm->max_iterations;
+  wp.initial_lambda = prm->initial_lambda;
+  wp.loss_scale_px = prm->loss_scale_px;
+  wp.robust_loss = prm->robust_loss;
+  launch_window_lm<T>(d_views, (uint32_t)nw, max_dim, wp, d_res, d_oc, sg.s);
+  HIP_CHECK(hipStreamSynchronize(sg.s));
+  const auto t3 = clock::now();
+  std::vector<WindowResult> res(nw);
+  std::vector<double> oc(nw * max_it);
+  if (nw) {
+    HIP_CHECK(hipMemcpyAsync(res.data(), d_res, nw * sizeof(WindowResult), hipMemcpyDeviceToHost, sg.s));
+    HIP_CHECK(hipMemcpyAsync(oc.data(), d_oc, oc.size() * sizeof(double), hipMemcpyDeviceToHost, sg.s));
+    HIP_CHECK(hipMemcpyAsync(host.data() + up_struct, base + up_struct, state_end - up_struct,
+                             hipMemcpyDeviceToHost, sg.s));
+  }
+  HIP_CHECK(hipStreamSynchronize(sg.s));
+  const double kernel_s = std::chrono::duration<double>(t3 - t2).count();
+  const double upload_s = std::chrono::duration<double>(t2 - t0).count();
+  for (size_t k = 0; k < nw; ++k) {
+    const size_t w = win[k];
+    const Pack& P = packs[w];
+    const WindowResult& r = res[k];
+    const double* sb = reinterpret_cast<const double*>(host.data() + plan[k].state) + (size_t)r.cur * state_dbl[k];
+    sfmba_output* out = &outs[w];
+    for (uint32_t i = 0; i < P.n_img; ++i) {
+      const double* q = sb + 7 * (size_t)i;
+      out->poses[i] = sfmba_pose{q[0], q[1], q[2], q[3], q[4], q[5], q[6]};
+    }
+    const double* cs = sb + 7 * (size_t)P.n_img;
+    for (uint32_t c = 0; c < P.n_cam; ++c) {
+      out->cameras[c] = inputs[w].cameras[c];
+      const int kp = model_param_count_host(inputs[w].cameras[c].model_id);
+      for (int j = 0; j < kp; ++j) out->cameras[c].params[j] = cs[8 * (size_t)c + j];
+    }
+    if (P.n_pt) std::memcpy(out->points, cs + 8 * (size_t)P.n_cam, 24 * (size_t)P.n_pt);
+    out->initial_cost = r.initial_cost;
+    out->final_cost = r.final_cost;
+    out->iterations_run = r.iterations_run;
+    if (stats) {
+      sfmba_stats& st = stats[w];
+      st.dim = P.D;
+      st.num_linearizations = r.num_linearizations;
+      st.num_factorizations = r.num_factorizations;
+      st.num_cholesky_failures = r.num_cholesky_failures;
+      st.num_lu_solves = r.num_lu_solves;
+      st.num_rejected_steps = r.num_rejected_steps;
+      st.num_schur_pairs = P.pairs.size() / 2;
+      for (uint64_t i = 0; i < r.num_outer; ++i)
+        push_stat(st.outer_costs, st.outer_costs_capacity, &st.outer_costs_len, oc[k * max_it + i]);
+      st.time_total_s = kernel_s;
+      st.time_upload_s = upload_s;
+    }
+  }
+  // Windows too large for one workgroup: the single-problem path.
+  for (size_t w : big) {
+    std::unique_ptr<sfmba_problem> H(create_problem(&inputs[w], opt.device));
+    solve_problem(H.get(), prm, &opt, &outs[w], stats ? &stats[w] : nullptr);
+  }
+}
+
+}  // namespace sfmba
+
 using namespace sfmba;
 
 template <typename F>
@@ -1088,11 +1381,31 @@ int sfmba_bundle_adjust_batch(const sfmba_input* inputs, size_t count, const sfm
                               const sfmba_options* options, sfmba_output* outputs,
                               sfmba_stats* stats) {
   return guarded([&] {
-    if (count && (!inputs || !outputs)) throw Error(SFMBA_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (count && (!inputs || !outputs || !params)) throw Error(SFMBA_ERR_INVALID_ARGUMENT, "NULL argument");
+    sfmba_options o;
+    sfmba_options_default(&o);
+    if (options) o = *options;
+    if (o.precision != SFMBA_PRECISION_FP64 && o.precision != SFMBA_PRECISION_MIXED)
+      throw Error(SFMBA_ERR_INVALID_ARGUMENT, "unknown precision");
+    if (o.solver == SFMBA_SOLVER_PCG) throw Error(SFMBA_ERR_UNSUPPORTED, "PCG solver not built");
     for (size_t i = 0; i < count; ++i) {
-      const int rc = sfmba_bundle_adjust_ex(&inputs[i], params, options, &outputs[i], stats ? &stats[i] : nullptr);
-      if (rc != SFMBA_OK) throw Error(rc, "problem " + std::to_string(i) + ": " + g_last_error);
+      const sfmba_output& out = outputs[i];
+      if ((inputs[i].num_images && !out.poses) || (inputs[i].num_points && !out.points) ||
+          (inputs[i].num_cameras && !out.cameras))
+        throw Error(SFMBA_ERR_INVALID_ARGUMENT, "problem " + std::to_string(i) + ": output arrays are NULL");
+      if (stats) {
+        sfmba_stats z{};
+        z.outer_costs = stats[i].outer_costs;
+        z.outer_costs_capacity = stats[i].outer_costs_capacity;
+        z.trial_costs = stats[i].trial_costs;
+        z.trial_costs_capacity = stats[i].trial_costs_capacity;
+        stats[i] = z;
+      }
     }
+    if (o.precision == SFMBA_PRECISION_MIXED)
+      batch_solve<float>(inputs, count, params, o, outputs, stats);
+    else
+      batch_solve<double>(inputs, count, params, o, outputs, stats);
   });
 }
 
