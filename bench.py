@@ -1,19 +1,23 @@
 """Benchmark: bundle-adjustment time-to-converge on the 1,000-camera scene,
-mixed precision vs FP64 plain, on one MI355X.
+mixed precision vs FP64 plain, on one MI355X; and the 256-window batch.
 
 A step is one full Levenberg-Marquardt solve to convergence (the reference's
 stopping rule) of the 1,000-PINHOLE-camera / 500k-point / ~3M-observation
 scene (1% outliers, Huber 2 px), starting from the same perturbed state,
 with the problem already resident in HBM (created once, untimed).
 
-Prints one JSON line.  ``value`` is the mean mixed-precision time-to-converge
-in seconds (lower is better); the FP64 plain time, the speed-up, Jacobian
-and Schur-assembly throughput (observations/s), the converged RMS of both
-forms, the roofline of the dominant kernel group (the FP64 MFMA
-factorization) and of the Schur assembly, and a CPU baseline from the oracle
-are reported beside it.
+Prints one JSON line.  ``value`` is the mixed-precision time to converge to
+the FP64 plain solve, in seconds (lower is better): the mixed solve cut at
+the first iteration whose cost reaches the FP64 solve's final cost, timed
+as a solve of its own.  The mixed solve run to the reference's stopping
+rule, the FP64 plain time, the speed-up, Jacobian and Schur-assembly
+throughput (observations/s), residual statistics and pose error of both
+forms, the roofline of the dominant kernel (the factorization's trailing
+update, timed per launch with HIP events) with the whole factorization and
+the Schur assembly beside it, a CPU baseline from the oracle, and the
+256-window batched solve (windows/s) are reported beside it.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--scene c2] [--no-cpu-baseline]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--scene c2] [--no-cpu-baseline] [--no-windows]
 """
 from __future__ import annotations
 
@@ -81,6 +85,46 @@ def residual_stats(ba, inp, out, inlier_px=10.0):
             "inlier_rms_px": float(np.sqrt((inl ** 2).mean())), "inlier_fraction": float(len(inl) / len(n))}
 
 
+def windows_section(ba, scenes, threads, cpu=True):
+    """Config 3: 256 independent local-BA windows (10-40 SIMPLE_RADIAL
+    cameras, 2k-20k points each) solved by the persistent batched launch,
+    next to a sequential GPU loop (16-window sample) and the oracle on the
+    host (4-window sample)."""
+    wins = scenes.window_scenes(256)
+    inps = [w.inp for w in wins]
+    p = ba.BaParams(max_iterations=50)
+    res = {"count": len(inps), "observations": int(sum(len(i.observations) for i in inps))}
+    for prec in ("fp64", "mixed"):
+        ba.bundle_adjust_batch(inps[:8], p, precision=prec)
+        t = time.perf_counter()
+        outs = ba.bundle_adjust_batch(inps, p, precision=prec)
+        wall = time.perf_counter() - t
+        n = 16
+        ba.bundle_adjust(inps[0], p, precision=prec)
+        t = time.perf_counter()
+        seq = [ba.bundle_adjust(i, p, precision=prec) for i in inps[:n]]
+        seq_s = time.perf_counter() - t
+        res[prec] = {"batch_windows_per_s": len(inps) / wall,
+                     "batch_kernel_windows_per_s": len(inps) / outs[0].stats["time_total_s"],
+                     "sequential_gpu_windows_per_s": n / seq_s,
+                     "batch_wall_s": wall, "batch_kernel_s": outs[0].stats["time_total_s"],
+                     "mean_iterations": float(np.mean([o.iterations_run for o in outs])),
+                     "max_rel_cost_diff_vs_sequential": float(max(abs(a.final_cost - b.final_cost) / b.final_cost
+                                                                  for a, b in zip(outs[:n], seq)))}
+        log(f"windows {prec}: batch {res[prec]['batch_windows_per_s']:.1f}/s, "
+            f"sequential {res[prec]['sequential_gpu_windows_per_s']:.1f}/s")
+    if cpu:
+        import oracle
+        k = 4
+        t = time.perf_counter()
+        for i in inps[:k]:
+            oracle.bundle_adjust(i, p, nthreads=threads)
+        res["cpu_oracle_windows_per_s"] = k / (time.perf_counter() - t)
+        res["cpu_oracle_sample"] = f"first {k} windows, {threads} threads"
+        log(f"windows cpu oracle {res['cpu_oracle_windows_per_s']:.2f}/s")
+    return res
+
+
 def per_unit(stats, key, count_key):
     n = stats[count_key]
     return stats[key] / n if n else float("nan")
@@ -118,6 +162,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--scene", default="c2")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-windows", action="store_true", help="skip the 256-window batched section")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (the engine has no multi-GPU path); got --gpus {args.gpus}",
@@ -224,6 +269,10 @@ def main() -> int:
                   "peak": PEAK_HBM / 1e9, "unit": "GB/s", "frac": bytes_schur / t_as / PEAK_HBM,
                   "traffic": None, "algorithmic_bytes": bytes_schur}
 
+    windows = None
+    if not args.no_windows:
+        windows = windows_section(ba, scenes, min(16, os.cpu_count() or 1), cpu=not args.no_cpu_baseline)
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
@@ -231,14 +280,18 @@ def main() -> int:
         cpu = cpu_baseline(inp, params, prof_fp64, threads)
         log(f"cpu baseline estimate {cpu['value']:.1f}s")
 
+    # value: the mixed form's time to converge to the FP64 plain solve (its
+    # final cost), a timed solve cut at the first iteration that reaches it;
+    # the mixed solve run to the reference's own stopping rule is beside it.
+    value = reach_t if reach_t is not None else mixed_t
     line = {
         "metric": "BA time-to-converge (s) + Jac/Schur obs/s, mixed vs FP64 plain, 1000-cam scene",
-        "value": mixed_t,
+        "value": value,
         "unit": "s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": mixed_t * 1e3,
+        "ms_per_step": value * 1e3,
         "higher_is_better": False,
         "scaling": "strong",
         "vs_baseline": None,
@@ -247,8 +300,10 @@ def main() -> int:
         "config": {"workload": "1000 PINHOLE cameras, 500k points, %d observations (mean track 6), 1%% outliers, "
                                "Huber 2px, pose 0 fixed, principal points pinned, D=%d" % (n_obs, D),
                    "scene": args.scene, "max_iterations": 50},
+        "value_definition": "mixed-precision solve time to reach the FP64 plain solve's final cost",
         "fp64_time_to_converge_s": fp64_t,
-        "mixed_speedup_vs_fp64": fp64_t / mixed_t,
+        "mixed_time_to_stopping_rule_s": mixed_t,
+        "mixed_speedup_vs_fp64": fp64_t / value,
         "mixed_time_to_fp64_final_cost_s": reach_t,
         "mixed_iterations_to_fp64_final_cost": reach,
         "iterations": {"mixed": mixed_outs[-1].iterations_run, "fp64": fp64_outs[-1].iterations_run},
@@ -265,6 +320,7 @@ def main() -> int:
         "roofline_factorization": roof_fact,
         "roofline_schur": roof_schur,
         "cpu_baseline": cpu,
+        "windows": windows,
     }
     print(json.dumps(line))
     return 0

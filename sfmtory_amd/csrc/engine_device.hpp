@@ -316,11 +316,12 @@ constexpr int PART_STRIDE = 2 * 196 + 16;
 // step with all their loads issued together.  A chunk (ch.w == NONE) that
 // covers its whole destination is written straight into S (with the
 // Marquardt damping of the diagonal); others leave partials.
-template <typename T, int NQ>
+// UMAX: pairs in flight per step (the grid kernel's default measured best
+// on gfx950 at full occupancy).
+template <typename T, int NQ, int UMAX = (sizeof(T) == 4 ? 4 : 2)>
 __device__ inline void schur_chunk(const View<T>& v, const uint4 ch, double lambda, double* parts) {
   using V4 = typename Vec4<T>::type;
   constexpr uint32_t HI = 0x80000000u, BEFORE = 0x40000000u, OFF = 0x3FFFFFFFu;
-  constexpr int UMAX = sizeof(T) == 4 ? 4 : 2;  // measured best on gfx950
   constexpr int U = UMAX / NQ > 1 ? UMAX / NQ : 1;
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t ga = ch.x >> 16, gb = ch.x & 0xFFFFu;

@@ -1,6 +1,6 @@
 // Persistent batched solver for many small problems (local-BA windows).
 //
-// One workgroup (512 threads FP32, 256 FP64) runs the whole Levenberg-Marquardt solve of
+// One workgroup (1024 threads FP32, 512 FP64) runs the whole Levenberg-Marquardt solve of
 // one problem: the reference's bundle_adjust (crates/sfm-ba/src/lib.rs:
 // 747-1232) with the same schedule, acceptance test, damping and
 // Cholesky -> LU -> damping-bump chain as the single-problem path.  All
@@ -22,13 +22,19 @@
 
 namespace sfmba {
 
-// Workgroup size: 512 threads for the FP32 form (2 waves per SIMD), 256 for
-// FP64, whose linearization needs more than 256 registers per lane.  The
+// Workgroup size and Schur pairs in flight, measured on gfx950 over 256
+// windows of 10-40 cameras: 1024 threads / 8 pairs for the FP32 form, 512 /
+// 4 for FP64 (its linearization then spills a few registers, which costs less
+// than the lost parallelism of the gather-bound Schur reduction).  The
 // helpers stride by blockDim.x.
-constexpr int MAXWAVE = 8;
+constexpr int MAXWAVE = 16;
 template <typename T>
 constexpr int window_wg() {
-  return sizeof(T) == 4 ? 512 : 256;
+  return sizeof(T) == 4 ? 1024 : 512;
+}
+template <typename T>
+constexpr int window_pairs() {
+  return sizeof(T) == 4 ? 8 : 4;
 }
 #define WG ((uint32_t)blockDim.x)
 #define NWAVE ((int)(blockDim.x >> 6))
@@ -309,7 +315,7 @@ __device__ inline void wg_assemble(const View<T>& v, double lam) {
   for (uint32_t a = threadIdx.x; a < v.n_inc; a += WG) compute_F(v, a);
   __syncthreads();
   for (uint32_t ci = threadIdx.x >> 6; ci < v.n_chunks; ci += NWAVE)
-    schur_chunk<T, NQ>(v, v.chunks[ci], lam, nullptr);
+    schur_chunk<T, NQ, window_pairs<T>()>(v, v.chunks[ci], lam, nullptr);
   __syncthreads();
 }
 
