@@ -192,6 +192,7 @@ typedef struct sfmba_stats {
   double time_trailing_update_s;
   uint64_t count_trailing_update;
   double flop_trailing_update;
+  uint64_t num_pcg_iterations; /* solver = PCG: iterations over all tries */
 } sfmba_stats;
 
 /* bundle_adjust with engine options and diagnostics (stats may be NULL). */

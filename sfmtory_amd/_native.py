@@ -90,6 +90,7 @@ class Stats(C.Structure):
         ("time_update_s", C.c_double), ("count_linearize", C.c_uint64), ("count_schur", C.c_uint64),
         ("count_factor", C.c_uint64), ("time_trailing_update_s", C.c_double),
         ("count_trailing_update", C.c_uint64), ("flop_trailing_update", C.c_double),
+        ("num_pcg_iterations", C.c_uint64),
     ]
 
 

@@ -241,7 +241,8 @@ PRECISIONS = {"fp64": N.PRECISION_FP64, "mixed": N.PRECISION_MIXED}
 SOLVERS = {"cholesky": N.SOLVER_CHOLESKY, "pcg": N.SOLVER_PCG}
 
 
-def _options(precision, solver, device, profile, refine_steps=None) -> N.Options:
+def _options(precision, solver, device, profile, refine_steps=None, pcg_max_iterations=None,
+             pcg_tolerance=None) -> N.Options:
     lib = N.load_engine()
     o = N.Options()
     lib.sfmba_options_default(C.byref(o))
@@ -251,6 +252,10 @@ def _options(precision, solver, device, profile, refine_steps=None) -> N.Options
     o.profile = 1 if profile else 0
     if refine_steps is not None:
         o.refine_steps = int(refine_steps)
+    if pcg_max_iterations is not None:
+        o.pcg_max_iterations = int(pcg_max_iterations)
+    if pcg_tolerance is not None:
+        o.pcg_tolerance = float(pcg_tolerance)
     return o
 
 
@@ -284,14 +289,16 @@ class _Stats:
 
 def bundle_adjust(inp: BaInput, params: BaParams | None = None, *, precision: str = "fp64",
                   solver: str = "cholesky", device: int = 0, profile: bool = False,
-                  refine_steps: int | None = None) -> BaOutput:
-    """crates/sfm-ba/src/lib.rs:747 ``pub fn bundle_adjust``."""
+                  refine_steps: int | None = None, pcg_max_iterations: int | None = None,
+                  pcg_tolerance: float | None = None) -> BaOutput:
+    """crates/sfm-ba/src/lib.rs:747 ``pub fn bundle_adjust``.  ``solver="pcg"``
+    solves the reduced system by block-Jacobi PCG instead of factoring it."""
     params = params or BaParams()
     lib = N.load_engine()
     ci = _CInput(inp)
     outs = _Outputs(ci)
     st = _Stats()
-    opt = _options(precision, solver, device, profile, refine_steps)
+    opt = _options(precision, solver, device, profile, refine_steps, pcg_max_iterations, pcg_tolerance)
     N.check(lib.sfmba_bundle_adjust_ex(C.byref(ci.c), C.byref(_c_params(params)), C.byref(opt),
                                        C.byref(outs.c), C.byref(st.c)))
     return BaOutput(outs.poses, outs.points, outs.cams, outs.c.initial_cost, outs.c.final_cost,
@@ -312,11 +319,12 @@ class Problem:
         self.device = device
 
     def solve(self, params: BaParams | None = None, *, precision: str = "fp64", solver: str = "cholesky",
-              profile: bool = False, refine_steps: int | None = None) -> BaOutput:
+              profile: bool = False, refine_steps: int | None = None, pcg_max_iterations: int | None = None,
+              pcg_tolerance: float | None = None) -> BaOutput:
         params = params or BaParams()
         outs = _Outputs(self._ci)
         st = _Stats()
-        opt = _options(precision, solver, self.device, profile, refine_steps)
+        opt = _options(precision, solver, self.device, profile, refine_steps, pcg_max_iterations, pcg_tolerance)
         N.check(self._lib.sfmba_problem_solve(self._h, C.byref(_c_params(params)), C.byref(opt), C.byref(outs.c),
                                               C.byref(st.c)))
         return BaOutput(outs.poses, outs.points, outs.cams, outs.c.initial_cost, outs.c.final_cost,
@@ -341,7 +349,7 @@ class Problem:
 
 
 def bundle_adjust_batch(inputs: Sequence[BaInput], params: BaParams | None = None, *,
-                        precision: str = "fp64", device: int = 0) -> list:
+                        precision: str = "fp64", device: int = 0, solver: str = "cholesky") -> list:
     """Many independent problems (local-BA windows) in one batched call."""
     params = params or BaParams()
     lib = N.load_engine()
@@ -351,7 +359,7 @@ def bundle_adjust_batch(inputs: Sequence[BaInput], params: BaParams | None = Non
     arr_in = (N.Input * len(cis))(*[ci.c for ci in cis])
     arr_out = (N.Output * len(cis))(*[o.c for o in outs])
     arr_st = (N.Stats * len(cis))(*[s.c for s in stats])
-    opt = _options(precision, "cholesky", device, False)
+    opt = _options(precision, solver, device, False)
     N.check(lib.sfmba_bundle_adjust_batch(arr_in, len(cis), C.byref(_c_params(params)), C.byref(opt),
                                           arr_out, arr_st))
     res = []

@@ -22,7 +22,7 @@ LIB = PKG / "lib"
 OBJ = PKG / "_build"
 
 HIP_SOURCES = ["engine_host.hip", "engine_kernels.hip", "dense_kernels.hip", "intrinsics_kernels.hip",
-               "window_kernels.hip"]
+               "window_kernels.hip", "pcg_kernels.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

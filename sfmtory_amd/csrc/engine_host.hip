@@ -22,6 +22,7 @@
 #include "camera_models.hpp"
 #include "engine.hpp"
 #include "engine_device.hpp"
+#include "pcg.hpp"
 #include "window.hpp"
 
 namespace sfmba {
@@ -478,11 +479,14 @@ struct sfmba_problem {
   double* vec3 = nullptr;
   int* piv = nullptr;
   double* pinned = nullptr;
+  sfmba::PcgBuffers pcg;
+  bool have_pcg = false;
   double pack_s = 0.0, upload_s = 0.0;
   ~sfmba_problem() {
     // device buffers (pr.arena) are freed after this body, by member destruction
     if (s) (void)hipStreamSynchronize(s);
     if (pinned) (void)hipHostFree(pinned);
+    if (pcg.host) (void)hipHostFree(pcg.host);
     if (s) (void)hipStreamDestroy(s);
   }
 };
@@ -518,6 +522,36 @@ static sfmba_problem* create_problem(const sfmba_input* in, int device) {
   H->pack_s = std::chrono::duration<double>(t1 - t0).count();
   H->upload_s = std::chrono::duration<double>(clock::now() - t1).count();
   return H.release();
+}
+
+// PCG work buffers, allocated on the first PCG solve of a problem.
+static void alloc_pcg(sfmba_problem& H) {
+  if (H.have_pcg) return;
+  const Layout& L = H.pr.L;
+  DevArena& A = H.pr.arena;
+  PcgBuffers& B = H.pcg;
+  std::vector<uint32_t> moff(L.n_grp), row_grp(std::max<uint32_t>(L.D, 1), 0);
+  uint32_t m = 0;
+  for (uint32_t g = 0; g < L.n_grp; ++g) {
+    moff[g] = m;
+    m += L.h_grp_dim[g] * L.h_grp_dim[g];
+    for (uint32_t r = 0; r < L.h_grp_dim[g]; ++r) row_grp[L.h_grp_off[g] + r] = g;
+  }
+  B.Minv = A.alloc<double>(std::max<uint32_t>(m, 1));
+  B.moff = A.upload(moff, H.s);
+  B.row_grp = A.upload(row_grp, H.s);
+  const size_t n = std::max<uint32_t>(L.Dp, 1);
+  B.r = A.alloc<double>(n);
+  B.z = A.alloc<double>(n);
+  B.p = A.alloc<double>(n);
+  B.q = A.alloc<double>(n);
+  B.y2 = A.alloc<double>(n);
+  B.symv_part = A.alloc<double>(std::max<size_t>(pcg_symv_part_doubles(L.D), 1));
+  B.dot_part = A.alloc<double>(pcg_dot_part_doubles(L.D));
+  B.scal = A.alloc<double>(4);
+  HIP_CHECK(hipHostMalloc((void**)&B.host, 8 * sizeof(double), hipHostMallocDefault));
+  HIP_CHECK(hipStreamSynchronize(H.s));
+  H.have_pcg = true;
 }
 
 template <typename T>
@@ -573,6 +607,8 @@ static void solve(sfmba_problem& H, Work<T>& W, const sfmba_params* prm, const s
   if (mixed && opt->refine_steps > 0 && L.D && !W.S_copy)
     W.S_copy = H.pr.arena.alloc<double>((size_t)L.ld * L.Dp);
   const bool refine = mixed && opt->refine_steps > 0 && L.D;
+  const bool pcg = opt->solver == SFMBA_SOLVER_PCG;
+  if (pcg && L.D) alloc_pcg(H);
   double* h = H.pinned;
   const int loss = prm->robust_loss;
   const double scale = prm->loss_scale_px;
@@ -618,6 +654,24 @@ static void solve(sfmba_problem& H, Work<T>& W, const sfmba_params* prm, const s
         dense_mirror_copy(W.S, W.S_copy, L.Dp, L.ld, s);
         dense_extract_row(W.S, L.Dp, L.ld, L.Dp, W.rhs, s);
       }
+      if (pcg) {
+        // Iterative solve of the damped reduced system (no factorization).
+        prof.begin(3);
+        const PcgResult pr = pcg_solve(H.pcg, W.S, L.ld, L.D, L.Dp, L.n_grp, L.grp_off, L.grp_dim,
+                                       opt->pcg_max_iterations, opt->pcg_tolerance, W.delta, W.flags,
+                                       H.pcg.host, s);
+        prof.end();
+        if (st) st->num_pcg_iterations += pr.iterations;
+        if (!pr.ok) return true;
+        prof.begin(4);
+        launch_backsub_update<T>(L, W, cur, s);
+        launch_cost(L, W.poses[1 - cur], W.cams[1 - cur], W.pts[1 - cur], loss, scale, W.cost_part, W.cost, 0, s);
+        prof.end();
+        HIP_CHECK(hipMemcpyAsync(h, W.cost, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+        trial = h[0];
+        return false;
+      }
       prof.begin(2);
       HIP_CHECK(hipMemsetAsync(W.flags, 0, 4 * sizeof(int), s));
       dense_cholesky(W.S, W.Linv, L.Dp, L.ld, L.D, W.flags, s, prof.on ? &prof.trailing : nullptr);
@@ -658,6 +712,7 @@ static void solve(sfmba_problem& H, Work<T>& W, const sfmba_params* prm, const s
       const double lam = mixed ? std::max(lam_lm, mixed_floor) : lam_lm;
       if (!assemble_factor_step(lam, trial)) return true;
       if (st) st->num_cholesky_failures++;
+      if (pcg) return false;  // breakdown: the damping bump takes over
       if (mixed) {
         for (double shift : {1e-6, 1e-5, 1e-4, 1e-3, 1e-2}) {
           if (shift <= lam) continue;
@@ -770,7 +825,8 @@ static void solve_problem(sfmba_problem* H, const sfmba_params* params, const sf
   }
   if (o.solver != SFMBA_SOLVER_CHOLESKY && o.solver != SFMBA_SOLVER_PCG)
     throw Error(SFMBA_ERR_INVALID_ARGUMENT, "unknown solver");
-  if (o.solver == SFMBA_SOLVER_PCG) throw Error(SFMBA_ERR_UNSUPPORTED, "PCG solver not built");
+  if (o.solver == SFMBA_SOLVER_PCG && !(o.pcg_max_iterations > 0 && o.pcg_tolerance >= 0.0))
+    throw Error(SFMBA_ERR_INVALID_ARGUMENT, "PCG needs pcg_max_iterations > 0 and pcg_tolerance >= 0");
   if (o.precision == SFMBA_PRECISION_MIXED) {
     if (!H->have32) {
       alloc_work<float>(*H, H->w32);
@@ -1387,7 +1443,8 @@ int sfmba_bundle_adjust_batch(const sfmba_input* inputs, size_t count, const sfm
     if (options) o = *options;
     if (o.precision != SFMBA_PRECISION_FP64 && o.precision != SFMBA_PRECISION_MIXED)
       throw Error(SFMBA_ERR_INVALID_ARGUMENT, "unknown precision");
-    if (o.solver == SFMBA_SOLVER_PCG) throw Error(SFMBA_ERR_UNSUPPORTED, "PCG solver not built");
+    if (o.solver == SFMBA_SOLVER_PCG)
+      throw Error(SFMBA_ERR_UNSUPPORTED, "batched windows are solved by Cholesky only");
     for (size_t i = 0; i < count; ++i) {
       const sfmba_output& out = outputs[i];
       if ((inputs[i].num_images && !out.poses) || (inputs[i].num_points && !out.points) ||
