@@ -14,10 +14,11 @@ rule, the FP64 plain time, the speed-up, Jacobian and Schur-assembly
 throughput (observations/s), residual statistics and pose error of both
 forms, the roofline of the dominant kernel (the factorization's trailing
 update, timed per launch with HIP events) with the whole factorization and
-the Schur assembly beside it, a CPU baseline from the oracle, and the
-256-window batched solve (windows/s) are reported beside it.
+the Schur assembly beside it, a CPU baseline from the oracle, the
+256-window batched solve (windows/s) and the 5,000-camera scene solved by
+PCG and by Cholesky are reported beside it.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--scene c2] [--no-cpu-baseline] [--no-windows]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--scene c2] [--no-cpu-baseline] [--no-windows] [--no-large]
 """
 from __future__ import annotations
 
@@ -125,6 +126,42 @@ def windows_section(ba, scenes, threads, cpu=True):
     return res
 
 
+def large_section(ba, scenes):
+    """Config 4: 5,000 RADIAL cameras, 2M points (D ~ 45k): the reduced
+    system solved by block-Jacobi PCG vs the FP64 MFMA Cholesky, FP64 and
+    mixed, one solve each to the reference's stopping rule."""
+    sc = scenes.generate("c4")
+    t = time.perf_counter()
+    prob = ba.Problem(sc.inp)
+    res = {"observations": len(sc.inp.observations), "problem_create_s": time.perf_counter() - t}
+    p = ba.BaParams(max_iterations=50)
+    for solver in ("pcg", "cholesky"):
+        for prec in ("fp64", "mixed"):
+            t = time.perf_counter()
+            out = prob.solve(p, precision=prec, solver=solver, profile=True)
+            wall = time.perf_counter() - t
+            st = out.stats
+            r = {"time_to_converge_s": wall, "iterations": out.iterations_run, "tries": st["num_factorizations"],
+                 "final_cost": out.final_cost, "pcg_iterations": st["num_pcg_iterations"],
+                 "schur_assembly_ms": per_unit(st, "time_schur_s", "count_schur") * 1e3}
+            if solver == "cholesky":
+                r["factor_ms"] = per_unit(st, "time_factor_s", "count_factor") * 1e3
+                r["trailing_update_tflops"] = st["flop_trailing_update"] / st["time_trailing_update_s"] / 1e12
+            else:
+                r["solve_ms_per_try"] = st["time_solve_s"] / max(1, st["num_factorizations"]) * 1e3
+            res[f"{solver}_{prec}"] = r
+            res["dim"] = st["dim"]
+            log(f"large scene {solver} {prec}: {wall:.2f}s, {out.iterations_run} iterations, cost {out.final_cost:.6f}")
+    D = res["dim"]
+    f = res["cholesky_fp64"]["factor_ms"] * 1e-3
+    res["roofline_factorization"] = {"bound": "mfma", "achieved": (D ** 3 / 3.0 + 2.0 * D ** 2) / f / 1e12,
+                                     "peak": PEAK_FP64_MFMA / 1e12, "unit": "TFLOP/s",
+                                     "frac": (D ** 3 / 3.0 + 2.0 * D ** 2) / f / PEAK_FP64_MFMA}
+    res["pcg_speedup_vs_cholesky_fp64"] = res["cholesky_fp64"]["time_to_converge_s"] / res["pcg_fp64"]["time_to_converge_s"]
+    prob.close()
+    return res
+
+
 def per_unit(stats, key, count_key):
     n = stats[count_key]
     return stats[key] / n if n else float("nan")
@@ -163,6 +200,7 @@ def main() -> int:
     ap.add_argument("--scene", default="c2")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-windows", action="store_true", help="skip the 256-window batched section")
+    ap.add_argument("--no-large", action="store_true", help="skip the 5,000-camera PCG vs Cholesky section")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (the engine has no multi-GPU path); got --gpus {args.gpus}",
@@ -273,6 +311,11 @@ def main() -> int:
     if not args.no_windows:
         windows = windows_section(ba, scenes, min(16, os.cpu_count() or 1), cpu=not args.no_cpu_baseline)
 
+    large = None
+    if not args.no_large:
+        prob.close()
+        large = large_section(ba, scenes)
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
@@ -321,6 +364,7 @@ def main() -> int:
         "roofline_schur": roof_schur,
         "cpu_baseline": cpu,
         "windows": windows,
+        "large_scene": large,
     }
     print(json.dumps(line))
     return 0
