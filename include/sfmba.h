@@ -223,6 +223,61 @@ int sfmba_bundle_adjust_batch(const sfmba_input* inputs, size_t count,
                               const sfmba_options* options,
                               sfmba_output* outputs, sfmba_stats* stats);
 
+/* ------------------------------------------------ caller solve policy
+ * The solve part of the reference's caller, from the assembled problem on.
+ *
+ * Replaces: fn filter_and_reoptimize
+ * (crates/sfm-reconstruction/src/lib.rs:1710-1768): bundle_adjust, then up
+ * to two rounds of dropping the observations behind their camera or with
+ * plain residual norm > threshold_px and re-solving the survivors from the
+ * current state; stops when nothing was dropped or fewer than 12
+ * observations remain.  *num_observations_kept (may be NULL) receives the
+ * survivor count. */
+int sfmba_filter_and_reoptimize(const sfmba_input* input, const sfmba_params* params,
+                                const sfmba_options* options, double threshold_px,
+                                sfmba_output* out, uint64_t* num_observations_kept);
+
+/* IntrinsicsMode (crates/sfm-reconstruction/src/lib.rs:1301-1315). */
+#define SFMBA_INTRINSICS_FIXED 0        /* intrinsics held */
+#define SFMBA_INTRINSICS_FREE 1         /* one joint solve, cameras with >= 5 images free */
+#define SFMBA_INTRINSICS_FREE_GUARDED 2 /* fixed vs free solves, filtered, guarded */
+
+typedef struct sfmba_policy {
+  int32_t intrinsics;                 /* SFMBA_INTRINSICS_* (default FREE_GUARDED) */
+  int32_t robust_loss;                /* loss of every solve (default Huber) */
+  double max_reprojection_error_px;   /* outlier threshold of the guarded pass (default 4) */
+  const uint8_t* camera_caller_fixed; /* per camera: intrinsics the caller pinned */
+  size_t num_camera_caller_fixed;     /* shorter = the rest not pinned */
+} sfmba_policy;
+
+typedef struct sfmba_policy_report {
+  int32_t chose_free;            /* the free-intrinsics result was kept */
+  int32_t distortion_plausible;  /* every |distortion coefficient| < 0.5 (free result) */
+  double fixed_mean_error_px;    /* plain mean reprojection error of each candidate, */
+  double free_mean_error_px;     /* over all of the problem's observations */
+  uint64_t observations_kept_fixed, observations_kept_free;
+  uint64_t filter_rounds_fixed, filter_rounds_free;
+} sfmba_policy_report;
+
+void sfmba_policy_default(sfmba_policy* p);
+
+/* Replaces: the solve of fn run_bundle_adjustment
+ * (crates/sfm-reconstruction/src/lib.rs:1479-1695) on the problem the
+ * caller assembled for its scope (images, points, observations and the
+ * fixed poses of the gauge and of pinned extrinsics).  The input's
+ * fixed_cameras and fixed_camera_params are replaced by the policy's:
+ * principal points pinned (default_fixed_params_mask); FIXED holds every
+ * camera; FREE frees the cameras with >= 5 images that the caller did not
+ * pin, in one 50-iteration solve; FREE_GUARDED (200 iterations) solves
+ * with all cameras fixed and, when a camera has >= 5 images, with those
+ * cameras free -- concurrently, each through filter_and_reoptimize -- and
+ * keeps the free result only if its distortion is plausible and its plain
+ * mean reprojection error is lower.  A problem with fewer than 2 images or
+ * no observations is returned unchanged.  report may be NULL. */
+int sfmba_run_bundle_adjustment(const sfmba_input* input, const sfmba_policy* policy,
+                                const sfmba_options* options, sfmba_output* out,
+                                sfmba_policy_report* report);
+
 /* ------------------------------------------------ mean reprojection error
  * Replaces: pub fn mean_reprojection_error(input: &BaInput) -> f64
  * (crates/sfm-ba/src/lib.rs:708-725): mean of the plain residual norm over

@@ -20,10 +20,12 @@ from .ba import (  # noqa: F401
     cameras_to_array,
     default_fixed_params_mask,
     device_count,
+    filter_and_reoptimize,
     mean_reprojection_error,
     observation_jacobians,
     observations_to_array,
     poses_to_array,
     refine_intrinsics,
+    run_bundle_adjustment,
     total_cost,
 )

@@ -21,6 +21,9 @@ PRECISION_FP64 = 0
 PRECISION_MIXED = 1
 SOLVER_CHOLESKY = 0
 SOLVER_PCG = 1
+INTRINSICS_FIXED = 0
+INTRINSICS_FREE = 1
+INTRINSICS_FREE_GUARDED = 2
 
 CAMERA_DTYPE = np.dtype([("model_id", "<i4"), ("reserved", "<i4"), ("params", "<f8", (8,))])
 POSE_DTYPE = np.dtype([("qw", "<f8"), ("qx", "<f8"), ("qy", "<f8"), ("qz", "<f8"),
@@ -94,6 +97,18 @@ class Stats(C.Structure):
     ]
 
 
+class Policy(C.Structure):
+    _fields_ = [("intrinsics", C.c_int32), ("robust_loss", C.c_int32), ("max_reprojection_error_px", C.c_double),
+                ("camera_caller_fixed", C.c_void_p), ("num_camera_caller_fixed", C.c_size_t)]
+
+
+class PolicyReport(C.Structure):
+    _fields_ = [("chose_free", C.c_int32), ("distortion_plausible", C.c_int32),
+                ("fixed_mean_error_px", C.c_double), ("free_mean_error_px", C.c_double),
+                ("observations_kept_fixed", C.c_uint64), ("observations_kept_free", C.c_uint64),
+                ("filter_rounds_fixed", C.c_uint64), ("filter_rounds_free", C.c_uint64)]
+
+
 class IntrinsicsParams(C.Structure):
     _fields_ = [("max_iterations", C.c_uint64), ("initial_lambda", C.c_double),
                 ("robust_loss", C.c_int32), ("reserved", C.c_int32), ("loss_scale_px", C.c_double)]
@@ -123,6 +138,11 @@ SIGNATURES = {
                                               C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "sfmba_dense_solve": (C.c_int, [C.c_void_p, C.c_size_t, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p,
                                     C.POINTER(C.c_int32)]),
+    "sfmba_filter_and_reoptimize": (C.c_int, [C.POINTER(Input), C.POINTER(Params), C.POINTER(Options), C.c_double,
+                                              C.POINTER(Output), C.POINTER(C.c_uint64)]),
+    "sfmba_policy_default": (None, [C.POINTER(Policy)]),
+    "sfmba_run_bundle_adjustment": (C.c_int, [C.POINTER(Input), C.POINTER(Policy), C.POINTER(Options),
+                                              C.POINTER(Output), C.POINTER(PolicyReport)]),
     "sfmba_device_count": (C.c_int, []),
     "sfmba_last_error": (C.c_char_p, []),
 }

@@ -370,6 +370,51 @@ def bundle_adjust_batch(inputs: Sequence[BaInput], params: BaParams | None = Non
     return res
 
 
+INTRINSICS = {"fixed": N.INTRINSICS_FIXED, "free": N.INTRINSICS_FREE, "free_guarded": N.INTRINSICS_FREE_GUARDED}
+
+
+def filter_and_reoptimize(inp: BaInput, params: BaParams | None = None, threshold_px: float = 4.0, *,
+                          precision: str = "fp64", device: int = 0) -> tuple:
+    """crates/sfm-reconstruction/src/lib.rs:1710-1768.  Returns (BaOutput,
+    number of observations kept)."""
+    params = params or BaParams()
+    lib = N.load_engine()
+    ci = _CInput(inp)
+    outs = _Outputs(ci)
+    kept = C.c_uint64()
+    opt = _options(precision, "cholesky", device, False)
+    N.check(lib.sfmba_filter_and_reoptimize(C.byref(ci.c), C.byref(_c_params(params)), C.byref(opt),
+                                            float(threshold_px), C.byref(outs.c), C.byref(kept)))
+    return (BaOutput(outs.poses, outs.points, outs.cams, outs.c.initial_cost, outs.c.final_cost,
+                     int(outs.c.iterations_run), {}), int(kept.value))
+
+
+def run_bundle_adjustment(inp: BaInput, intrinsics: str = "free_guarded", *,
+                          robust_loss: RobustLoss = RobustLoss.HUBER, max_reprojection_error_px: float = 4.0,
+                          camera_caller_fixed=(), precision: str = "fp64", device: int = 0) -> tuple:
+    """The solve of crates/sfm-reconstruction/src/lib.rs:1479-1695 on an
+    assembled problem (its fixed_cameras / fixed_camera_params are set by the
+    policy).  Returns (BaOutput, report dict)."""
+    lib = N.load_engine()
+    ci = _CInput(inp)
+    outs = _Outputs(ci)
+    pol = N.Policy()
+    lib.sfmba_policy_default(C.byref(pol))
+    pol.intrinsics = INTRINSICS[intrinsics]
+    pol.robust_loss = int(robust_loss)
+    pol.max_reprojection_error_px = float(max_reprojection_error_px)
+    cf = np.ascontiguousarray(np.asarray(camera_caller_fixed, dtype=np.uint8))
+    pol.camera_caller_fixed = N.ptr(cf)
+    pol.num_camera_caller_fixed = cf.size
+    rep = N.PolicyReport()
+    opt = _options(precision, "cholesky", device, False)
+    N.check(lib.sfmba_run_bundle_adjustment(C.byref(ci.c), C.byref(pol), C.byref(opt), C.byref(outs.c),
+                                            C.byref(rep)))
+    report = {name: getattr(rep, name) for name, _ in N.PolicyReport._fields_}
+    return (BaOutput(outs.poses, outs.points, outs.cams, outs.c.initial_cost, outs.c.final_cost,
+                     int(outs.c.iterations_run), {}), report)
+
+
 def mean_reprojection_error(inp: BaInput) -> float:
     """crates/sfm-ba/src/lib.rs:708-725."""
     lib = N.load_engine()

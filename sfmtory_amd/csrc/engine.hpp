@@ -152,6 +152,10 @@ template <typename T>
 void launch_backsub_update(const Layout& L, Work<T>& W, int cur, hipStream_t s);
 void launch_cost(const Layout& L, const double* poses, const double* cams, const double* pts,
                  int loss, double scale, double* part, double* out, int mode, hipStream_t s);
+// |residual| and in-front flag per observation, in the caller's order.
+void launch_residual_norms(uint32_t n_obs, const sfmba_observation* obs, const uint64_t* cam_of_img,
+                           const int32_t* cam_model, const double* poses, const double* cams,
+                           const double* pts, double* norm, uint8_t* in_front, hipStream_t s);
 void launch_jacobians_debug(const Layout& L, const double* poses, const double* cams,
                             const double* pts, int fp32, double* r, double* jp, double* jx,
                             double* jc, uint8_t* valid, hipStream_t s);

@@ -1139,6 +1139,309 @@ static void batch_solve(const sfmba_input* inputs, size_t count, const sfmba_par
 
 }  // namespace sfmba
 
+// ------------------------------------------------------- caller policy
+// The solve policy of the reference's caller (crates/sfm-reconstruction/
+// src/lib.rs:1479-1768) on an assembled problem: the intrinsics modes, the
+// fixed- vs free-intrinsics solves (run concurrently, each on its own
+// stream), the residual filter of filter_and_reoptimize (residual norms on
+// the device, compaction on the host), the distortion plausibility bound and
+// the plain mean-error comparison.
+namespace sfmba {
+
+// An owned copy of a problem, from which filtered or re-seeded inputs are made.
+struct OwnedInput {
+  std::vector<uint64_t> coi;
+  std::vector<sfmba_camera> cams;
+  std::vector<sfmba_pose> poses;
+  std::vector<double> pts;
+  std::vector<sfmba_observation> obs;
+  std::vector<uint8_t> fp, fc, fcp;
+  std::vector<size_t> fcp_len;
+  sfmba_input view() const {
+    sfmba_input v{};
+    v.num_images = poses.size();
+    v.num_cameras = cams.size();
+    v.num_points = pts.size() / 3;
+    v.num_observations = obs.size();
+    v.camera_of_image = coi.data();
+    v.num_camera_of_image = coi.size();
+    v.cameras = cams.data();
+    v.poses = poses.data();
+    v.points = pts.data();
+    v.observations = obs.data();
+    v.fixed_poses = fp.data();
+    v.num_fixed_poses = fp.size();
+    v.fixed_cameras = fc.data();
+    v.num_fixed_cameras = fc.size();
+    v.fixed_camera_params = fcp.data();
+    v.fixed_camera_params_len = fcp_len.data();
+    v.num_fixed_camera_params = fcp_len.size();
+    return v;
+  }
+  void set_state(const sfmba_output& o) {
+    std::copy(o.poses, o.poses + poses.size(), poses.begin());
+    std::copy(o.points, o.points + pts.size(), pts.begin());
+    std::copy(o.cameras, o.cameras + cams.size(), cams.begin());
+  }
+};
+
+static OwnedInput own_input(const sfmba_input* in) {
+  OwnedInput c;
+  c.coi.assign(in->camera_of_image, in->camera_of_image + in->num_images);
+  c.cams.assign(in->cameras, in->cameras + in->num_cameras);
+  c.poses.assign(in->poses, in->poses + in->num_images);
+  c.pts.assign(in->points, in->points + 3 * in->num_points);
+  c.obs.assign(in->observations, in->observations + in->num_observations);
+  if (in->num_fixed_poses) c.fp.assign(in->fixed_poses, in->fixed_poses + in->num_fixed_poses);
+  if (in->num_fixed_cameras) c.fc.assign(in->fixed_cameras, in->fixed_cameras + in->num_fixed_cameras);
+  size_t m = 0;
+  for (size_t k = 0; k < in->num_fixed_camera_params; ++k) m += in->fixed_camera_params_len[k];
+  if (m) c.fcp.assign(in->fixed_camera_params, in->fixed_camera_params + m);
+  if (in->num_fixed_camera_params)
+    c.fcp_len.assign(in->fixed_camera_params_len, in->fixed_camera_params_len + in->num_fixed_camera_params);
+  return c;
+}
+
+// Output arrays owned by the caller of the policy helpers.
+struct OwnedOutput {
+  std::vector<sfmba_pose> poses;
+  std::vector<double> pts;
+  std::vector<sfmba_camera> cams;
+  sfmba_output o{};
+  explicit OwnedOutput(const OwnedInput& in) : poses(in.poses.size()), pts(in.pts.size()), cams(in.cams.size()) {
+    o.poses = poses.data();
+    o.points = pts.data();
+    o.cameras = cams.data();
+  }
+};
+
+static void solve_once(const sfmba_input* in, const sfmba_params* p, const sfmba_options* o, sfmba_output* out) {
+  std::unique_ptr<sfmba_problem> H(create_problem(in, o->device));
+  solve_problem(H.get(), p, o, out, nullptr);
+}
+
+static double mean_error(const sfmba_input* in, int device) {
+  StreamGuard sg(device);
+  Problem pr;
+  Pack P;
+  double *po, *ca, *pt;
+  upload_state(in, pr, P, sg.s, &po, &ca, &pt);
+  double* part = pr.arena.alloc<double>(2 * ((size_t)pr.L.n_obs / 2048 + 2));
+  double* res = pr.arena.alloc<double>(2);
+  launch_cost(pr.L, po, ca, pt, SFMBA_LOSS_NONE, 1.0, part, res, 1, sg.s);
+  double h[2];
+  HIP_CHECK(hipMemcpyAsync(h, res, sizeof(h), hipMemcpyDeviceToHost, sg.s));
+  HIP_CHECK(hipStreamSynchronize(sg.s));
+  return h[1] == 0.0 ? 0.0 : h[0] / h[1];
+}
+
+// Observations that survive the filter (in front, |r| <= threshold) under
+// the state held by `st`.
+static std::vector<uint8_t> filter_mask(const OwnedInput& st, double threshold, int device) {
+  const size_t n = st.obs.size();
+  std::vector<uint8_t> keep(n, 0);
+  if (n == 0) return keep;
+  StreamGuard sg(device);
+  DevArena A;
+  std::vector<int32_t> model(st.cams.size());
+  std::vector<double> params(8 * st.cams.size());
+  for (size_t c = 0; c < st.cams.size(); ++c) {
+    model[c] = st.cams[c].model_id;
+    std::copy(st.cams[c].params, st.cams[c].params + 8, params.begin() + 8 * c);
+  }
+  std::vector<double> poses(7 * st.poses.size());
+  for (size_t i = 0; i < st.poses.size(); ++i) {
+    const sfmba_pose& q = st.poses[i];
+    const double v[7] = {q.qw, q.qx, q.qy, q.qz, q.tx, q.ty, q.tz};
+    std::copy(v, v + 7, poses.begin() + 7 * i);
+  }
+  const sfmba_observation* d_obs = A.upload(st.obs, sg.s);
+  const uint64_t* d_coi = A.upload(st.coi, sg.s);
+  const int32_t* d_model = A.upload(model, sg.s);
+  const double* d_cams = A.upload(params, sg.s);
+  const double* d_poses = A.upload(poses, sg.s);
+  const double* d_pts = A.upload(st.pts, sg.s);
+  double* d_norm = A.alloc<double>(n);
+  uint8_t* d_front = A.alloc<uint8_t>(n);
+  launch_residual_norms((uint32_t)n, d_obs, d_coi, d_model, d_poses, d_cams, d_pts, d_norm, d_front, sg.s);
+  std::vector<double> norm(n);
+  HIP_CHECK(hipMemcpyAsync(norm.data(), d_norm, n * sizeof(double), hipMemcpyDeviceToHost, sg.s));
+  HIP_CHECK(hipMemcpyAsync(keep.data(), d_front, n, hipMemcpyDeviceToHost, sg.s));
+  HIP_CHECK(hipStreamSynchronize(sg.s));
+  for (size_t o = 0; o < n; ++o) keep[o] = keep[o] && norm[o] <= threshold;
+  return keep;
+}
+
+// filter_and_reoptimize (lib.rs:1710-1768): solve, then up to two rounds of
+// dropping observations behind their camera or with |r| > threshold and
+// re-solving from the current state; stops when nothing was dropped or
+// fewer than 12 observations remain.
+static void filter_and_reoptimize(const sfmba_input* in, const sfmba_params* p, const sfmba_options* o,
+                                  double threshold, sfmba_output* out, uint64_t* kept, uint64_t* rounds) {
+  OwnedInput cur = own_input(in);
+  solve_once(in, p, o, out);
+  uint64_t nr = 0;
+  for (int round = 0; round < 2; ++round) {
+    OwnedInput st = cur;
+    st.set_state(*out);
+    const std::vector<uint8_t> keep = filter_mask(st, threshold, o->device);
+    std::vector<sfmba_observation> surv;
+    surv.reserve(st.obs.size());
+    for (size_t k = 0; k < st.obs.size(); ++k)
+      if (keep[k]) surv.push_back(st.obs[k]);
+    if (surv.size() == st.obs.size() || surv.size() < 12) break;
+    st.obs = std::move(surv);
+    cur = std::move(st);
+    const sfmba_input v = cur.view();
+    solve_once(&v, p, o, out);
+    ++nr;
+  }
+  if (kept) *kept = cur.obs.size();
+  if (rounds) *rounds = nr;
+}
+
+// |distortion coefficient| < 0.5 for every camera (lib.rs:1674-1679, with
+// CameraModel::opencv_distortion, camera.rs:192-200).
+static bool distortion_plausible(const sfmba_camera* cams, size_t n) {
+  for (size_t c = 0; c < n; ++c) {
+    const double* q = cams[c].params;
+    double d[4] = {0, 0, 0, 0};
+    int nd = 0;
+    switch (cams[c].model_id) {
+      case SFMBA_SIMPLE_RADIAL: d[0] = q[3]; nd = 1; break;
+      case SFMBA_RADIAL: d[0] = q[3]; d[1] = q[4]; nd = 2; break;
+      case SFMBA_OPENCV:
+      case SFMBA_OPENCV_FISHEYE: d[0] = q[4]; d[1] = q[5]; d[2] = q[6]; d[3] = q[7]; nd = 4; break;
+      default: break;
+    }
+    for (int k = 0; k < nd; ++k)
+      if (!(std::fabs(d[k]) < 0.5)) return false;
+  }
+  return true;
+}
+
+static void run_policy(const sfmba_input* in, const sfmba_policy* pol, const sfmba_options* o,
+                       sfmba_output* out, sfmba_policy_report* rep) {
+  validate(in);
+  if (pol->intrinsics < SFMBA_INTRINSICS_FIXED || pol->intrinsics > SFMBA_INTRINSICS_FREE_GUARDED)
+    throw Error(SFMBA_ERR_INVALID_ARGUMENT, "unknown intrinsics mode");
+  sfmba_policy_report r{};
+  OwnedInput base = own_input(in);
+  OwnedOutput keep_state(base);
+  // Nothing to adjust (lib.rs:1426-1428, :1472-1474): the state is returned unchanged.
+  if (in->num_images < 2 || in->num_observations == 0) {
+    std::copy(base.poses.begin(), base.poses.end(), out->poses);
+    std::copy(base.pts.begin(), base.pts.end(), out->points);
+    std::copy(base.cams.begin(), base.cams.end(), out->cameras);
+    out->initial_cost = out->final_cost = 0.0;
+    out->iterations_run = 0;
+    if (rep) *rep = r;
+    return;
+  }
+  const size_t nc = base.cams.size();
+  // principal points pinned (lib.rs:1489)
+  base.fcp.assign(8 * nc, 0);
+  base.fcp_len.assign(nc, 0);
+  {
+    std::vector<uint8_t> mask(8 * nc);
+    std::vector<size_t> lens(nc);
+    for (size_t c = 0; c < nc; ++c) {
+      const int m = base.cams[c].model_id;
+      lens[c] = (size_t)model_param_count_host(m);
+      const bool f_first = m == SFMBA_SIMPLE_PINHOLE || m == SFMBA_SIMPLE_RADIAL || m == SFMBA_RADIAL;
+      mask[8 * c + (f_first ? 1 : 2)] = 1;
+      mask[8 * c + (f_first ? 2 : 3)] = 1;
+    }
+    base.fcp.clear();
+    for (size_t c = 0; c < nc; ++c) base.fcp.insert(base.fcp.end(), mask.begin() + 8 * c, mask.begin() + 8 * c + lens[c]);
+    base.fcp_len = lens;
+  }
+  std::vector<size_t> per_cam(nc, 0);
+  for (uint64_t c : base.coi) ++per_cam[c];
+  auto caller_fixed = [&](size_t c) {
+    return c < pol->num_camera_caller_fixed && pol->camera_caller_fixed[c] != 0;
+  };
+  sfmba_params p;
+  sfmba_params_default(&p);
+  p.robust_loss = pol->robust_loss;
+  p.max_iterations = pol->intrinsics == SFMBA_INTRINSICS_FREE_GUARDED ? 200 : p.max_iterations;
+  OwnedInput fixed_in = base, free_in = base;
+  fixed_in.fc.assign(nc, 1);
+  free_in.fc.assign(nc, 0);
+  bool any_eligible = false;
+  for (size_t c = 0; c < nc; ++c) {
+    free_in.fc[c] = (per_cam[c] < 5 || caller_fixed(c)) ? 1 : 0;
+    any_eligible = any_eligible || per_cam[c] >= 5;
+  }
+  if (pol->intrinsics == SFMBA_INTRINSICS_FIXED || pol->intrinsics == SFMBA_INTRINSICS_FREE) {
+    const OwnedInput& use = pol->intrinsics == SFMBA_INTRINSICS_FIXED ? fixed_in : free_in;
+    const sfmba_input v = use.view();
+    solve_once(&v, &p, o, out);
+    r.chose_free = pol->intrinsics == SFMBA_INTRINSICS_FREE;
+    if (rep) *rep = r;
+    return;
+  }
+  // FreeGuarded: both candidates, filtered, concurrently.
+  OwnedOutput fixed_out(base), free_out(base);
+  std::exception_ptr err;
+  std::thread free_thread;
+  if (any_eligible)
+    free_thread = std::thread([&] {
+      try {
+        HIP_CHECK(hipSetDevice(o->device));
+        const sfmba_input v = free_in.view();
+        filter_and_reoptimize(&v, &p, o, pol->max_reprojection_error_px, &free_out.o, &r.observations_kept_free,
+                              &r.filter_rounds_free);
+      } catch (...) {
+        err = std::current_exception();
+      }
+    });
+  {
+    const sfmba_input v = fixed_in.view();
+    try {
+      filter_and_reoptimize(&v, &p, o, pol->max_reprojection_error_px, &fixed_out.o, &r.observations_kept_fixed,
+                            &r.filter_rounds_fixed);
+    } catch (...) {
+      if (free_thread.joinable()) free_thread.join();
+      throw;
+    }
+  }
+  if (free_thread.joinable()) free_thread.join();
+  if (err) std::rethrow_exception(err);
+  // Plain mean reprojection error over all of the problem's observations
+  // (lib.rs:1542-1554).
+  OwnedInput eval = base;
+  eval.fp.clear();
+  eval.fc.clear();
+  eval.fcp.clear();
+  eval.fcp_len.clear();
+  eval.set_state(fixed_out.o);
+  {
+    const sfmba_input v = eval.view();
+    r.fixed_mean_error_px = mean_error(&v, o->device);
+  }
+  const OwnedOutput* chosen = &fixed_out;
+  if (any_eligible) {
+    r.distortion_plausible = distortion_plausible(free_out.cams.data(), nc) ? 1 : 0;
+    eval.set_state(free_out.o);
+    const sfmba_input v = eval.view();
+    r.free_mean_error_px = mean_error(&v, o->device);
+    if (r.distortion_plausible && r.free_mean_error_px < r.fixed_mean_error_px) {
+      chosen = &free_out;
+      r.chose_free = 1;
+    }
+  }
+  std::copy(chosen->poses.begin(), chosen->poses.end(), out->poses);
+  std::copy(chosen->pts.begin(), chosen->pts.end(), out->points);
+  std::copy(chosen->cams.begin(), chosen->cams.end(), out->cameras);
+  out->initial_cost = chosen->o.initial_cost;
+  out->final_cost = chosen->o.final_cost;
+  out->iterations_run = chosen->o.iterations_run;
+  if (rep) *rep = r;
+}
+
+}  // namespace sfmba
+
 using namespace sfmba;
 
 template <typename F>
@@ -1256,18 +1559,44 @@ int sfmba_mean_reprojection_error(const sfmba_input* input, double* out) {
   return guarded([&] {
     validate(input);
     if (!out) throw Error(SFMBA_ERR_INVALID_ARGUMENT, "out is NULL");
-    StreamGuard sg(0);
-    Problem pr;
-    Pack P;
-    double *po, *ca, *pt;
-    upload_state(input, pr, P, sg.s, &po, &ca, &pt);
-    double* part = pr.arena.alloc<double>(2 * ((size_t)pr.L.n_obs / 2048 + 2));
-    double* res = pr.arena.alloc<double>(2);
-    launch_cost(pr.L, po, ca, pt, SFMBA_LOSS_NONE, 1.0, part, res, 1, sg.s);
-    double h[2];
-    HIP_CHECK(hipMemcpyAsync(h, res, sizeof(h), hipMemcpyDeviceToHost, sg.s));
-    HIP_CHECK(hipStreamSynchronize(sg.s));
-    *out = h[1] == 0.0 ? 0.0 : h[0] / h[1];
+    *out = mean_error(input, 0);
+  });
+}
+
+int sfmba_filter_and_reoptimize(const sfmba_input* input, const sfmba_params* params,
+                                const sfmba_options* options, double threshold_px, sfmba_output* out,
+                                uint64_t* num_observations_kept) {
+  return guarded([&] {
+    if (!input || !params || !out) throw Error(SFMBA_ERR_INVALID_ARGUMENT, "NULL argument");
+    validate(input);
+    sfmba_options o;
+    sfmba_options_default(&o);
+    if (options) o = *options;
+    uint64_t kept = 0, rounds = 0;
+    filter_and_reoptimize(input, params, &o, threshold_px, out, &kept, &rounds);
+    if (num_observations_kept) *num_observations_kept = kept;
+  });
+}
+
+void sfmba_policy_default(sfmba_policy* p) {
+  if (!p) return;
+  *p = sfmba_policy{};
+  p->intrinsics = SFMBA_INTRINSICS_FREE_GUARDED;
+  p->robust_loss = SFMBA_LOSS_HUBER;
+  p->max_reprojection_error_px = 4.0;
+}
+
+int sfmba_run_bundle_adjustment(const sfmba_input* input, const sfmba_policy* policy,
+                                const sfmba_options* options, sfmba_output* out, sfmba_policy_report* report) {
+  return guarded([&] {
+    if (!input || !policy || !out) throw Error(SFMBA_ERR_INVALID_ARGUMENT, "NULL argument");
+    if ((input->num_images && !out->poses) || (input->num_points && !out->points) ||
+        (input->num_cameras && !out->cameras))
+      throw Error(SFMBA_ERR_INVALID_ARGUMENT, "output arrays are NULL");
+    sfmba_options o;
+    sfmba_options_default(&o);
+    if (options) o = *options;
+    run_policy(input, policy, &o, out, report);
   });
 }
 

@@ -207,6 +207,33 @@ __global__ __launch_bounds__(256) void k_jacobians_debug(
   }
 }
 
+// Residual norm and in-front flag of observations given in the caller's
+// order (the outlier filter of filter_and_reoptimize,
+// crates/sfm-reconstruction/src/lib.rs:1734-1746): |project(R X + t) - x|,
+// in_front = z > 1e-9.
+__global__ __launch_bounds__(256) void k_residual_norms(
+    uint32_t n_obs, const sfmba_observation* __restrict__ obs, const uint64_t* __restrict__ cam_of_img,
+    const int32_t* __restrict__ cam_model, const double* __restrict__ poses,
+    const double* __restrict__ cams, const double* __restrict__ pts, double* __restrict__ norm,
+    uint8_t* __restrict__ in_front) {
+  const uint32_t o = blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= n_obs) return;
+  const sfmba_observation ob = obs[o];
+  const uint64_t cam = cam_of_img[ob.image_idx];
+  double c0, c1, c2;
+  const double* X = pts + 3 * ob.point_idx;
+  transform_point(poses + 7 * ob.image_idx, X[0], X[1], X[2], c0, c1, c2);
+  if (c2 <= 1e-9) {
+    in_front[o] = 0;
+    norm[o] = 0.0;
+    return;
+  }
+  double r0, r1;
+  project_residual(cam_model[cam], cams + 8 * cam, c0, c1, c2, ob.x, ob.y, r0, r1);
+  in_front[o] = 1;
+  norm[o] = sqrt(r0 * r0 + r1 * r1);
+}
+
 // ------------------------------------------------------------- launchers
 static inline uint32_t blocks_for(uint64_t n, uint32_t per) { return (uint32_t)((n + per - 1) / per); }
 
@@ -338,6 +365,15 @@ void launch_cost(const Layout& L, const double* poses, const double* cams, const
     HIP_CHECK(hipGetLastError());
   }
   hipLaunchKernelGGL(k_cost_final, dim3(1), dim3(1024), 0, s, nb, part, out);
+  HIP_CHECK(hipGetLastError());
+}
+
+void launch_residual_norms(uint32_t n_obs, const sfmba_observation* obs, const uint64_t* cam_of_img,
+                           const int32_t* cam_model, const double* poses, const double* cams,
+                           const double* pts, double* norm, uint8_t* in_front, hipStream_t s) {
+  if (n_obs == 0) return;
+  hipLaunchKernelGGL(k_residual_norms, dim3(blocks_for(n_obs, 256)), dim3(256), 0, s, n_obs, obs, cam_of_img,
+                     cam_model, poses, cams, pts, norm, in_front);
   HIP_CHECK(hipGetLastError());
 }
 

@@ -1,0 +1,147 @@
+"""The caller's solve policy on the GPU (sfmba_run_bundle_adjustment,
+sfmba_filter_and_reoptimize) against a restatement of
+crates/sfm-reconstruction/src/lib.rs:1479-1768 driven by the oracle's
+bundle_adjust, residual and mean reprojection error: the observations the
+filter keeps, the candidate the guard chooses, and the final costs."""
+import numpy as np
+import pytest
+
+import oracle
+from sfmtory_amd import ba, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _state(inp, out, obs=None):
+    return ba.BaInput(inp.camera_of_image, out["cameras"], out["poses"], out["points"],
+                      inp.observations if obs is None else obs, fixed_poses=inp.fixed_poses,
+                      fixed_cameras=inp.fixed_cameras, fixed_camera_params=inp.fixed_camera_params)
+
+
+def oracle_filter(inp, params, thr):
+    """filter_and_reoptimize (lib.rs:1710-1768) on the oracle."""
+    out = oracle.bundle_adjust(inp, params)
+    cur = inp
+    for _ in range(2):
+        obs = cur.observations
+        keep = np.zeros(len(obs), dtype=bool)
+        for k, o in enumerate(obs):
+            i, p = int(o["image_idx"]), int(o["point_idx"])
+            cam = out["cameras"][int(cur.camera_of_image[i])]
+            r = oracle.residual(out["poses"][i], int(cam["model_id"]), cam["params"], out["points"][p],
+                                (o["x"], o["y"]))
+            keep[k] = r is not None and np.hypot(r[0], r[1]) <= thr
+        if keep.all() or keep.sum() < 12:
+            break
+        cur = _state(cur, out, obs[keep])
+        out = oracle.bundle_adjust(cur, params)
+    return out, len(cur.observations)
+
+
+def oracle_policy(inp, mode, thr=4.0, loss=ba.RobustLoss.HUBER, caller_fixed=()):
+    """run_bundle_adjustment's solve (lib.rs:1479-1695) on the oracle."""
+    nc = len(inp.cameras)
+    per_cam = np.bincount(np.asarray(inp.camera_of_image, dtype=np.int64), minlength=nc)
+    mask = oracle.default_fixed_params_mask(inp.cameras)
+    p = ba.BaParams(max_iterations=200 if mode == "free_guarded" else 50, robust_loss=loss)
+    pinned = lambda c: c < len(caller_fixed) and bool(caller_fixed[c])
+    free_fc = np.array([per_cam[c] < 5 or pinned(c) for c in range(nc)])
+
+    def make(fc):
+        return ba.BaInput(inp.camera_of_image, inp.cameras, inp.poses, inp.points, inp.observations,
+                          fixed_poses=inp.fixed_poses, fixed_cameras=fc, fixed_camera_params=mask)
+
+    if mode == "fixed":
+        return oracle.bundle_adjust(make(np.ones(nc, dtype=bool)), p), None
+    if mode == "free":
+        return oracle.bundle_adjust(make(free_fc), p), None
+    fixed, kf = oracle_filter(make(np.ones(nc, dtype=bool)), p, thr)
+    rep = dict(chose_free=0, observations_kept_fixed=kf)
+
+    def err(out):
+        return oracle.mean_reprojection_error(ba.BaInput(inp.camera_of_image, out["cameras"], out["poses"],
+                                                         out["points"], inp.observations))
+
+    rep["fixed_mean_error_px"] = err(fixed)
+    if not (per_cam >= 5).any():
+        return fixed, rep
+    free, kr = oracle_filter(make(free_fc), p, thr)
+    rep["observations_kept_free"] = kr
+    rep["free_mean_error_px"] = err(free)
+    d = free["cameras"]["params"]
+    models = free["cameras"]["model_id"]
+    coeffs = np.concatenate([d[c, {2: [3], 3: [3, 4], 4: [4, 5, 6, 7], 5: [4, 5, 6, 7]}.get(int(models[c]), [])]
+                             for c in range(nc)])
+    rep["distortion_plausible"] = int(np.all(np.abs(coeffs) < 0.5))
+    if rep["distortion_plausible"] and rep["free_mean_error_px"] < rep["fixed_mean_error_px"]:
+        rep["chose_free"] = 1
+        return free, rep
+    return fixed, rep
+
+
+def close(gpu, ref, rtol=1e-9):
+    assert gpu.final_cost == pytest.approx(ref["final_cost"], rel=rtol, abs=1e-9)
+    assert gpu.iterations_run == ref["iterations_run"]
+    assert np.allclose(gpu.points, ref["points"], rtol=0, atol=1e-6)
+    assert np.allclose(gpu.cameras["params"], ref["cameras"]["params"], rtol=1e-8, atol=1e-6)
+
+
+def test_filter_and_reoptimize_matches_oracle():
+    sc = scenes.generate("c0", num_points=600, seed=81, outlier_frac=0.05, outlier_px=30.0)
+    p = ba.BaParams(max_iterations=30)
+    out, kept = ba.filter_and_reoptimize(sc.inp, p, 4.0)
+    ref, rkept = oracle_filter(sc.inp, p, 4.0)
+    assert kept == rkept < len(sc.inp.observations)
+    close(out, ref)
+
+
+def test_guarded_prefers_refined_shared_focal():
+    sc = scenes.generate("c0", num_points=600, seed=82, shared_camera=1, sigma_f=0.05)
+    out, rep = ba.run_bundle_adjustment(sc.inp, "free_guarded")
+    ref, rrep = oracle_policy(sc.inp, "free_guarded")
+    assert rep["chose_free"] == rrep["chose_free"] == 1
+    assert rep["distortion_plausible"] == 1
+    assert rep["observations_kept_fixed"] == rrep["observations_kept_fixed"]
+    assert rep["observations_kept_free"] == rrep["observations_kept_free"]
+    assert rep["free_mean_error_px"] == pytest.approx(rrep["free_mean_error_px"], rel=1e-9)
+    assert rep["fixed_mean_error_px"] == pytest.approx(rrep["fixed_mean_error_px"], rel=1e-9)
+    close(out, ref)
+
+
+def test_guarded_rejects_implausible_distortion():
+    # True radial distortion -0.6: the free solve finds it, the plausibility
+    # bound (|k| < 0.5) rejects it and the fixed-intrinsics result is kept.
+    sc = scenes.generate("c0", num_points=600, seed=83, shared_camera=1, k_fixed=-0.6, sigma_f=0.02)
+    out, rep = ba.run_bundle_adjustment(sc.inp, "free_guarded")
+    ref, rrep = oracle_policy(sc.inp, "free_guarded")
+    assert rep["distortion_plausible"] == rrep["distortion_plausible"] == 0
+    assert rep["chose_free"] == rrep["chose_free"] == 0
+    close(out, ref)
+
+
+def test_guarded_without_eligible_camera_keeps_fixed(c0_scene):
+    # One camera per image: no camera has 5 images, the free solve never runs.
+    out, rep = ba.run_bundle_adjustment(c0_scene.inp, "free_guarded")
+    ref, rrep = oracle_policy(c0_scene.inp, "free_guarded")
+    assert rep["chose_free"] == 0 and rep["observations_kept_free"] == 0
+    close(out, ref)
+
+
+@pytest.mark.parametrize("mode", ["fixed", "free"])
+def test_fixed_and_free_modes(mode):
+    sc = scenes.generate("c0", num_points=500, seed=84, shared_camera=1)
+    caller_fixed = [0]
+    out, rep = ba.run_bundle_adjustment(sc.inp, mode, camera_caller_fixed=caller_fixed)
+    ref, _ = oracle_policy(sc.inp, mode, caller_fixed=caller_fixed)
+    close(out, ref)
+    if mode == "free":
+        pinned, _ = ba.run_bundle_adjustment(sc.inp, "free", camera_caller_fixed=[1])
+        assert np.array_equal(pinned.cameras["params"], sc.inp.cameras["params"])
+
+
+def test_too_small_problem_is_returned_unchanged():
+    inp = scenes.generate("c0", num_points=50, seed=85).inp
+    obs = inp.observations[inp.observations["image_idx"] == 0]
+    one = ba.BaInput(inp.camera_of_image[:1], inp.cameras[:1], inp.poses[:1], inp.points, obs)
+    out, rep = ba.run_bundle_adjustment(one, "free_guarded")
+    assert np.array_equal(out.points, inp.points) and out.iterations_run == 0
