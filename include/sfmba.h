@@ -346,6 +346,50 @@ int sfmba_observation_jacobians(const sfmba_input* input, int32_t precision,
 int sfmba_dense_solve(const double* A, size_t n, const double* b, int32_t method,
                       int32_t device, double* x, int32_t* status);
 
+/* ------------------------------------------------ COLMAP text model refine
+ * Replaces: fn cmd_refine (crates/sfm-cli/src/main.rs:946-1130) over the
+ * plain-text model reader/writer (crates/sfm-io/src/colmap.rs:25-300).
+ * Reads cameras.txt / images.txt / points3D.txt from model_dir, builds the
+ * compact problem (one pose per image in id order, one camera block per
+ * physical camera, one point per 3D point, observations from the tracks,
+ * keypoints as single precision), fixes image 0's pose, solves with the
+ * engine, recomputes each point's mean reprojection error against the
+ * refined model and writes the three files to output_dir (which may be
+ * model_dir, as the reference does).  Errors: fewer than 2 images, no
+ * points, no observations, or a malformed file (SFMBA_ERR_INVALID_ARGUMENT
+ * with the file and field named, as the reference's parse errors are). */
+typedef struct sfmba_colmap_refine_options {
+  int32_t refine_intrinsics; /* 0: cameras fixed; 1: free, principal point pinned */
+  int32_t robust_loss;       /* default SFMBA_LOSS_HUBER */
+  uint64_t max_iterations;   /* default 200 */
+} sfmba_colmap_refine_options;
+
+typedef struct sfmba_colmap_refine_report {
+  uint64_t num_images;
+  uint64_t num_points3d;
+  uint64_t num_observations;
+  double initial_cost;
+  double final_cost;
+  uint64_t iterations_run;
+  double mean_reprojection_error_px_before; /* Reconstruction::mean_reprojection_error */
+  double mean_reprojection_error_px_after;
+} sfmba_colmap_refine_report;
+
+void sfmba_colmap_refine_options_default(sfmba_colmap_refine_options* o);
+
+int sfmba_colmap_refine(const char* model_dir, const char* output_dir,
+                        const sfmba_colmap_refine_options* options,
+                        const sfmba_options* engine_options,
+                        sfmba_colmap_refine_report* report);
+
+/* Read the model in model_dir and write it to output_dir unchanged, through
+ * the same reader and writer (host only, no device call).  counts receives
+ * the number of cameras, images and 3D points (3 entries; may be NULL).
+ * Replaces: sfm_io::read_colmap_model + write_colmap_model
+ * (crates/sfm-io/src/colmap.rs:25-43). */
+int sfmba_colmap_rewrite(const char* model_dir, const char* output_dir,
+                         uint64_t* counts);
+
 /* Number of usable HIP devices (0 without a GPU). */
 int sfmba_device_count(void);
 

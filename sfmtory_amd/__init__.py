@@ -29,3 +29,4 @@ from .ba import (  # noqa: F401
     run_bundle_adjustment,
     total_cost,
 )
+from .colmap import refine_model, rewrite_model  # noqa: F401

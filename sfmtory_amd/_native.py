@@ -114,6 +114,17 @@ class IntrinsicsParams(C.Structure):
                 ("robust_loss", C.c_int32), ("reserved", C.c_int32), ("loss_scale_px", C.c_double)]
 
 
+class ColmapRefineOptions(C.Structure):
+    _fields_ = [("refine_intrinsics", C.c_int32), ("robust_loss", C.c_int32), ("max_iterations", C.c_uint64)]
+
+
+class ColmapRefineReport(C.Structure):
+    _fields_ = [("num_images", C.c_uint64), ("num_points3d", C.c_uint64), ("num_observations", C.c_uint64),
+                ("initial_cost", C.c_double), ("final_cost", C.c_double), ("iterations_run", C.c_uint64),
+                ("mean_reprojection_error_px_before", C.c_double),
+                ("mean_reprojection_error_px_after", C.c_double)]
+
+
 # Every symbol include/sfmba.h declares, with its ctypes signature.
 SIGNATURES = {
     "sfmba_params_default": (None, [C.POINTER(Params)]),
@@ -143,6 +154,10 @@ SIGNATURES = {
     "sfmba_policy_default": (None, [C.POINTER(Policy)]),
     "sfmba_run_bundle_adjustment": (C.c_int, [C.POINTER(Input), C.POINTER(Policy), C.POINTER(Options),
                                               C.POINTER(Output), C.POINTER(PolicyReport)]),
+    "sfmba_colmap_refine_options_default": (None, [C.POINTER(ColmapRefineOptions)]),
+    "sfmba_colmap_refine": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(ColmapRefineOptions), C.POINTER(Options),
+                                      C.POINTER(ColmapRefineReport)]),
+    "sfmba_colmap_rewrite": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(C.c_uint64)]),
     "sfmba_device_count": (C.c_int, []),
     "sfmba_last_error": (C.c_char_p, []),
 }

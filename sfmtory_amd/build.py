@@ -24,6 +24,11 @@ OBJ = PKG / "_build"
 HIP_SOURCES = ["engine_host.hip", "engine_kernels.hip", "dense_kernels.hip", "intrinsics_kernels.hip",
                "window_kernels.hip", "pcg_kernels.hip"]
 
+# host-only translation units (no device code): g++, linked into libsfmba
+HOST_SOURCES = ["colmap_io.cpp"]
+HOST_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-Wall", "-ffp-contract=off", "-D__HIP_PLATFORM_AMD__",
+              "-I/opt/rocm/include"]
+
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
     "-O3",
@@ -64,6 +69,10 @@ def build_engine() -> Path:
         obj = OBJ / (Path(src).stem + ".o")
         objs.append(obj)
         jobs.append([hipcc, *HIPCC_FLAGS, "-c", str(CSRC / src), "-o", str(obj)])
+    for src in HOST_SOURCES:
+        obj = OBJ / (Path(src).stem + ".o")
+        objs.append(obj)
+        jobs.append(["g++", *HOST_FLAGS, "-c", str(CSRC / src), "-o", str(obj)])
     with cf.ThreadPoolExecutor(max_workers=min(4, len(jobs))) as ex:
         list(ex.map(_run, jobs))
     out = LIB / "libsfmba.so"

@@ -156,6 +156,13 @@ void launch_cost(const Layout& L, const double* poses, const double* cams, const
 void launch_residual_norms(uint32_t n_obs, const sfmba_observation* obs, const uint64_t* cam_of_img,
                            const int32_t* cam_model, const double* poses, const double* cams,
                            const double* pts, double* norm, uint8_t* in_front, hipStream_t s);
+// Host helpers shared by the entry points: the calling thread's error text,
+// and |residual| / in-front flags of observations evaluated on `device`.
+void set_error(const std::string& m);
+void residual_norms(const std::vector<sfmba_observation>& obs, const std::vector<uint64_t>& coi,
+                    const std::vector<sfmba_camera>& cams, const std::vector<sfmba_pose>& poses,
+                    const std::vector<double>& pts, int device, std::vector<double>& norm,
+                    std::vector<uint8_t>& front);
 void launch_jacobians_debug(const Layout& L, const double* poses, const double* cams,
                             const double* pts, int fp32, double* r, double* jp, double* jx,
                             double* jc, uint8_t* valid, hipStream_t s);

@@ -29,7 +29,7 @@ namespace sfmba {
 
 static thread_local std::string g_last_error;
 
-static void set_error(const std::string& m) { g_last_error = m; }
+void set_error(const std::string& m) { g_last_error = m; }
 
 // Device allocations owned by one call; freed in the destructor.
 struct DevArena {
@@ -1069,7 +1069,6 @@ static void batch_solve(const sfmba_input* inputs, size_t count, const sfmba_par
   WindowResult* d_res = arena.alloc<WindowResult>(std::max<size_t>(nw, 1));
   const uint64_t max_it = std::max<uint64_t>(prm->max_iterations, 1);
   double* d_oc = arena.alloc<double>(std::max<size_t>(nw, 1) * max_it);
-  const auto t1 = clock::now();
   if (nw) {
     HIP_CHECK(hipMemcpyAsync(base, host.data(), state_end, hipMemcpyHostToDevice, sg.s));
     HIP_CHECK(hipMemcpyAsync(d_views, views.data(), nw * sizeof(View<T>), hipMemcpyHostToDevice, sg.s));
@@ -1237,38 +1236,47 @@ static double mean_error(const sfmba_input* in, int device) {
 
 // Observations that survive the filter (in front, |r| <= threshold) under
 // the state held by `st`.
-static std::vector<uint8_t> filter_mask(const OwnedInput& st, double threshold, int device) {
-  const size_t n = st.obs.size();
-  std::vector<uint8_t> keep(n, 0);
-  if (n == 0) return keep;
+void residual_norms(const std::vector<sfmba_observation>& obs, const std::vector<uint64_t>& coi,
+                    const std::vector<sfmba_camera>& cams, const std::vector<sfmba_pose>& pose_list,
+                    const std::vector<double>& pts, int device, std::vector<double>& norm,
+                    std::vector<uint8_t>& front) {
+  const size_t n = obs.size();
+  norm.assign(n, 0.0);
+  front.assign(n, 0);
+  if (n == 0) return;
   StreamGuard sg(device);
   DevArena A;
-  std::vector<int32_t> model(st.cams.size());
-  std::vector<double> params(8 * st.cams.size());
-  for (size_t c = 0; c < st.cams.size(); ++c) {
-    model[c] = st.cams[c].model_id;
-    std::copy(st.cams[c].params, st.cams[c].params + 8, params.begin() + 8 * c);
+  std::vector<int32_t> model(cams.size());
+  std::vector<double> params(8 * cams.size());
+  for (size_t c = 0; c < cams.size(); ++c) {
+    model[c] = cams[c].model_id;
+    std::copy(cams[c].params, cams[c].params + 8, params.begin() + 8 * c);
   }
-  std::vector<double> poses(7 * st.poses.size());
-  for (size_t i = 0; i < st.poses.size(); ++i) {
-    const sfmba_pose& q = st.poses[i];
+  std::vector<double> poses(7 * pose_list.size());
+  for (size_t i = 0; i < pose_list.size(); ++i) {
+    const sfmba_pose& q = pose_list[i];
     const double v[7] = {q.qw, q.qx, q.qy, q.qz, q.tx, q.ty, q.tz};
     std::copy(v, v + 7, poses.begin() + 7 * i);
   }
-  const sfmba_observation* d_obs = A.upload(st.obs, sg.s);
-  const uint64_t* d_coi = A.upload(st.coi, sg.s);
+  const sfmba_observation* d_obs = A.upload(obs, sg.s);
+  const uint64_t* d_coi = A.upload(coi, sg.s);
   const int32_t* d_model = A.upload(model, sg.s);
   const double* d_cams = A.upload(params, sg.s);
   const double* d_poses = A.upload(poses, sg.s);
-  const double* d_pts = A.upload(st.pts, sg.s);
+  const double* d_pts = A.upload(pts, sg.s);
   double* d_norm = A.alloc<double>(n);
   uint8_t* d_front = A.alloc<uint8_t>(n);
   launch_residual_norms((uint32_t)n, d_obs, d_coi, d_model, d_poses, d_cams, d_pts, d_norm, d_front, sg.s);
-  std::vector<double> norm(n);
   HIP_CHECK(hipMemcpyAsync(norm.data(), d_norm, n * sizeof(double), hipMemcpyDeviceToHost, sg.s));
-  HIP_CHECK(hipMemcpyAsync(keep.data(), d_front, n, hipMemcpyDeviceToHost, sg.s));
+  HIP_CHECK(hipMemcpyAsync(front.data(), d_front, n, hipMemcpyDeviceToHost, sg.s));
   HIP_CHECK(hipStreamSynchronize(sg.s));
-  for (size_t o = 0; o < n; ++o) keep[o] = keep[o] && norm[o] <= threshold;
+}
+
+static std::vector<uint8_t> filter_mask(const OwnedInput& st, double threshold, int device) {
+  std::vector<double> norm;
+  std::vector<uint8_t> keep;
+  residual_norms(st.obs, st.coi, st.cams, st.poses, st.pts, device, norm, keep);
+  for (size_t o = 0; o < keep.size(); ++o) keep[o] = keep[o] && norm[o] <= threshold;
   return keep;
 }
 
