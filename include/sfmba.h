@@ -346,6 +346,104 @@ int sfmba_observation_jacobians(const sfmba_input* input, int32_t precision,
 int sfmba_dense_solve(const double* A, size_t n, const double* b, int32_t method,
                       int32_t device, double* x, int32_t* status);
 
+/* ------------------------------------------- scoped BA over a reconstruction
+ * The problem assembly of the reference's caller: which images, points and
+ * observations of the growing reconstruction enter one bundle adjustment,
+ * and which poses are held.
+ *
+ * Replaces: fn run_bundle_adjustment's scope handling
+ * (crates/sfm-reconstruction/src/lib.rs:1339-1474) with BaScope
+ * (:1287-1299), LOCAL_BA_NUM_IMAGES (:1320) and the write-back (:1697-1707).
+ *
+ * GLOBAL: every registered image; every point whose track has >= 2
+ * elements; only the seed image's pose held.
+ * LOCAL (center): the points with >= 2 track elements one of which is in
+ * image `center`; the registered images ranked by how many of those
+ * track elements they hold (descending, ties by lower index), the first
+ * SFMBA_LOCAL_BA_NUM_IMAGES of them plus `center` free; every other
+ * registered image observing one of the points included with its pose
+ * held; the seed's pose held too.
+ * Both: pinned poses held; if no pose is held, the lowest-indexed image's
+ * is (gauge); observations are the track elements whose image is in the
+ * problem, with the keypoint's single-precision coordinates; cameras in
+ * order of first use.  Fewer than 2 images or no observation: nothing to
+ * adjust (the window is empty and the state is left as it is). */
+#define SFMBA_SCOPE_GLOBAL 0
+#define SFMBA_SCOPE_LOCAL 1
+#define SFMBA_LOCAL_BA_NUM_IMAGES 6
+
+/* The incremental pipeline's working state (crates/sfm-reconstruction/
+ * src/lib.rs:126-129 PointWork, the per-image poses, registered flags and
+ * camera map).  Image-indexed arrays have num_images entries; keypoints are
+ * the images' feature coordinates, image-major, keypoint_offsets[i] ..
+ * keypoint_offsets[i+1] for image i; point p's track is
+ * track_offsets[p] .. track_offsets[p+1] in track_images / track_keypoints. */
+typedef struct sfmba_reconstruction {
+  size_t num_images;
+  const uint8_t* registered;
+  sfmba_pose* poses;                 /* in/out; entries of unregistered images unused */
+  const uint32_t* image_camera_id;   /* the id of each image's physical camera */
+  const uint8_t* pose_pinned;        /* NULL or per image: caller-measured extrinsics, always held */
+  const uint64_t* keypoint_offsets;  /* num_images + 1 */
+  const float* keypoints;            /* x, y per keypoint */
+  size_t num_points;
+  double* points;                    /* in/out, x, y, z per point */
+  const uint64_t* track_offsets;     /* num_points + 1 */
+  const uint32_t* track_images;
+  const uint32_t* track_keypoints;
+  size_t num_cameras;
+  const uint32_t* camera_ids;        /* id of each camera entry */
+  sfmba_camera* cameras;             /* in/out */
+  const uint32_t* caller_fixed_camera_ids; /* intrinsics the caller pinned, by id */
+  size_t num_caller_fixed_camera_ids;
+  uint64_t seed_image;               /* the gauge image */
+} sfmba_reconstruction;
+
+typedef struct sfmba_window_summary {
+  uint64_t num_images, num_free_images, num_cameras, num_points, num_observations;
+  double initial_cost, final_cost;
+  uint64_t iterations_run;
+} sfmba_window_summary;
+
+/* One assembled problem, kept so that its result can be written back.  The
+ * input it exposes holds every camera fixed with principal points masked,
+ * the in-loop local bundle's semantics (lib.rs:708-719). */
+typedef struct sfmba_window sfmba_window;
+
+int sfmba_window_create(const sfmba_reconstruction* rec, int32_t scope,
+                        uint64_t center, sfmba_window** out);
+/* Pointers into the window; valid until sfmba_window_destroy.  An empty
+ * window has num_images == 0. */
+int sfmba_window_input(const sfmba_window* w, sfmba_input* input);
+/* Original image / point index of each compact index (NULL pointers allowed). */
+int sfmba_window_indices(const sfmba_window* w, const uint64_t** image_indices,
+                         size_t* num_images, const uint64_t** point_indices,
+                         size_t* num_points);
+/* Writes poses, points and cameras of a solve of this window back into rec
+ * (lib.rs:1697-1707); nothing for an empty window. */
+int sfmba_window_scatter(const sfmba_window* w, const sfmba_output* out,
+                         sfmba_reconstruction* rec);
+void sfmba_window_destroy(sfmba_window* w);
+
+/* Assemble (scope, center), solve it under `policy` (its camera_caller_fixed
+ * is ignored: rec's caller_fixed_camera_ids are used) and write back.
+ * summary and report may be NULL. */
+int sfmba_reconstruction_bundle_adjust(sfmba_reconstruction* rec, int32_t scope,
+                                       uint64_t center, const sfmba_policy* policy,
+                                       const sfmba_options* options,
+                                       sfmba_window_summary* summary,
+                                       sfmba_policy_report* report);
+
+/* Many independent reconstructions (seed trials, separate models), one
+ * window each, with intrinsics held -- the in-loop local bundle -- solved in
+ * one batched launch (sfmba_bundle_adjust_batch) and written back.
+ * summaries may be NULL. */
+int sfmba_reconstruction_bundle_adjust_batch(sfmba_reconstruction* recs, size_t count,
+                                             const int32_t* scopes, const uint64_t* centers,
+                                             const sfmba_params* params,
+                                             const sfmba_options* options,
+                                             sfmba_window_summary* summaries);
+
 /* ------------------------------------------------ COLMAP text model refine
  * Replaces: fn cmd_refine (crates/sfm-cli/src/main.rs:946-1130) over the
  * plain-text model reader/writer (crates/sfm-io/src/colmap.rs:25-300).

@@ -30,3 +30,4 @@ from .ba import (  # noqa: F401
     total_cost,
 )
 from .colmap import refine_model, rewrite_model  # noqa: F401
+from .scope import Reconstruction, Window, bundle_adjust_scope, bundle_adjust_scope_batch  # noqa: F401

@@ -125,6 +125,26 @@ class ColmapRefineReport(C.Structure):
                 ("mean_reprojection_error_px_after", C.c_double)]
 
 
+class Reconstruction(C.Structure):
+    _fields_ = [("num_images", C.c_size_t), ("registered", C.c_void_p), ("poses", C.c_void_p),
+                ("image_camera_id", C.c_void_p), ("pose_pinned", C.c_void_p), ("keypoint_offsets", C.c_void_p),
+                ("keypoints", C.c_void_p), ("num_points", C.c_size_t), ("points", C.c_void_p),
+                ("track_offsets", C.c_void_p), ("track_images", C.c_void_p), ("track_keypoints", C.c_void_p),
+                ("num_cameras", C.c_size_t), ("camera_ids", C.c_void_p), ("cameras", C.c_void_p),
+                ("caller_fixed_camera_ids", C.c_void_p), ("num_caller_fixed_camera_ids", C.c_size_t),
+                ("seed_image", C.c_uint64)]
+
+
+class WindowSummary(C.Structure):
+    _fields_ = [("num_images", C.c_uint64), ("num_free_images", C.c_uint64), ("num_cameras", C.c_uint64),
+                ("num_points", C.c_uint64), ("num_observations", C.c_uint64), ("initial_cost", C.c_double),
+                ("final_cost", C.c_double), ("iterations_run", C.c_uint64)]
+
+
+SCOPE_GLOBAL = 0
+SCOPE_LOCAL = 1
+LOCAL_BA_NUM_IMAGES = 6
+
 # Every symbol include/sfmba.h declares, with its ctypes signature.
 SIGNATURES = {
     "sfmba_params_default": (None, [C.POINTER(Params)]),
@@ -154,6 +174,18 @@ SIGNATURES = {
     "sfmba_policy_default": (None, [C.POINTER(Policy)]),
     "sfmba_run_bundle_adjustment": (C.c_int, [C.POINTER(Input), C.POINTER(Policy), C.POINTER(Options),
                                               C.POINTER(Output), C.POINTER(PolicyReport)]),
+    "sfmba_window_create": (C.c_int, [C.POINTER(Reconstruction), C.c_int32, C.c_uint64, C.POINTER(C.c_void_p)]),
+    "sfmba_window_input": (C.c_int, [C.c_void_p, C.POINTER(Input)]),
+    "sfmba_window_indices": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_size_t),
+                                       C.POINTER(C.c_void_p), C.POINTER(C.c_size_t)]),
+    "sfmba_window_scatter": (C.c_int, [C.c_void_p, C.POINTER(Output), C.POINTER(Reconstruction)]),
+    "sfmba_window_destroy": (None, [C.c_void_p]),
+    "sfmba_reconstruction_bundle_adjust": (C.c_int, [C.POINTER(Reconstruction), C.c_int32, C.c_uint64,
+                                                     C.POINTER(Policy), C.POINTER(Options),
+                                                     C.POINTER(WindowSummary), C.POINTER(PolicyReport)]),
+    "sfmba_reconstruction_bundle_adjust_batch": (C.c_int, [C.POINTER(Reconstruction), C.c_size_t, C.c_void_p,
+                                                           C.c_void_p, C.POINTER(Params), C.POINTER(Options),
+                                                           C.c_void_p]),
     "sfmba_colmap_refine_options_default": (None, [C.POINTER(ColmapRefineOptions)]),
     "sfmba_colmap_refine": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(ColmapRefineOptions), C.POINTER(Options),
                                       C.POINTER(ColmapRefineReport)]),

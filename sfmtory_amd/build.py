@@ -25,7 +25,7 @@ HIP_SOURCES = ["engine_host.hip", "engine_kernels.hip", "dense_kernels.hip", "in
                "window_kernels.hip", "pcg_kernels.hip"]
 
 # host-only translation units (no device code): g++, linked into libsfmba
-HOST_SOURCES = ["colmap_io.cpp"]
+HOST_SOURCES = ["colmap_io.cpp", "scope.cpp"]
 HOST_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-Wall", "-ffp-contract=off", "-D__HIP_PLATFORM_AMD__",
               "-I/opt/rocm/include"]
 

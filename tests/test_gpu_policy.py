@@ -6,77 +6,10 @@ filter keeps, the candidate the guard chooses, and the final costs."""
 import numpy as np
 import pytest
 
-import oracle
+from oracle.policy import oracle_filter, oracle_policy
 from sfmtory_amd import ba, scenes
 
 pytestmark = pytest.mark.gpu
-
-
-def _state(inp, out, obs=None):
-    return ba.BaInput(inp.camera_of_image, out["cameras"], out["poses"], out["points"],
-                      inp.observations if obs is None else obs, fixed_poses=inp.fixed_poses,
-                      fixed_cameras=inp.fixed_cameras, fixed_camera_params=inp.fixed_camera_params)
-
-
-def oracle_filter(inp, params, thr):
-    """filter_and_reoptimize (lib.rs:1710-1768) on the oracle."""
-    out = oracle.bundle_adjust(inp, params)
-    cur = inp
-    for _ in range(2):
-        obs = cur.observations
-        keep = np.zeros(len(obs), dtype=bool)
-        for k, o in enumerate(obs):
-            i, p = int(o["image_idx"]), int(o["point_idx"])
-            cam = out["cameras"][int(cur.camera_of_image[i])]
-            r = oracle.residual(out["poses"][i], int(cam["model_id"]), cam["params"], out["points"][p],
-                                (o["x"], o["y"]))
-            keep[k] = r is not None and np.hypot(r[0], r[1]) <= thr
-        if keep.all() or keep.sum() < 12:
-            break
-        cur = _state(cur, out, obs[keep])
-        out = oracle.bundle_adjust(cur, params)
-    return out, len(cur.observations)
-
-
-def oracle_policy(inp, mode, thr=4.0, loss=ba.RobustLoss.HUBER, caller_fixed=()):
-    """run_bundle_adjustment's solve (lib.rs:1479-1695) on the oracle."""
-    nc = len(inp.cameras)
-    per_cam = np.bincount(np.asarray(inp.camera_of_image, dtype=np.int64), minlength=nc)
-    mask = oracle.default_fixed_params_mask(inp.cameras)
-    p = ba.BaParams(max_iterations=200 if mode == "free_guarded" else 50, robust_loss=loss)
-    pinned = lambda c: c < len(caller_fixed) and bool(caller_fixed[c])
-    free_fc = np.array([per_cam[c] < 5 or pinned(c) for c in range(nc)])
-
-    def make(fc):
-        return ba.BaInput(inp.camera_of_image, inp.cameras, inp.poses, inp.points, inp.observations,
-                          fixed_poses=inp.fixed_poses, fixed_cameras=fc, fixed_camera_params=mask)
-
-    if mode == "fixed":
-        return oracle.bundle_adjust(make(np.ones(nc, dtype=bool)), p), None
-    if mode == "free":
-        return oracle.bundle_adjust(make(free_fc), p), None
-    fixed, kf = oracle_filter(make(np.ones(nc, dtype=bool)), p, thr)
-    rep = dict(chose_free=0, observations_kept_fixed=kf)
-
-    def err(out):
-        return oracle.mean_reprojection_error(ba.BaInput(inp.camera_of_image, out["cameras"], out["poses"],
-                                                         out["points"], inp.observations))
-
-    rep["fixed_mean_error_px"] = err(fixed)
-    if not (per_cam >= 5).any():
-        return fixed, rep
-    free, kr = oracle_filter(make(free_fc), p, thr)
-    rep["observations_kept_free"] = kr
-    rep["free_mean_error_px"] = err(free)
-    d = free["cameras"]["params"]
-    models = free["cameras"]["model_id"]
-    coeffs = np.concatenate([d[c, {2: [3], 3: [3, 4], 4: [4, 5, 6, 7], 5: [4, 5, 6, 7]}.get(int(models[c]), [])]
-                             for c in range(nc)])
-    rep["distortion_plausible"] = int(np.all(np.abs(coeffs) < 0.5))
-    if rep["distortion_plausible"] and rep["free_mean_error_px"] < rep["fixed_mean_error_px"]:
-        rep["chose_free"] = 1
-        return free, rep
-    return fixed, rep
 
 
 def close(gpu, ref, rtol=1e-9):
