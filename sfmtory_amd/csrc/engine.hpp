@@ -39,6 +39,7 @@ inline size_t dense_workspace_doubles(uint32_t Dp) {
   return (size_t)(Dp / NB) * (LINV_BLOCK + TOP_BLOCK);
 }
 constexpr int PAIR_CHUNK = 128;   // max point pairs reduced by one wavefront
+constexpr uint64_t SCHUR_TILE = 64; // destination tile edge (parameter blocks) of the chunk order
 
 struct Error : std::runtime_error {
   int code;
@@ -84,7 +85,7 @@ struct Layout {
   uint32_t* inc_grp = nullptr;
   uint32_t* inc_pt = nullptr;
   uint32_t* inc_obs = nullptr;
-  uint32_t* inc_roff = nullptr;  // first row (point-major J, E)
+  uint32_t* inc_roff = nullptr;  // first row of J, E and F (group-major)
   uint32_t* inc_froff = nullptr; // first row of F (group-major)
   uint32_t* inc_row0 = nullptr;  // first entry of the observation's side vector
   uint32_t* pt_inc_ptr = nullptr;  // incidences of each point (CSR, n_pt + 1)

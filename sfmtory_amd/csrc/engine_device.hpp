@@ -310,9 +310,9 @@ constexpr int PART_STRIDE = 2 * 196 + 16;
 // da x db block.  Sums, over the chunk's point pairs (a, b), B = w J_a^T J_b
 // for pairs of the same observation and Schur = F_a E_b^T = E_a Cinv E_b^T;
 // on diagonal blocks also the right-hand side F_a g_point - w J_a^T r over
-// pairs a == b (F4.w - E4.w).  A pair is two words: a's first F row (F is
-// group-major) with bit 31 "same observation" and bit 30 "a's rows precede
-// b's", and b's first row with bit 31 "a == b".  U pairs are processed per
+// pairs a == b (F4.w - E4.w).  A pair is two words: a's first row with bit
+// 31 "same observation", and b's first row with bit 31 "a == b" (J, E and F
+// rows share one group-major row numbering).  U pairs are processed per
 // step with all their loads issued together.  A chunk (ch.w == NONE) that
 // covers its whole destination is written straight into S (with the
 // Marquardt damping of the diagonal); others leave partials.
@@ -321,7 +321,7 @@ constexpr int PART_STRIDE = 2 * 196 + 16;
 template <typename T, int NQ, int UMAX = (sizeof(T) == 4 ? 4 : 2)>
 __device__ inline void schur_chunk(const View<T>& v, const uint4 ch, double lambda, double* parts) {
   using V4 = typename Vec4<T>::type;
-  constexpr uint32_t HI = 0x80000000u, BEFORE = 0x40000000u, OFF = 0x3FFFFFFFu;
+  constexpr uint32_t HI = 0x80000000u, OFF = 0x7FFFFFFFu;
   constexpr int U = UMAX / NQ > 1 ? UMAX / NQ : 1;
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t ga = ch.x >> 16, gb = ch.x & 0xFFFFu;
@@ -367,13 +367,11 @@ __device__ inline void schur_chunk(const View<T>& v, const uint4 ch, double lamb
       for (int q = 0; q < NQ; ++q)
         if (on[q]) accS[q].add(f[u][q].x * e[u][q].x + f[u][q].y * e[u][q].y + f[u][q].z * e[u][q].z);
       if (pr[u].x & HI) {
-        // same observation: a's point-major first row from b's
-        const bool self = (pr[u].y & HI) != 0u;
-        const uint32_t raj = self ? rb : ((pr[u].x & BEFORE) ? rb - da : rb + db);
+        // same observation: w J_a^T J_b from the J rows of a and b
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
           if (on[q]) {
-            const V4 ja = v.J4[raj + er[q]];
+            const V4 ja = v.J4[ra + er[q]];
             const V4 jb = v.J4[rb + ec[q]];
             accB[q].add(ja.z * jb.x + ja.w * jb.y);
           }
@@ -404,6 +402,147 @@ __device__ inline void schur_chunk(const View<T>& v, const uint4 ch, double lamb
         pp[lane + 64 * q] = accB[q].value();
         pp[196 + lane + 64 * q] = accS[q].value();
       }
+    }
+    if (rlane) pp[392 + lane] = accR.value();
+  }
+}
+
+// ----------------------------------------------------- MFMA Schur chunk
+// 16x16x4 matrix-core product for T: lane l supplies A[l % 16][l / 16] and
+// B[l / 16][l % 16]; register v of lane l holds D[row(l, v)][l % 16], with
+// row 4 (l / 16) + v in FP32 and l / 16 + 4 v in FP64 (checked on gfx950
+// with a one-wavefront probe of both builtins).
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<float> {
+  typedef float acc_t __attribute__((ext_vector_type(4)));
+  static __device__ __attribute__((always_inline)) inline acc_t step(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __attribute__((always_inline)) inline uint32_t row(uint32_t lane, int v) {
+    return 4u * (lane >> 4) + (uint32_t)v;
+  }
+};
+template <>
+struct Mfma<double> {
+  typedef double acc_t __attribute__((ext_vector_type(4)));
+  static __device__ __attribute__((always_inline)) inline acc_t step(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __attribute__((always_inline)) inline uint32_t row(uint32_t lane, int v) {
+    return (lane >> 4) + 4u * (uint32_t)v;
+  }
+};
+
+// The same destination-block reduction as schur_chunk, on the matrix cores:
+// the block is one 16x16 MFMA tile (da, db <= 16).  Per pair, lane l loads
+// one component, F_a[l % 16][l / 16] and E_b[l % 16][l / 16] for the three
+// point coordinates (K slot 3 stays zero), and one MFMA adds F_a E_b^T; a
+// pair of the same observation adds w J_a^T J_b with a second MFMA over the
+// two residual components.  The FP32 instantiation flushes its MFMA sums
+// into FP64 after every U pairs, so no more than U terms are summed in
+// FP32.  Output entries, partials and the right-hand side follow
+// schur_chunk.
+template <typename T, int U>
+__device__ inline void schur_chunk_mfma(const View<T>& v, const uint4 ch, double lambda, double* parts) {
+  using M = Mfma<T>;
+  using acc_t = typename M::acc_t;
+  constexpr uint32_t HI = 0x80000000u, OFF = 0x7FFFFFFFu;
+  static_assert(PAIR_CHUNK <= 128, "a chunk's pair words are held in two registers per lane");
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t i = lane & 15, k = lane >> 4;
+  const uint32_t ga = ch.x >> 16, gb = ch.x & 0xFFFFu;
+  const uint32_t da = v.grp_dim[ga], db = v.grp_dim[gb];
+  const bool diag = ga == gb;
+  const bool fa_on = i < da && k < 3, eb_on = i < db && k < 3;
+  const bool ja_on = i < da && k < 2, jb_on = i < db && k < 2;
+  const T* F = reinterpret_cast<const T*>(v.F4);
+  const T* E = reinterpret_cast<const T*>(v.E4);
+  const T* J = reinterpret_cast<const T*>(v.J4);
+  // All pair words of the chunk in one coalesced load: lane l holds pairs
+  // l and 64 + l; pair q reaches every lane by v_readlane.
+  const uint32_t n = ch.z - ch.y;
+  const uint2 w0 = lane < n ? v.pairs[ch.y + lane] : make_uint2(0u, 0u);
+  const uint2 w1 = lane + 64 < n ? v.pairs[ch.y + 64 + lane] : make_uint2(0u, 0u);
+  auto word = [&](uint32_t q, uint32_t& x, uint32_t& y) {
+    const uint32_t sl = q & 63;
+    x = __builtin_amdgcn_readlane(q < 64 ? w0.x : w1.x, sl);
+    y = __builtin_amdgcn_readlane(q < 64 ? w0.y : w1.y, sl);
+  };
+  // Operands of U pairs starting at q (zero past the chunk's end).
+  struct Stage {
+    T fa[U], eb[U], ja[U], jb[U];
+  };
+  auto load = [&](uint32_t q, Stage& st) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const bool ok = q + u < n;
+      uint32_t px = 0u, py = 0u;
+      if (ok) word(q + u, px, py);
+      const uint32_t ra = px & OFF, rb = py & OFF;
+      st.fa[u] = (ok && fa_on) ? F[4 * (size_t)(ra + i) + k] : T(0);
+      st.eb[u] = (ok && eb_on) ? E[4 * (size_t)(rb + i) + k] : T(0);
+      const bool same = ok && (px & HI);
+      st.ja[u] = (same && ja_on) ? J[4 * (size_t)(ra + i) + 2 + k] : T(0);
+      st.jb[u] = (same && jb_on) ? J[4 * (size_t)(rb + i) + k] : T(0);
+    }
+  };
+  acc_t s = {T(0), T(0), T(0), T(0)}, b = {T(0), T(0), T(0), T(0)};
+  double ss[4] = {0.0, 0.0, 0.0, 0.0}, bs[4] = {0.0, 0.0, 0.0, 0.0};
+  Acc<T> accR;
+  const bool rlane = diag && lane < da;
+  Stage cur, nxt;
+  load(0, cur);
+  for (uint32_t q = 0; q < n; q += U) {
+    if (q + U < n) load(q + U, nxt);  // next group's loads in flight during this group's MFMAs
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (q + u >= n) break;
+      uint32_t px, py;
+      word(q + u, px, py);
+      s = M::step(cur.fa[u], cur.eb[u], s);
+      if (px & HI) b = M::step(cur.ja[u], cur.jb[u], b);
+      if (rlane && (py & HI)) accR.add(v.F4[(px & OFF) + lane].w - v.E4[(py & OFF) + lane].w);
+    }
+    if (sizeof(T) == 4) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        ss[c] += (double)s[c];
+        bs[c] += (double)b[c];
+        s[c] = T(0);
+        b[c] = T(0);
+      }
+    }
+    cur = nxt;
+  }
+  if (sizeof(T) == 8) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      ss[c] = (double)s[c];
+      bs[c] = (double)b[c];
+    }
+  }
+  const uint32_t oa = v.grp_off[ga], ob = v.grp_off[gb];
+  const uint32_t c = i;
+  if (ch.w == NONE) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const uint32_t r = M::row(lane, q);
+      if (r >= da || c >= db || (diag && r < c)) continue;
+      double val = bs[q] - ss[q];
+      if (diag && r == c) val = (bs[q] + bs[q] * lambda) - ss[q];
+      v.S[(size_t)(ob + c) * v.ld + oa + r] = val;
+    }
+    if (rlane) v.S[(size_t)(oa + lane) * v.ld + v.Dp] = accR.value();
+  } else {
+    double* pp = parts + (size_t)ch.w * PART_STRIDE;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const uint32_t r = M::row(lane, q);
+      if (r >= da || c >= db) continue;
+      pp[r + c * da] = bs[q];
+      pp[196 + r + c * da] = ss[q];
     }
     if (rlane) pp[392 + lane] = accR.value();
   }

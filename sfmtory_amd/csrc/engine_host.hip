@@ -261,10 +261,11 @@ static void build_pack(const sfmba_input* in, Pack& P, bool need_pairs,
   }
   for (uint32_t p = 0; p < P.n_pt; ++p) P.pt_inc_ptr[p + 1] = std::max(P.pt_inc_ptr[p + 1], P.pt_inc_ptr[p]);
   P.n_rows = rows;
-  if (rows >= 0x40000000u) throw Error(SFMBA_ERR_UNSUPPORTED, "too many Jacobian rows for 30-bit offsets");
-  // F rows (the Schur assembly's row-side operand) are stored group-major:
-  // each group's incidences contiguous, in point order, so the destination
-  // blocks of one block row read a compact range that stays in L2.
+  if (rows >= 0x80000000u) throw Error(SFMBA_ERR_UNSUPPORTED, "too many Jacobian rows for 31-bit offsets");
+  // J, E and F rows are stored group-major: each group's incidences
+  // contiguous, in point order, so a destination block (ga, gb) of the Schur
+  // assembly reads two compact row ranges (group ga's F and J rows, group
+  // gb's E and J rows) that stay in L2 while its pairs are reduced.
   {
     std::vector<uint32_t> gstart(P.n_grp, 0);
     for (uint32_t g : P.inc_grp) gstart[g] += P.grp_dim[g];
@@ -280,6 +281,7 @@ static void build_pack(const sfmba_input* in, Pack& P, bool need_pairs,
       P.inc_froff[a] = gstart[g];
       gstart[g] += P.grp_dim[g];
     }
+    P.inc_roff = P.inc_froff;
   }
   if (!need_pairs || D == 0) return;
   if (P.n_grp > 0xFFFFu) throw Error(SFMBA_ERR_UNSUPPORTED, "more than 65535 parameter blocks");
@@ -312,30 +314,27 @@ static void build_pack(const sfmba_input* in, Pack& P, bool need_pairs,
           const uint64_t ga = P.inc_grp[a], gb = P.inc_grp[b];
           if (ga < gb) continue;
           const uint32_t q = pos[ga * (ga + 1) / 2 + gb]++;
-          // First word: a's first F row (group-major), bit 31 "same
-          // observation", bit 30 "a's rows precede b's" (the two incidences
-          // of one observation are adjacent in the point-major layout).
-          // Second word: b's first row (point-major), bit 31 "a == b".
+          // First word: a's first row, bit 31 "same observation".  Second
+          // word: b's first row, bit 31 "a == b".
           const bool same = P.inc_obs[a] == P.inc_obs[b];
-          const bool before = same && a != b && P.inc_roff[a] < P.inc_roff[b];
-          P.pairs[2 * (size_t)q] = P.inc_froff[a] | (same ? 0x80000000u : 0u) | (before ? 0x40000000u : 0u);
-          P.pairs[2 * (size_t)q + 1] = P.inc_roff[b] | (a == b ? 0x80000000u : 0u);
+          P.pairs[2 * (size_t)q] = P.inc_froff[a] | (same ? 0x80000000u : 0u);
+          P.pairs[2 * (size_t)q + 1] = P.inc_froff[b] | (a == b ? 0x80000000u : 0u);
         }
     }
   }
-  // Chunks name their destination as (ga << 16) | gb.
-  for (uint64_t d = 0, ga = 0, gb = 0; d < n_dests; ++d) {
+  // Chunks name their destination as (ga << 16) | gb.  They are emitted in
+  // tiles of SCHUR_TILE x SCHUR_TILE destination blocks, so the wavefronts in
+  // flight read the F and E rows of about 2 SCHUR_TILE groups (a working set
+  // that stays in the MALL) rather than of every group of a block row.
+  auto emit = [&](uint64_t ga, uint64_t gb) {
+    const uint64_t d = ga * (ga + 1) / 2 + gb;
     const uint32_t code = (uint32_t)((ga << 16) | gb);
-    if (++gb > ga) {
-      ++ga;
-      gb = 0;
-    }
     const uint32_t b0 = cnt[d], b1 = cnt[d + 1];
     const uint32_t nch =
         pair_chunk == 0 ? 1u : std::max<uint32_t>(1, (b1 - b0 + pair_chunk - 1) / pair_chunk);
     if (nch == 1) {
       P.chunks.push_back(uint4{code, b0, b1, NONE});
-      continue;
+      return;
     }
     P.multi_dest.push_back(code);
     P.dest_part0.push_back(P.n_parts);
@@ -344,7 +343,11 @@ static void build_pack(const sfmba_input* in, Pack& P, bool need_pairs,
       const uint32_t lo = b0 + k * pair_chunk, hi = std::min(b1, lo + pair_chunk);
       P.chunks.push_back(uint4{code, lo, hi, P.n_parts++});
     }
-  }
+  };
+  for (uint64_t ta = 0; ta < G; ta += SCHUR_TILE)
+    for (uint64_t tb = 0; tb <= ta; tb += SCHUR_TILE)
+      for (uint64_t ga = ta; ga < std::min<uint64_t>(G, ta + SCHUR_TILE); ++ga)
+        for (uint64_t gb = tb; gb < std::min<uint64_t>(ga + 1, tb + SCHUR_TILE); ++gb) emit(ga, gb);
 }
 
 struct Problem {

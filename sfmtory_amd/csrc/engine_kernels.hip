@@ -3,6 +3,8 @@
 // engine_device.hpp, shared with the persistent window solver.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "engine_device.hpp"
 
 namespace sfmba {
@@ -31,13 +33,13 @@ __global__ __launch_bounds__(256) void k_compute_F(View<T> v) {
   if (a < v.n_inc) compute_F(v, a);
 }
 
-// One wavefront per chunk, in destination order: the workgroups in flight on
-// all XCDs then share one block row's F rows (a compact group-major range,
-// held by L2 and the MALL).
-template <typename T, int NQ>
-__global__ __launch_bounds__(256) void k_schur(View<T> v, double lambda, double* __restrict__ parts) {
+// The Schur reduction on the matrix cores, one wavefront per chunk, chunks
+// in destination-tile order: the wavefronts in flight work on neighbouring
+// destinations and share their F and E rows in L2 and the MALL.
+template <typename T>
+__global__ __launch_bounds__(256) void k_schur_mfma(View<T> v, double lambda, double* __restrict__ parts) {
   const uint32_t ci = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
-  if (ci < v.n_chunks) schur_chunk<T, NQ>(v, v.chunks[ci], lambda, parts);
+  if (ci < v.n_chunks) schur_chunk_mfma<T, 4>(v, v.chunks[ci], lambda, parts);
 }
 
 // Destinations split over several chunks: sum the partials in chunk order.
@@ -321,13 +323,8 @@ template <typename T>
 void launch_schur(const Layout& L, Work<T>& W, double lambda, hipStream_t s) {
   const View<T> v = make_view(L, W);
   if (L.n_chunks) {
-    const unsigned grid = blocks_for(L.n_chunks, 4);
-    const uint32_t ne = L.max_grp_dim * L.max_grp_dim;
-    auto go = [&](auto kern) { hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, s, v, lambda, W.parts); };
-    if (ne <= 64) go(k_schur<T, 1>);
-    else if (ne <= 128) go(k_schur<T, 2>);
-    else if (ne <= 192) go(k_schur<T, 3>);
-    else go(k_schur<T, 4>);
+    if (L.max_grp_dim > 16) throw Error(SFMBA_ERR_INVALID_ARGUMENT, "group dimension above 16");
+    hipLaunchKernelGGL(k_schur_mfma<T>, dim3(blocks_for(L.n_chunks, 4)), dim3(256), 0, s, v, lambda, W.parts);
     HIP_CHECK(hipGetLastError());
   }
   if (L.n_multi) {
