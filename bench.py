@@ -18,7 +18,8 @@ the Schur assembly beside it, a CPU baseline from the oracle, the
 256-window batched solve (windows/s) and the 5,000-camera scene solved by
 PCG and by Cholesky are reported beside it.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--scene c2] [--no-cpu-baseline] [--no-windows] [--no-large]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--scene c2] [--no-cpu-baseline] [--no-windows]
+                  [--no-large] [--copy-calibration]
 """
 from __future__ import annotations
 
@@ -39,6 +40,52 @@ PEAK_FP64_MFMA = 78.6e12   # FLOP/s, MI355X dense FP64 matrix
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_syrk_update_hbm_traffic.txt")
+
+
+def measured_traffic():
+    """HBM bytes per k_syrk_update launch of this workload, from the counter
+    runs summarised in profiles/k_syrk_update_hbm_traffic.txt (FETCH_SIZE +
+    WRITE_SIZE per dispatch, each scaled by the same counter's reading for a
+    1 GiB copy); None when that summary is absent."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def copy_calibration():
+    """Three device-to-device copies of 1 GiB (hipMemcpy: the runtime's copy
+    kernel reads 1 GiB and writes 1 GiB each): the plain copy of known size
+    that a FETCH_SIZE / WRITE_SIZE counter run divides by."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    hip.hipFree.argtypes = [C.c_void_p]
+    n = 1 << 30
+    src, dst = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(src), n) or hip.hipMalloc(C.byref(dst), n):
+        raise RuntimeError("hipMalloc failed")
+    try:
+        if hip.hipMemset(src, 1, n):
+            raise RuntimeError("hipMemset failed")
+        for _ in range(3):
+            if hip.hipMemcpy(dst, src, n, 3):  # hipMemcpyDeviceToDevice
+                raise RuntimeError("hipMemcpy failed")
+        if hip.hipDeviceSynchronize():
+            raise RuntimeError("hipDeviceSynchronize failed")
+    finally:
+        hip.hipFree(src)
+        hip.hipFree(dst)
+    log("copy calibration: 3 x 1 GiB device-to-device copies done")
 
 
 def quat_to_R(q):
@@ -201,6 +248,9 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-windows", action="store_true", help="skip the 256-window batched section")
     ap.add_argument("--no-large", action="store_true", help="skip the 5,000-camera PCG vs Cholesky section")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="end with three 1 GiB device-to-device copies (the known-size kernel that scales "
+                         "FETCH_SIZE / WRITE_SIZE in a counter run)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (the engine has no multi-GPU path); got --gpus {args.gpus}",
@@ -292,7 +342,7 @@ def main() -> int:
     f_tu = prof_fp64["flop_trailing_update"] / max(1, n_tu)
     roof = {"bound": "mfma", "kernel": "k_syrk_update (FP64 trailing update + lookahead diagonal factorization)",
             "achieved": f_tu / t_tu / 1e12, "peak": PEAK_FP64_MFMA / 1e12, "unit": "TFLOP/s",
-            "frac": f_tu / t_tu / PEAK_FP64_MFMA, "traffic": None,
+            "frac": f_tu / t_tu / PEAK_FP64_MFMA, "traffic": measured_traffic(),
             "flop_per_launch": f_tu, "avg_launch_us": t_tu * 1e6, "launches": n_tu}
     # The whole factorization (D^3/3 + 2 D^2 FP64 FLOP) as a kernel group.
     flop = D ** 3 / 3.0 + 2.0 * D ** 2
@@ -366,7 +416,9 @@ def main() -> int:
         "windows": windows,
         "large_scene": large,
     }
-    print(json.dumps(line))
+    print(json.dumps(line), flush=True)
+    if args.copy_calibration:
+        copy_calibration()
     return 0
 
 

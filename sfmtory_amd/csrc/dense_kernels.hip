@@ -19,8 +19,15 @@
 //                   the rhs row) by blocked substitution with the X11, one
 //                   wavefront per 16-row strip, all in MFMA registers;
 //   k_syrk_update   A_ij -= L_ik L_jk^T on the trailing lower triangle,
-//                   128x128 tiles, v_mfma_f64_16x16x4_f64; its first
-//                   workgroup updates and factors the next diagonal block.
+//                   128x128 tiles, v_mfma_f64_16x16x4_f64, panels staged
+//                   through LDS by global_load_lds (double buffered); its
+//                   first workgroup updates and factors the next diagonal
+//                   block.
+// While the trailing matrix is large, block columns go in pairs: the update
+// of column block k+1 alone, its factorization and panel solve, then one
+// update of the rest with both panels (K = 2 NB), which halves the trailing
+// matrix's HBM traffic per FLOP.  Near the end, where the diagonal chain
+// bounds each step, single steps keep that chain short.
 // L_kk^{-1} (for the triangular solves) is formed for all blocks at once
 // after the factorization, off the critical path.
 #include <hip/hip_runtime.h>
@@ -33,9 +40,16 @@ namespace sfmba {
 
 typedef double double4_t __attribute__((ext_vector_type(4)));
 
-constexpr int KC = 8;    // K chunk staged through LDS
+constexpr int KC = 16;   // K chunk staged through LDS
 constexpr int KS = 144;  // LDS row stride of a staged K chunk (2 KS mod 64 = 32 banks)
 constexpr int PW = 16;   // panel width inside a diagonal block
+// Block columns go in pairs while at least this many trailing block columns
+// remain: the trailing lower triangle is then larger than the 256 MB MALL
+// (64 blocks of 128 doubles: 8192^2 / 2 x 8 bytes), so its HBM traffic, not
+// the chain of diagonal factorizations, bounds each step.  Measured on
+// MI355X: D = 44,994 factors in 560 ms in pairs vs 673 ms in single steps;
+// D = 7,994 (always below the threshold) in 7.0 ms vs 7.4 ms in pairs.
+constexpr int PAIR_MIN_TRAIL = 64;
 // The diagonal block in LDS is stored as its 36 lower 16x16 tiles, tile
 // (ti, tj), ti >= tj, at (ti (ti + 1) / 2 + tj) * 256, each column-major
 // with stride 16: every MFMA operand or accumulator access (16 rows of 4
@@ -61,56 +75,70 @@ __device__ __attribute__((always_inline)) inline int doff(int r, int c) {
 // The v_mfma_f64_16x16x4_f64 operand layout: lane l supplies A[l%16][l/16]
 // and B^T[l/16][l%16]; result register v of lane l is D[l/16 + 4v][l%16]
 // (checked on gfx950 with a one-wavefront probe of the builtin).
-// Software-pipelined: the next K chunk's global loads are in flight while
-// the current chunk's MFMAs run.
+// Software-pipelined: the next K chunk moves from global memory to LDS
+// while the current chunk's MFMAs run (two LDS buffers, one barrier per
+// chunk).
+typedef __attribute__((address_space(3))) void lds_void;
+
+// One chunk (KC columns of 128 rows) of A and of B straight from global
+// memory into LDS (global_load_lds_dwordx4: lane l moves 16 bytes to
+// dst + 16 l), column k of the chunk to offset k * KS; wave w moves columns
+// w, w + 4, ...
+__device__ __attribute__((always_inline)) inline void chunk_to_lds(const double* __restrict__ A, size_t lda,
+                                                                    const double* __restrict__ B, size_t ldb,
+                                                                    int k0, double* As, double* Bs) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = wave; k < KC; k += 4) {
+    __builtin_amdgcn_global_load_lds((const void*)(A + (size_t)(k0 + k) * lda + 2 * lane),
+                                     (lds_void*)(As + k * KS), 16, 0, 0);
+    __builtin_amdgcn_global_load_lds((const void*)(B + (size_t)(k0 + k) * ldb + 2 * lane),
+                                     (lds_void*)(Bs + k * KS), 16, 0, 0);
+  }
+}
+
+__device__ __attribute__((always_inline)) inline void tile_mma_begin(const double* __restrict__ A, size_t lda,
+                                                                      const double* __restrict__ B, size_t ldb,
+                                                                      double* As) {
+  __syncthreads();
+  chunk_to_lds(A, lda, B, ldb, 0, As, As + KC * KS);
+}
+
 __device__ inline void tile_mma(double4_t acc[4][4], const double* __restrict__ A, size_t lda,
                                 const double* __restrict__ B, size_t ldb, int K, double sign,
                                 double* As, double* Bs) {
+  (void)Bs;
   const int t = threadIdx.x;
   const int lane = t & 63, wave = t >> 6;
   const int wr = wave >> 1, wc = wave & 1;
-  constexpr int PER = (KC * 128) / 256;
-  double ra[PER], rb[PER];
-#pragma unroll
-  for (int it = 0; it < PER; ++it) {
-    const int idx = t + 256 * it;
-    const int k = idx >> 7, row = idx & 127;
-    ra[it] = A[(size_t)k * lda + row];
-    rb[it] = B[(size_t)k * ldb + row];
-  }
+  // chunk 0 was issued by the caller (tile_mma_begin)
+  __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  __syncthreads();
+  int buf = 0;
   for (int k0 = 0; k0 < K; k0 += KC) {
-    __syncthreads();
-#pragma unroll
-    for (int it = 0; it < PER; ++it) {
-      const int idx = t + 256 * it;
-      const int k = idx >> 7, row = idx & 127;
-      As[k * KS + row] = ra[it];
-      Bs[k * KS + row] = rb[it];
-    }
-    __syncthreads();
     if (k0 + KC < K) {
-#pragma unroll
-      for (int it = 0; it < PER; ++it) {
-        const int idx = t + 256 * it;
-        const int k = idx >> 7, row = idx & 127;
-        ra[it] = A[(size_t)(k0 + KC + k) * lda + row];
-        rb[it] = B[(size_t)(k0 + KC + k) * ldb + row];
-      }
+      double* An = As + (buf ^ 1) * 2 * KC * KS;
+      chunk_to_lds(A, lda, B, ldb, k0 + KC, An, An + KC * KS);
     }
+    const double* Ac = As + buf * 2 * KC * KS;
+    const double* Bc = Ac + KC * KS;
 #pragma unroll
     for (int kk = 0; kk < KC; kk += 4) {
       double a[4], b[4];
       const int kr = kk + (lane >> 4);
 #pragma unroll
-      for (int m = 0; m < 4; ++m) a[m] = sign * As[kr * KS + wr * 64 + m * 16 + (lane & 15)];
+      for (int m = 0; m < 4; ++m) a[m] = sign * Ac[kr * KS + wr * 64 + m * 16 + (lane & 15)];
 #pragma unroll
-      for (int n = 0; n < 4; ++n) b[n] = Bs[kr * KS + wc * 64 + n * 16 + (lane & 15)];
+      for (int n = 0; n < 4; ++n) b[n] = Bc[kr * KS + wc * 64 + n * 16 + (lane & 15)];
 #pragma unroll
       for (int m = 0; m < 4; ++m)
 #pragma unroll
         for (int n = 0; n < 4; ++n)
           acc[m][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[m], b[n], acc[m][n], 0, 0, 0);
     }
+    __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    buf ^= 1;
   }
 }
 
@@ -480,29 +508,59 @@ __global__ __launch_bounds__(64) void k_trsm_panel(double* __restrict__ S, size_
   }
 }
 
-// A_ij -= L_ik L_jk^T for k < j <= i, j < nblk, i <= nblk (i == nblk: rhs
-// block).  The workgroup of the next diagonal tile (k+1, k+1) factors it
-// right away (lookahead), so the next step's panel solve can start.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_syrk_update(double* __restrict__ S, size_t ld, int k,
-                                                     int nblk, uint32_t D, uint32_t Dp,
-                                                     double* __restrict__ Top,
-                                                     int* __restrict__ flags) {
+// A_ij -= sum over the panels kp .. kp + K/NB - 1 of L_ip L_jp^T, for
+// j0 <= j <= i, j < nblk, i <= nblk (i == nblk: the rhs block); a grid of
+// one column (gridDim.x == 1) updates block column j0 only.  The workgroup
+// of the diagonal tile (j0, j0) factors it right away (lookahead), so the
+// next panel solve can start.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_syrk_update(
+    double* __restrict__ S, size_t ld, int kp, int K, int j0, int nblk, uint32_t D, uint32_t Dp,
+    double* __restrict__ Top, int* __restrict__ flags) {
   __shared__ double smem[SMEM_DIAG];
-  // Workgroup (0, 0) is the next diagonal tile (k+1, k+1): dispatched first,
-  // so its factorization overlaps the other tiles' updates.
-  const int j = k + 1 + blockIdx.x;
-  const int i = k + 1 + blockIdx.y;
+  // Workgroup (0, 0) is the diagonal tile (j0, j0): dispatched first, so its
+  // factorization overlaps the other tiles' updates.
+  const int j = j0 + blockIdx.x;
+  const int i = j0 + blockIdx.y;
   if (j > i || j >= nblk) return;
   double* C = S + (size_t)j * NB * ld + (size_t)i * NB;
-  const double* Ai = S + (size_t)k * NB * ld + (size_t)i * NB;
-  const double* Aj = S + (size_t)k * NB * ld + (size_t)j * NB;
+  const double* Ai = S + (size_t)kp * NB * ld + (size_t)i * NB;
+  const double* Aj = S + (size_t)kp * NB * ld + (size_t)j * NB;
   double4_t acc[4][4];
-  tile_load_t(acc, C, ld);
-  tile_mma(acc, Aj, ld, Ai, ld, NB, -1.0, smem, smem + KC * KS);
-  if (i == k + 1 && j == k + 1) {
+  // The workgroups of the second dispatch round (two per CU) take C after
+  // the products rather than before, so the two workgroups sharing a CU do
+  // not load C at the same time (a scheduling heuristic; either order gives
+  // C - sum L L^T).
+  const uint32_t lin = blockIdx.y * gridDim.x + blockIdx.x;
+  const bool late = (lin >> 8) & 1u;
+  tile_mma_begin(Aj, ld, Ai, ld, smem);
+  if (!late) {
+    tile_load_t(acc, C, ld);
+  } else {
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+      for (int n = 0; n < 4; ++n) acc[m][n] = double4_t{0.0, 0.0, 0.0, 0.0};
+  }
+  tile_mma(acc, Aj, ld, Ai, ld, K, -1.0, smem, smem + KC * KS);
+  if (late) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      double cm[4][4];
+#pragma unroll
+      for (int n = 0; n < 4; ++n)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) cm[n][v] = C[(size_t)acc_col(m, v, lane) * ld + acc_row(n, lane)];
+#pragma unroll
+      for (int n = 0; n < 4; ++n)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[m][n][v] += cm[n][v];
+    }
+  }
+  if (i == j0 && j == j0) {
     __syncthreads();
     tile_to_lds(acc, smem, (uint32_t)j * NB, D);
-    diag_factor(smem, S, ld, k + 1, D, Top, flags);
+    diag_factor(smem, S, ld, j0, D, Top, flags);
     return;
   }
   tile_store_t(acc, C, ld, i == j, (uint32_t)i * NB, (uint32_t)j * NB, D, Dp);
@@ -742,24 +800,42 @@ void dense_cholesky(double* S, double* Linv, uint32_t Dp, uint32_t ld, uint32_t 
                     hipStream_t s, KernelTimer* kt) {
   const int nblk = (int)(Dp / NB);
   double* Top = Linv + (size_t)nblk * LINV_BLOCK;
-  hipLaunchKernelGGL(k_potrf_block, dim3(1), dim3(256), 0, s, S, (size_t)ld, 0, D, Top, flags);
-  for (int k = 0; k < nblk; ++k) {
+  auto trsm = [&](int k) {
     // 8 strips per row block k+1 .. nblk-1, one for the rhs row
     const int strips = 8 * (nblk - 1 - k) + 1;
     hipLaunchKernelGGL(k_trsm_panel, dim3(strips), dim3(64), 0, s, S, (size_t)ld, k, Top, D, Dp);
+  };
+  // Trailing update with panels kp .. kp + np - 1 of block columns j0 ..
+  // (all of them, or j0 only); factors diagonal block j0.
+  auto syrk = [&](int kp, int np, int j0, bool column_only) {
+    const int ncol = column_only ? 1 : nblk - j0;
+    hipEvent_t e0 = kt ? kt->before(s) : nullptr;
+    hipLaunchKernelGGL(k_syrk_update, dim3(ncol, nblk - j0 + 1), dim3(256), 0, s, S, (size_t)ld, kp, np * NB, j0,
+                       nblk, D, Dp, Top, flags);
+    if (kt) {
+      // algorithmic FLOP: the panel columns that carry data (nk) times the
+      // lower-triangle entries updated (rows >= j0 NB, below D) and the rhs
+      // row, plus the factorization of the diagonal block
+      const double nk = std::max(0.0, std::min<double>(np * NB, (double)D - (double)kp * NB));
+      const double m = std::max(0.0, (double)D - (double)j0 * NB);
+      const double nb = std::min<double>(NB, m);
+      const double entries = column_only ? nb * m - nb * (nb - 1.0) / 2.0 + nb : m * (m + 1.0) / 2.0 + m;
+      kt->after(e0, s, 2.0 * nk * entries + nb * nb * nb / 3.0);
+    }
+  };
+  hipLaunchKernelGGL(k_potrf_block, dim3(1), dim3(256), 0, s, S, (size_t)ld, 0, D, Top, flags);
+  int k = 0;
+  while (k < nblk) {
+    trsm(k);
     const int ntrail = nblk - 1 - k;
-    if (ntrail > 0) {
-      hipEvent_t e0 = kt ? kt->before(s) : nullptr;
-      hipLaunchKernelGGL(k_syrk_update, dim3(ntrail, ntrail + 1), dim3(256), 0, s, S, (size_t)ld, k,
-                         nblk, D, Dp, Top, flags);
-      if (kt) {
-        // algorithmic FLOP: lower-triangle update of the m trailing rows with
-        // the panel's nk columns, the rhs row, and the next diagonal block
-        const double nk = std::min<double>(NB, (double)D - (double)k * NB);
-        const double m = std::max(0.0, (double)D - (double)(k + 1) * NB);
-        const double nb = std::min<double>(NB, m);
-        kt->after(e0, s, nk * m * (m + 1.0) + 2.0 * nk * m + nb * nb * nb / 3.0);
-      }
+    if (ntrail >= PAIR_MIN_TRAIL) {
+      syrk(k, 1, k + 1, true);   // block column k+1 with panel k; factors diagonal k+1
+      trsm(k + 1);
+      syrk(k, 2, k + 2, false);  // the rest with panels k and k+1; factors diagonal k+2
+      k += 2;
+    } else {
+      if (ntrail > 0) syrk(k, 1, k + 1, false);
+      k += 1;
     }
   }
   hipLaunchKernelGGL(k_block_inverse, dim3(nblk), dim3(256), 0, s, S, (size_t)ld, D, Top, Linv);
