@@ -343,24 +343,41 @@ __device__ __attribute__((always_inline)) inline void diag_factor(double* A, dou
   for (int p = 0; p < NB / PW; ++p) {
     panel_factor(A, PW * p, base, D, flags);
     __syncthreads();
-    // Column p of tiles is final: its global stores (L into S, -L tiles
-    // into Top) drain while the trailing update runs.
-    for (int idx = t; idx < (NB / PW - p) * 256; idx += blockDim.x) {
-      const int ti = p + (idx >> 8), e = idx & 255;
-      const int rr = e & 15, cc = e >> 4;
-      const double v = A[tile_index(ti, p) * 256 + e];
-      if (ti > p) T[(ti * (ti - 1) / 2 + p) * 256 + e] = -v;
-      const int r = PW * ti + rr;
-      if ((ti > p || rr >= cc) && base + r < D) G[(size_t)(PW * p + cc) * ld + r] = v;
-    }
     if (p + 1 < NB / PW) {
       lds_trailing_update(A, p);
       __syncthreads();
     }
   }
-  // X11_p by forward substitution, one thread per (p, column c), straight
-  // to Top.
-  if (t < 128) {
+  // Threads 0..127: X11_p by forward substitution, one thread per (p,
+  // column c), straight to Top.  Threads 128..255 meanwhile store L (lower
+  // triangle, real rows) into S and the -L tiles into Top, 16 bytes at a
+  // time for the 28 off-diagonal tiles.
+  if (t >= 128) {
+    const int u = t - 128;
+#pragma unroll 4
+    for (int it = 0; it < 28; ++it) {
+      // Top tile it = ti (ti - 1) / 2 + tj, ti > tj (uniform per iteration)
+      int ti = 1;
+      while ((ti + 1) * ti / 2 <= it) ++ti;
+      const int tj = it - ti * (ti - 1) / 2;
+      const int e = 2 * u, rr = e & 15, cc = e >> 4;
+      const double2 v = *reinterpret_cast<const double2*>(A + tile_index(ti, tj) * 256 + e);
+      *reinterpret_cast<double2*>(T + it * 256 + e) = make_double2(-v.x, -v.y);
+      const uint32_t r = base + PW * ti + rr;
+      double* g = G + (size_t)(PW * tj + cc) * ld + PW * ti + rr;
+      if (r + 1 < D) {
+        *reinterpret_cast<double2*>(g) = v;
+      } else if (r < D) {
+        g[0] = v.x;
+      }
+    }
+#pragma unroll 2
+    for (int idx = u; idx < 8 * 256; idx += 128) {
+      const int ti = idx >> 8, e = idx & 255, rr = e & 15, cc = e >> 4;
+      if (rr < cc || base + PW * ti + rr >= D) continue;
+      G[(size_t)(PW * ti + cc) * ld + PW * ti + rr] = A[tile_index(ti, ti) * 256 + e];
+    }
+  } else {
     const int p = t >> 4, c = t & 15;
     const double* Lpp = A + tile_index(p, p) * 256;
     const double* ip = A + DIAG_IP + PW * p;
