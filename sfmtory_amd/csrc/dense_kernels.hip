@@ -617,40 +617,118 @@ __global__ __launch_bounds__(256) void k_block_inverse(const double* __restrict_
 }
 
 // ------------------------------------------------------ triangular solves
-// Backward solve L^T x = y, block k (last to first), in two launches:
-// x_k = Linv_k^T y_k (one workgroup), then y_c -= L[block k, c] . x_k for
-// every column c < k*NB.
-__global__ __launch_bounds__(128) void k_backsolve_block(const double* __restrict__ Linv, int k,
-                                                         const double* __restrict__ y,
-                                                         double* __restrict__ x) {
-  __shared__ double ys[NB];
-  const int t = threadIdx.x;
-  ys[t] = y[(size_t)k * NB + t];
-  __syncthreads();
-  // x[t] = sum_r Linv[r][t] y[r] = sum_r LinvT[t][r] y[r]; LinvT column-major
-  // makes LinvT[t][r] = XT[r*NB + t] contiguous across threads.
-  const double* XT = Linv + (size_t)k * LINV_BLOCK + NB * NB;
-  double s = 0.0;
-#pragma unroll 8
-  for (int r = 0; r < NB; ++r) s += XT[(size_t)r * NB + t] * ys[r];
-  x[(size_t)k * NB + t] = s;
+// Back substitution x = L^-T y in one launch.  Workgroup w owns block
+// j = nblk - 1 - w: it applies every solved block x_k (k > j) to its part of
+// y as soon as x_k is published, y_j -= L_kj^T x_k, then solves
+// x_j = L_jj^-T y_j and publishes it (x_j stored, then chain[j] = epoch with
+// release semantics at device scope).  A workgroup waits only on blocks
+// owned by workgroups dispatched before it, so the waits cannot deadlock;
+// each wait is bounded (about a quarter of a second), and on timeout
+// status is set and the workgroup stops waiting for the rest of the launch
+// (the host reports the failure), so a broken hand-over costs one time-out.
+//
+// The critical path per block is one flag hand-over, one multiply of an
+// L block already held in registers (it is loaded while the workgroup
+// waits), a 32-column reduction and one solve against L_jj^-T held in LDS.
+constexpr uint32_t CHAIN_SPIN_LIMIT = 1u << 18;
+constexpr int CHAIN_COLS = NB / 4;  // columns of L_kj per wavefront
+
+// Lane l of wavefront w holds rows l and l + 64 of columns w*32 .. w*32+31
+// of block (row k, column j); each load is 512 contiguous bytes.
+__device__ __forceinline__ void chain_load(const double* __restrict__ S, size_t ld, int j, int k, int wave,
+                                           int lane, double (&a)[CHAIN_COLS], double (&b)[CHAIN_COLS]) {
+  const double* base = S + (size_t)(j * NB + wave * CHAIN_COLS) * ld + (size_t)k * NB + lane;
+#pragma unroll
+  for (int c = 0; c < CHAIN_COLS; ++c) {
+    a[c] = base[(size_t)c * ld];
+    b[c] = base[(size_t)c * ld + 64];
+  }
 }
 
-__global__ __launch_bounds__(256) void k_backsolve_update(const double* __restrict__ S, size_t ld, int k,
-                                                          const double* __restrict__ x,
-                                                          double* __restrict__ y) {
-  __shared__ double xs[NB];
-  const int t = threadIdx.x;
-  if (t < NB) xs[t] = x[(size_t)k * NB + t];
-  __syncthreads();
-  const int ncols = k * NB;
-  const int lane = t & 63, wave = t >> 6;
-  for (int c = blockIdx.x * 4 + wave; c < ncols; c += gridDim.x * 4) {
-    const double* col = S + (size_t)c * ld + (size_t)k * NB;
-    double s = col[lane] * xs[lane] + col[lane + 64] * xs[lane + 64];
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-    if (lane == 0) y[c] -= s;
+// One step of a recursive-halving reduction of H*2 values per lane across
+// lane pairs (l, l ^ O): lanes with bit O clear keep the lower H values,
+// lanes with it set the upper H, each summed over the pair.
+template <int O, int H>
+__device__ __forceinline__ void halve(double (&p)[CHAIN_COLS], int lane) {
+  const bool up = (lane & O) != 0;
+#pragma unroll
+  for (int i = 0; i < H; ++i) {
+    const double send = up ? p[i] : p[i + H];
+    const double keep = up ? p[i + H] : p[i];
+    p[i] = keep + __shfl_xor(send, O);
   }
+}
+
+__global__ __launch_bounds__(256) void k_backsolve_chain(const double* __restrict__ S, size_t ld,
+                                                         const double* __restrict__ Linv,
+                                                         const double* __restrict__ y, double* __restrict__ x,
+                                                         uint32_t* __restrict__ chain, uint32_t epoch, int nblk,
+                                                         int* __restrict__ status) {
+  __shared__ double xt[NB * NB];  // L_jj^-T, column-major
+  __shared__ double ys[NB];
+  __shared__ double xs[NB];
+  __shared__ double part[2 * NB];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int j = nblk - 1 - (int)blockIdx.x;
+  double a[CHAIN_COLS], b[CHAIN_COLS];
+  if (j + 1 < nblk) chain_load(S, ld, j, nblk - 1, wave, lane, a, b);
+  {
+    const double2* src = reinterpret_cast<const double2*>(Linv + (size_t)j * LINV_BLOCK + NB * NB);
+    double2* dst = reinterpret_cast<double2*>(xt);
+#pragma unroll 8
+    for (int e = t; e < NB * NB / 2; e += 256) dst[e] = src[e];
+  }
+  if (t < NB) ys[t] = y[(size_t)j * NB + t];
+  bool timed_out = false;  // thread 0 only
+  for (int k = nblk - 1; k > j; --k) {
+    if (t == 0 && !timed_out) {
+      uint32_t spins = 0;
+      while (__hip_atomic_load(&chain[k], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+        __builtin_amdgcn_s_sleep(2);
+        if (++spins > CHAIN_SPIN_LIMIT) {
+          __hip_atomic_fetch_or(status, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          timed_out = true;
+          break;
+        }
+      }
+    }
+    __syncthreads();
+    if (t < NB) xs[t] = __hip_atomic_load(&x[(size_t)k * NB + t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    const double x0 = xs[lane], x1 = xs[lane + 64];
+    double p[CHAIN_COLS];
+#pragma unroll
+    for (int c = 0; c < CHAIN_COLS; ++c) p[c] = a[c] * x0 + b[c] * x1;
+    if (k - 1 > j) chain_load(S, ld, j, k - 1, wave, lane, a, b);
+    // column (lane >> 1) & 31 of this wavefront's 32 ends in lanes 2m, 2m+1
+    halve<32, 16>(p, lane);
+    halve<16, 8>(p, lane);
+    halve<8, 4>(p, lane);
+    halve<4, 2>(p, lane);
+    halve<2, 1>(p, lane);
+    const double v = p[0] + __shfl_xor(p[0], 1);
+    if ((lane & 1) == 0) ys[wave * CHAIN_COLS + (lane >> 1)] -= v;
+  }
+  __syncthreads();
+  // x[c] = sum_r L_jj^-1[r][c] y[r] = sum_r xt[r*NB + c] y[r]; threads
+  // t and t + 128 take the two halves of r.
+  {
+    const int c = t & (NB - 1), r0 = (t >> 7) * 64;
+    double acc = 0.0;
+#pragma unroll 16
+    for (int r = r0; r < r0 + 64; ++r) acc += xt[r * NB + c] * ys[r];
+    part[t] = acc;
+  }
+  __syncthreads();
+  if (t < NB) {
+    __hip_atomic_store(&x[(size_t)j * NB + t], part[t] + part[t + NB], __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+    // every storing wavefront completes its stores at device scope before
+    // the barrier that precedes the flag
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  }
+  __syncthreads();
+  if (t == 0) __hip_atomic_store(&chain[j], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Forward solve L y = b: y_k = Linv_k b_k (one workgroup), then
@@ -866,20 +944,12 @@ void dense_extract_row(const double* S, uint32_t Dp, uint32_t ld, uint32_t row, 
   HIP_CHECK(hipGetLastError());
 }
 
-void dense_backsolve(const double* S, const double* Linv, uint32_t Dp, uint32_t ld, double* y,
-                     double* x, hipStream_t s) {
-  // y is consumed as scratch (its blocks are reduced in place); x receives
-  // the solution.
+void dense_backsolve(const double* S, const double* Linv, uint32_t Dp, uint32_t ld, const double* y,
+                     double* x, uint32_t* chain, uint32_t epoch, int* status, hipStream_t s) {
   const int nblk = (int)(Dp / NB);
-  for (int k = nblk - 1; k >= 0; --k) {
-    hipLaunchKernelGGL(k_backsolve_block, dim3(1), dim3(NB), 0, s, Linv, k, y, x);
-    const int ncols = k * NB;
-    if (ncols > 0) {
-      int grid = (ncols + 15) / 16;
-      if (grid > 2048) grid = 2048;
-      hipLaunchKernelGGL(k_backsolve_update, dim3(grid), dim3(256), 0, s, S, (size_t)ld, k, x, y);
-    }
-  }
+  if (nblk == 0) return;
+  hipLaunchKernelGGL(k_backsolve_chain, dim3(nblk), dim3(256), 0, s, S, (size_t)ld, Linv, y, x, chain, epoch,
+                     nblk, status);
   HIP_CHECK(hipGetLastError());
 }
 

@@ -135,7 +135,9 @@ struct Work {
   double* parts = nullptr;    // partial blocks of multi-chunk destinations
   double* cost_part = nullptr;
   double* cost = nullptr;     // [0] cost, [1] sum |r|, [2] count
-  int* flags = nullptr;       // [0] cholesky failure, [1] lu failure
+  int* flags = nullptr;       // [0] cholesky failure, [1] lu failure, [2] back-solve hand-over timeout
+  uint32_t* chain = nullptr;  // Dp / NB back-substitution flags
+  uint32_t chain_epoch = 0;   // last epoch used with chain
   double* h_pinned = nullptr; // pinned host mirror of cost + flags
 };
 
@@ -221,9 +223,17 @@ struct KernelTimer {
 // kt (optional) times every trailing-update launch.
 void dense_cholesky(double* S, double* Linv, uint32_t Dp, uint32_t ld, uint32_t D, int* flags,
                     hipStream_t s, KernelTimer* kt = nullptr);
-// Solve L^T x = y; y (Dp) is consumed as scratch, x (Dp) receives the result.
-void dense_backsolve(const double* S, const double* Linv, uint32_t Dp, uint32_t ld, double* y,
-                     double* x, hipStream_t s);
+// Solve L^T x = y (Dp entries each) in one launch.  chain holds Dp / NB
+// flags (zero at allocation); epoch must differ from every value passed
+// before with the same chain (a per-buffer counter, never 0).  status
+// receives a non-zero value if a block's hand-over timed out.
+void dense_backsolve(const double* S, const double* Linv, uint32_t Dp, uint32_t ld, const double* y,
+                     double* x, uint32_t* chain, uint32_t epoch, int* status, hipStream_t s);
+// The next epoch of a chain buffer (skips 0, the value at allocation).
+inline uint32_t next_epoch(uint32_t& e) {
+  if (++e == 0) e = 1;
+  return e;
+}
 // Solve L y = b; b (Dp) is consumed as scratch, y (Dp) receives the result.
 void dense_forwardsolve(const double* S, const double* Linv, uint32_t Dp, uint32_t ld, double* b,
                         double* y, hipStream_t s);

@@ -572,6 +572,8 @@ static void alloc_work(sfmba_problem& H, Work<T>& W) {
   W.cost = A.alloc<double>(8);
   W.flags = A.alloc<int>(4);
   if (L.D == 0) return;
+  W.chain = A.alloc<uint32_t>(L.Dp / NB);
+  HIP_CHECK(hipMemsetAsync(W.chain, 0, (L.Dp / NB) * sizeof(uint32_t), s));
   W.J4 = A.alloc<T>(4 * (size_t)L.n_rows);
   W.E4 = A.alloc<T>(4 * (size_t)L.n_rows);
   W.F4 = A.alloc<T>(4 * (size_t)L.n_rows);
@@ -681,13 +683,13 @@ static void solve(sfmba_problem& H, Work<T>& W, const sfmba_params* prm, const s
       prof.end();
       prof.begin(3);
       dense_extract_row(W.S, L.Dp, L.ld, L.Dp, W.vec, s);
-      dense_backsolve(W.S, W.Linv, L.Dp, L.ld, W.vec, W.delta, s);
+      dense_backsolve(W.S, W.Linv, L.Dp, L.ld, W.vec, W.delta, W.chain, next_epoch(W.chain_epoch), W.flags + 2, s);
       for (int it = 0; refine && it < opt->refine_steps; ++it) {
         // FP64 iterative refinement of the step: r = rhs - S delta,
         // delta += S^{-1} r with the same factor.
         dense_symv_residual(W.S_copy, L.Dp, L.ld, W.delta, W.rhs, W.vec, s);
         dense_forwardsolve(W.S, W.Linv, L.Dp, L.ld, W.vec, H.vec2, s);
-        dense_backsolve(W.S, W.Linv, L.Dp, L.ld, H.vec2, H.vec3, s);
+        dense_backsolve(W.S, W.Linv, L.Dp, L.ld, H.vec2, H.vec3, W.chain, next_epoch(W.chain_epoch), W.flags + 2, s);
         dense_vec_add(W.delta, H.vec3, L.Dp, s);
       }
       prof.end();
@@ -699,6 +701,8 @@ static void solve(sfmba_problem& H, Work<T>& W, const sfmba_params* prm, const s
       HIP_CHECK(hipMemcpyAsync(h + 4, W.flags, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
       HIP_CHECK(hipStreamSynchronize(s));
       trial = h[0];
+      if (reinterpret_cast<const int*>(h + 4)[2] != 0)
+        throw Error(SFMBA_ERR_DEVICE, "back substitution hand-over timed out");
       return reinterpret_cast<const int*>(h + 4)[0] != 0;
     };
     // Returns false when no step could be formed (the try counts as rejected).
@@ -1676,11 +1680,13 @@ int sfmba_dense_solve(const double* A, size_t n, const double* b, int32_t method
     double* xd = ar.alloc<double>(Dp);
     int* flags = ar.alloc<int>(4);
     int* piv = ar.alloc<int>(Dp);
+    uint32_t* chain = ar.alloc<uint32_t>(Dp / NB);
     HIP_CHECK(hipMemsetAsync(flags, 0, 4 * sizeof(int), sg.s));
+    HIP_CHECK(hipMemsetAsync(chain, 0, (Dp / NB) * sizeof(uint32_t), sg.s));
     if (method == 0) {
       dense_cholesky(S, Linv, Dp, ld, D, flags, sg.s);
       dense_extract_row(S, Dp, ld, Dp, y, sg.s);
-      dense_backsolve(S, Linv, Dp, ld, y, xd, sg.s);
+      dense_backsolve(S, Linv, Dp, ld, y, xd, chain, 1, flags + 2, sg.s);
     } else {
       dense_lu_solve(S, Dp, ld, D, xd, piv, flags, sg.s);
     }
@@ -1688,6 +1694,7 @@ int sfmba_dense_solve(const double* A, size_t n, const double* b, int32_t method
     HIP_CHECK(hipMemcpyAsync(hf, flags, sizeof(hf), hipMemcpyDeviceToHost, sg.s));
     HIP_CHECK(hipMemcpyAsync(x, xd, n * sizeof(double), hipMemcpyDeviceToHost, sg.s));
     HIP_CHECK(hipStreamSynchronize(sg.s));
+    if (hf[2] != 0) throw Error(SFMBA_ERR_DEVICE, "back substitution hand-over timed out");
     *status = method == 0 ? (hf[0] != 0) : (hf[1] != 0);
   });
 }
