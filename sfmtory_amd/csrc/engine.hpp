@@ -133,6 +133,7 @@ struct Work {
   double* vec = nullptr;      // Dp scratch
   double* rhs = nullptr;      // Dp
   double* parts = nullptr;    // partial blocks of multi-chunk destinations
+  double* inc_part = nullptr; // 3 per incidence: E^T delta (point back-substitution)
   double* cost_part = nullptr;
   double* cost = nullptr;     // [0] cost, [1] sum |r|, [2] count
   int* flags = nullptr;       // [0] cholesky failure, [1] lu failure, [2] back-solve hand-over timeout

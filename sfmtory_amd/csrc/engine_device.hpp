@@ -303,6 +303,28 @@ __device__ inline void compute_F(const View<T>& v, uint32_t a) {
   }
 }
 
+// Row r of incidence a, for the grid kernel: the lanes of one incidence
+// read and write its consecutive rows in one instruction (whole cache
+// lines), and share the point's Cinv and g by broadcast.
+template <typename T>
+__device__ inline void compute_F_row(const View<T>& v, uint32_t a, uint32_t r) {
+  using V4 = typename Vec4<T>::type;
+  const uint32_t n = v.grp_dim[v.inc_grp[a]];
+  if (r >= n) return;
+  const uint32_t pt = v.inc_pt[a];
+  const double* ci = v.pt_cinv + 9 * (size_t)pt;
+  T c[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) c[k] = (T)ci[k];
+  const T g0 = (T)v.pt_g[3 * (size_t)pt], g1 = (T)v.pt_g[3 * (size_t)pt + 1],
+          g2 = (T)v.pt_g[3 * (size_t)pt + 2];
+  const V4 e = v.E4[v.inc_roff[a] + r];
+  const T f0 = e.x * c[0] + e.y * c[3] + e.z * c[6];
+  const T f1 = e.x * c[1] + e.y * c[4] + e.z * c[7];
+  const T f2 = e.x * c[2] + e.y * c[5] + e.z * c[8];
+  v.F4[v.inc_froff[a] + r] = V4{f0, f1, f2, f0 * g0 + f1 * g1 + f2 * g2};
+}
+
 constexpr int PART_STRIDE = 2 * 196 + 16;
 
 // One wavefront reduces one chunk of a destination block (ga >= gb) of the

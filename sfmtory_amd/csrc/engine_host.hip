@@ -591,6 +591,7 @@ static void alloc_work(sfmba_problem& H, Work<T>& W) {
   W.vec = A.alloc<double>(L.Dp);
   W.rhs = A.alloc<double>(L.Dp);
   W.parts = A.alloc<double>((size_t)std::max<uint32_t>(L.n_parts, 1) * (2 * 196 + 16));
+  W.inc_part = A.alloc<double>(3 * (size_t)std::max<uint32_t>(L.n_inc, 1));
   // S: zero once, identity on the padding diagonal; the kernels rewrite the
   // D x D lower triangle and the rhs row on every try and never touch the
   // padding.

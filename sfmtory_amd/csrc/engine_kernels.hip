@@ -27,10 +27,13 @@ __global__ __launch_bounds__(256) void k_point_prepare(View<T> v, double lambda)
   if (p < v.n_pt) point_prepare(v, p, lambda);
 }
 
-template <typename T>
+// LPI lanes per incidence (a power of two >= its group dimension), one row
+// per lane.
+template <typename T, int LPI>
 __global__ __launch_bounds__(256) void k_compute_F(View<T> v) {
-  const uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
-  if (a < v.n_inc) compute_F(v, a);
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t a = t / LPI;
+  if (a < v.n_inc) compute_F_row(v, a, t % LPI);
 }
 
 // The Schur reduction on the matrix cores, one wavefront per chunk, chunks
@@ -77,10 +80,62 @@ __global__ __launch_bounds__(256) void k_schur_reduce(
   }
 }
 
+// Point back-substitution in two passes.  First E_a^T delta_g per
+// incidence a: LPI lanes per incidence, one row per lane (its rows are
+// read as whole cache lines), summed across the lanes.
+template <typename T, int LPI>
+__global__ __launch_bounds__(256) void k_backsub_inc(View<T> v, double* __restrict__ inc_part) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t a = t / LPI, r = t % LPI;
+  double c0 = 0.0, c1 = 0.0, c2 = 0.0;
+  if (a < v.n_inc) {
+    const uint32_t g = v.inc_grp[a];
+    if (r < v.grp_dim[g]) {
+      const auto e = v.E4[v.inc_roff[a] + r];
+      const double d = v.delta[v.grp_off[g] + r];
+      c0 = (double)e.x * d;
+      c1 = (double)e.y * d;
+      c2 = (double)e.z * d;
+    }
+  }
+  // every lane of the wavefront takes part (LPI divides 64)
+#pragma unroll
+  for (int o = LPI / 2; o > 0; o >>= 1) {
+    c0 += __shfl_xor(c0, o);
+    c1 += __shfl_xor(c1, o);
+    c2 += __shfl_xor(c2, o);
+  }
+  if (a < v.n_inc && r == 0) {
+    inc_part[3 * (size_t)a] = c0;
+    inc_part[3 * (size_t)a + 1] = c1;
+    inc_part[3 * (size_t)a + 2] = c2;
+  }
+}
+
+// Then per point: dp = -Cinv (g + sum over its incidences), added to the
+// point (a point left out of the Schur complement keeps its position).
 template <typename T>
-__global__ __launch_bounds__(256) void k_backsub(View<T> v, int cur) {
+__global__ __launch_bounds__(256) void k_backsub(View<T> v, int cur, const double* __restrict__ inc_part) {
   const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p < v.n_pt) backsub_point(v, p, cur);
+  if (p >= v.n_pt) return;
+  const double* X = v.pts[cur] + 3 * (size_t)p;
+  double* Y = v.pts[1 - cur] + 3 * (size_t)p;
+  if (!v.pt_ok[p]) {
+    Y[0] = X[0];
+    Y[1] = X[1];
+    Y[2] = X[2];
+    return;
+  }
+  double acc0 = v.pt_g[3 * (size_t)p], acc1 = v.pt_g[3 * (size_t)p + 1], acc2 = v.pt_g[3 * (size_t)p + 2];
+  for (uint32_t a = v.pt_inc_ptr[p]; a < v.pt_inc_ptr[p + 1]; ++a) {
+    acc0 += inc_part[3 * (size_t)a];
+    acc1 += inc_part[3 * (size_t)a + 1];
+    acc2 += inc_part[3 * (size_t)a + 2];
+  }
+  const double* ci = v.pt_cinv + 9 * (size_t)p;
+  Y[0] = X[0] + -(ci[0] * acc0 + ci[1] * acc1 + ci[2] * acc2);
+  Y[1] = X[1] + -(ci[3] * acc0 + ci[4] * acc1 + ci[5] * acc2);
+  Y[2] = X[2] + -(ci[6] * acc0 + ci[7] * acc1 + ci[8] * acc2);
 }
 
 template <typename T>
@@ -238,6 +293,12 @@ __global__ __launch_bounds__(256) void k_residual_norms(
 
 // ------------------------------------------------------------- launchers
 static inline uint32_t blocks_for(uint64_t n, uint32_t per) { return (uint32_t)((n + per - 1) / per); }
+// Lanes given to one incidence by the row-per-lane kernels: the power of two
+// at or above the largest group dimension.
+static inline uint32_t lanes_per_incidence(const Layout& L) {
+  if (L.max_grp_dim > 16) throw Error(SFMBA_ERR_INVALID_ARGUMENT, "group dimension above 16");
+  return L.max_grp_dim <= 2 ? 2 : L.max_grp_dim <= 4 ? 4 : L.max_grp_dim <= 8 ? 8 : 16;
+}
 
 template <typename T>
 View<T> make_view(const Layout& L, Work<T>& W) {
@@ -315,7 +376,14 @@ void launch_point_prepare(const Layout& L, Work<T>& W, double lambda, hipStream_
   hipLaunchKernelGGL(k_point_prepare<T>, dim3(blocks_for(L.n_pt, 256)), dim3(256), 0, s, v, lambda);
   HIP_CHECK(hipGetLastError());
   if (L.n_inc == 0) return;
-  hipLaunchKernelGGL(k_compute_F<T>, dim3(blocks_for(L.n_inc, 256)), dim3(256), 0, s, v);
+  const uint32_t lpi = lanes_per_incidence(L);
+  const dim3 grid(blocks_for((uint64_t)L.n_inc * lpi, 256));
+  switch (lpi) {
+    case 2: hipLaunchKernelGGL((k_compute_F<T, 2>), grid, dim3(256), 0, s, v); break;
+    case 4: hipLaunchKernelGGL((k_compute_F<T, 4>), grid, dim3(256), 0, s, v); break;
+    case 8: hipLaunchKernelGGL((k_compute_F<T, 8>), grid, dim3(256), 0, s, v); break;
+    default: hipLaunchKernelGGL((k_compute_F<T, 16>), grid, dim3(256), 0, s, v); break;
+  }
   HIP_CHECK(hipGetLastError());
 }
 
@@ -339,7 +407,17 @@ template <typename T>
 void launch_backsub_update(const Layout& L, Work<T>& W, int cur, hipStream_t s) {
   const View<T> v = make_view(L, W);
   if (L.n_pt) {
-    hipLaunchKernelGGL(k_backsub<T>, dim3(blocks_for(L.n_pt, 256)), dim3(256), 0, s, v, cur);
+    if (L.n_inc) {
+      const dim3 grid(blocks_for((uint64_t)L.n_inc * lanes_per_incidence(L), 256));
+      switch (lanes_per_incidence(L)) {
+        case 2: hipLaunchKernelGGL((k_backsub_inc<T, 2>), grid, dim3(256), 0, s, v, W.inc_part); break;
+        case 4: hipLaunchKernelGGL((k_backsub_inc<T, 4>), grid, dim3(256), 0, s, v, W.inc_part); break;
+        case 8: hipLaunchKernelGGL((k_backsub_inc<T, 8>), grid, dim3(256), 0, s, v, W.inc_part); break;
+        default: hipLaunchKernelGGL((k_backsub_inc<T, 16>), grid, dim3(256), 0, s, v, W.inc_part); break;
+      }
+      HIP_CHECK(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_backsub<T>, dim3(blocks_for(L.n_pt, 256)), dim3(256), 0, s, v, cur, W.inc_part);
     HIP_CHECK(hipGetLastError());
   }
   if (L.n_img) {
