@@ -101,9 +101,14 @@ struct Acc<double> {
 // ---------------------------------------------------------------- linearize
 // Observation o (point-major order): for each of its incidences one J4 and
 // one E4 row per unknown, and its contribution to its point's 3x3 block and
-// gradient (obs_pc).
-template <typename T>
-__device__ inline void linearize_obs(const View<T>& v, uint32_t o, int cur, int loss, double scale) {
+// gradient (obs_pc).  With LPO > 1, LPO lanes share observation o: each
+// computes the same values and lane r writes rows r, r + LPO, ... and
+// obs_pc entries r, r + LPO, ..., so that one store instruction writes
+// whole rows of consecutive observations (a full cache line per
+// observation) instead of one 16- or 32-byte piece per lane.
+template <typename T, int LPO = 1>
+__device__ inline void linearize_obs(const View<T>& v, uint32_t o, int cur, int loss, double scale,
+                                     uint32_t r = 0) {
   using V4 = typename Vec4<T>::type;
   const uint32_t img = v.obs_img[o];
   const uint32_t pt = v.obs_pt[o];
@@ -127,12 +132,14 @@ __device__ inline void linearize_obs(const View<T>& v, uint32_t o, int cur, int 
 
   if (c2 <= 1e-9) {
     // Behind the camera: dropped for this iteration (lib.rs:339-341).
-    v.obs_valid[o] = 0;
+    if (r == 0) v.obs_valid[o] = 0;
 #pragma unroll
-    for (int k = 0; k < 9; ++k) pc_out[k] = T(0);
+    for (uint32_t k0 = 0; k0 < 9; k0 += LPO)
+      if (k0 + r < 9) pc_out[k0 + r] = T(0);
     const V4 z = {T(0), T(0), T(0), T(0)};
 #pragma unroll
-    for (uint32_t s = 0; s < 14; ++s) {
+    for (uint32_t s0 = 0; s0 < 14; s0 += LPO) {
+      const uint32_t s = s0 + r;
       if (s >= ns) break;
       const uint32_t row = (two && s >= 6) ? roff1 + (s - 6) : roff0 + s;
       v.J4[row] = z;
@@ -140,7 +147,7 @@ __device__ inline void linearize_obs(const View<T>& v, uint32_t o, int cur, int 
     }
     return;
   }
-  v.obs_valid[o] = 1;
+  if (r == 0) v.obs_valid[o] = 1;
   const double2 xy = v.obs_xy[o];
   double r0, r1;
   linearization_residual(model, P, c0, c1, c2, xy.x, xy.y, r0, r1);
@@ -159,15 +166,24 @@ __device__ inline void linearize_obs(const View<T>& v, uint32_t o, int cur, int 
   // Point terms: w Jx^T Jx (upper 6), w Jx^T r.
   const T wx0 = w * jx[0], wx1 = w * jx[1], wx2 = w * jx[2];
   const T wy0 = w * jx[3], wy1 = w * jx[4], wy2 = w * jx[5];
-  pc_out[0] = wx0 * jx[0] + wy0 * jx[3];
-  pc_out[1] = wx0 * jx[1] + wy0 * jx[4];
-  pc_out[2] = wx0 * jx[2] + wy0 * jx[5];
-  pc_out[3] = wx1 * jx[1] + wy1 * jx[4];
-  pc_out[4] = wx1 * jx[2] + wy1 * jx[5];
-  pc_out[5] = wx2 * jx[2] + wy2 * jx[5];
-  pc_out[6] = wx0 * tr0 + wy0 * tr1;
-  pc_out[7] = wx1 * tr0 + wy1 * tr1;
-  pc_out[8] = wx2 * tr0 + wy2 * tr1;
+  T pc[9];
+  pc[0] = wx0 * jx[0] + wy0 * jx[3];
+  pc[1] = wx0 * jx[1] + wy0 * jx[4];
+  pc[2] = wx0 * jx[2] + wy0 * jx[5];
+  pc[3] = wx1 * jx[1] + wy1 * jx[4];
+  pc[4] = wx1 * jx[2] + wy1 * jx[5];
+  pc[5] = wx2 * jx[2] + wy2 * jx[5];
+  pc[6] = wx0 * tr0 + wy0 * tr1;
+  pc[7] = wx1 * tr0 + wy1 * tr1;
+  pc[8] = wx2 * tr0 + wy2 * tr1;
+#pragma unroll
+  for (uint32_t k0 = 0; k0 < 9; k0 += LPO) {
+    T val = pc[k0];
+#pragma unroll
+    for (uint32_t k = k0 + 1; k < k0 + LPO && k < 9; ++k)
+      if (k == k0 + r) val = pc[k];
+    if (k0 + r < 9) pc_out[k0 + r] = val;
+  }
 
   // Side vector: free pose columns, then the camera's free parameters.
   T s0[14], s1[14];
@@ -213,11 +229,19 @@ __device__ inline void linearize_obs(const View<T>& v, uint32_t o, int cur, int 
     }
   }
 #pragma unroll
-  for (uint32_t s = 0; s < 14; ++s) {
+  for (uint32_t q0 = 0; q0 < 14; q0 += LPO) {
+    const uint32_t s = q0 + r;
     if (s >= ns) break;
+    T a0 = s0[q0], a1 = s1[q0];
+#pragma unroll
+    for (uint32_t q = q0 + 1; q < q0 + LPO && q < 14; ++q)
+      if (q == s) {
+        a0 = s0[q];
+        a1 = s1[q];
+      }
     const uint32_t row = (two && s >= 6) ? roff1 + (s - 6) : roff0 + s;
-    const T w0 = w * s0[s], w1 = w * s1[s];
-    v.J4[row] = V4{s0[s], s1[s], w0, w1};
+    const T w0 = w * a0, w1 = w * a1;
+    v.J4[row] = V4{a0, a1, w0, w1};
     v.E4[row] = V4{w0 * jx[0] + w1 * jx[3], w0 * jx[1] + w1 * jx[4], w0 * jx[2] + w1 * jx[5],
                    w0 * tr0 + w1 * tr1};
   }

@@ -9,10 +9,16 @@
 
 namespace sfmba {
 
-template <typename T>
+// LPO lanes per observation (rows written as whole cache lines).  Four
+// lanes measured fastest on the 1,000-camera scene (FP32 rows 0.60 -> 0.51
+// ms, FP64 0.94 -> 0.81 ms against one lane; eight lanes: no gain), and
+// they also halve the FP64 kernel's registers (178 -> 111).
+constexpr int LINEARIZE_LANES = 4;
+template <typename T, int LPO>
 __global__ __launch_bounds__(256) void k_linearize(View<T> v, int cur, int loss, double scale) {
-  const uint32_t o = blockIdx.x * blockDim.x + threadIdx.x;
-  if (o < v.n_obs) linearize_obs(v, o, cur, loss, scale);
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t o = t / LPO;
+  if (o < v.n_obs) linearize_obs<T, LPO>(v, o, cur, loss, scale, t % LPO);
 }
 
 template <typename T>
@@ -356,8 +362,9 @@ View<T> make_view(const Layout& L, Work<T>& W) {
 template <typename T>
 void launch_linearize(const Layout& L, Work<T>& W, int cur, int loss, double scale, hipStream_t s) {
   if (L.n_obs == 0) return;
-  hipLaunchKernelGGL(k_linearize<T>, dim3(blocks_for(L.n_obs, 256)), dim3(256), 0, s, make_view(L, W),
-                     cur, loss, scale);
+  constexpr int LPO = LINEARIZE_LANES;
+  hipLaunchKernelGGL((k_linearize<T, LPO>), dim3(blocks_for((uint64_t)L.n_obs * LPO, 256)), dim3(256), 0, s,
+                     make_view(L, W), cur, loss, scale);
   HIP_CHECK(hipGetLastError());
 }
 
