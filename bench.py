@@ -15,7 +15,8 @@ throughput (observations/s), residual statistics and pose error of both
 forms, the roofline of the dominant kernel (the factorization's trailing
 update, timed per launch with HIP events) with the whole factorization and
 the Schur assembly beside it, a CPU baseline from the oracle, the
-256-window batched solve (windows/s) and the 5,000-camera scene solved by
+256-window batched solve (windows/s: a tuned, graph-replayed batch plan and
+one-off batched calls) and the 5,000-camera scene solved by
 PCG and by Cholesky are reported beside it.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--scene c2] [--no-cpu-baseline] [--no-windows]
@@ -135,7 +136,10 @@ def residual_stats(ba, inp, out, inlier_px=10.0):
 
 def windows_section(ba, scenes, threads, cpu=True):
     """Config 3: 256 independent local-BA windows (10-40 SIMPLE_RADIAL
-    cameras, 2k-20k points each) solved by the persistent batched launch,
+    cameras, 2k-20k points each) solved by the persistent batched launch:
+    one-off batched calls (packing and upload included), a batch plan whose
+    launch shape the autotuner picks (3-iteration timing runs) and whose
+    solve is a captured HIP graph replayed from the resident initial state,
     next to a sequential GPU loop (16-window sample) and the oracle on the
     host (4-window sample)."""
     wins = scenes.window_scenes(256)
@@ -152,14 +156,35 @@ def windows_section(ba, scenes, threads, cpu=True):
         t = time.perf_counter()
         seq = [ba.bundle_adjust(i, p, precision=prec) for i in inps[:n]]
         seq_s = time.perf_counter() - t
-        res[prec] = {"batch_windows_per_s": len(inps) / wall,
+        t = time.perf_counter()
+        plan = ba.BatchPlan(inps, p, precision=prec, autotune_iterations=3)
+        plan_create_s = time.perf_counter() - t
+        info = plan.info()
+        plan.solve()
+        reps = 3
+        t = time.perf_counter()
+        for _ in range(reps):
+            pouts = plan.solve()
+        graph_s = (time.perf_counter() - t) / reps
+        plan.close()
+        res[prec] = {"graph_windows_per_s": len(inps) / graph_s, "graph_solve_s": graph_s,
+                     "plan_create_s": plan_create_s,
+                     "launch_shape": {"threads_per_window": info["threads_per_window"],
+                                      "pairs_in_flight": info["pairs_in_flight"]},
+                     "autotune_seconds_per_shape": {f"{x['threads']}x{x['pairs']}": x["seconds"]
+                                                    for x in info["shapes"]},
+                     "graph_vs_batch_max_rel_cost_diff": float(max(abs(a.final_cost - b.final_cost) / b.final_cost
+                                                                   for a, b in zip(pouts, outs))),
+                     "batch_windows_per_s": len(inps) / wall,
                      "batch_kernel_windows_per_s": len(inps) / outs[0].stats["time_total_s"],
                      "sequential_gpu_windows_per_s": n / seq_s,
                      "batch_wall_s": wall, "batch_kernel_s": outs[0].stats["time_total_s"],
                      "mean_iterations": float(np.mean([o.iterations_run for o in outs])),
                      "max_rel_cost_diff_vs_sequential": float(max(abs(a.final_cost - b.final_cost) / b.final_cost
                                                                   for a, b in zip(outs[:n], seq)))}
-        log(f"windows {prec}: batch {res[prec]['batch_windows_per_s']:.1f}/s, "
+        log(f"windows {prec}: graph {res[prec]['graph_windows_per_s']:.1f}/s "
+            f"(shape {info['threads_per_window']}x{info['pairs_in_flight']}), "
+            f"batch {res[prec]['batch_windows_per_s']:.1f}/s, "
             f"sequential {res[prec]['sequential_gpu_windows_per_s']:.1f}/s")
     if cpu:
         import oracle

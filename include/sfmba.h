@@ -223,6 +223,48 @@ int sfmba_bundle_adjust_batch(const sfmba_input* inputs, size_t count,
                               const sfmba_options* options,
                               sfmba_output* outputs, sfmba_stats* stats);
 
+/* A batch plan: the same batched solve, prepared once and replayed.  No
+ * counterpart in the reference (its callers loop over bundle_adjust,
+ * crates/sfm-reconstruction/src/lib.rs:708-719); for callers that solve the
+ * same batch of windows repeatedly (benchmarks, re-solves after outlier
+ * edits made by rebuilding the plan).
+ *
+ * sfmba_batch_create packs and uploads the problems (the inputs need not
+ * outlive the call), picks the window kernel's launch shape -- threads per
+ * window and Schur point pairs in flight -- and captures one solve in a HIP
+ * graph: restore every window's initial state, the persistent window
+ * kernel, the gather of the final states and their copy to pinned host
+ * memory.  autotune_iterations > 0 runs the autotuner first: every shape is
+ * timed on this batch with the iteration count capped at that value and the
+ * fastest is kept (and remembered for later batches of the same size class
+ * in this process, sfmba_bundle_adjust_batch included); 0 uses the
+ * remembered or default shape.  Problems too large for the window kernel
+ * are solved by the single-problem path on each solve.
+ * sfmba_batch_solve replays the graph from the initial state and writes
+ * outputs/stats exactly as sfmba_bundle_adjust_batch does.  Solves of one
+ * plan must not overlap. */
+typedef struct sfmba_batch sfmba_batch;
+#define SFMBA_MAX_WINDOW_SHAPES 8
+typedef struct sfmba_batch_info {
+  int32_t threads_per_window;  /* the launch shape in use */
+  int32_t pairs_in_flight;
+  int32_t tuned;               /* 1 when the autotuner chose it for this plan */
+  int32_t num_shapes;          /* candidate shapes (below) */
+  int32_t shape_threads[SFMBA_MAX_WINDOW_SHAPES];
+  int32_t shape_pairs[SFMBA_MAX_WINDOW_SHAPES];
+  double shape_seconds[SFMBA_MAX_WINDOW_SHAPES]; /* tuning run time per shape (0: not timed) */
+  double create_seconds;       /* packing, upload, buffers */
+  double capture_seconds;      /* graph capture and instantiation */
+  uint64_t num_windows;        /* solved by the captured window kernel */
+  uint64_t num_large;          /* solved by the single-problem path */
+} sfmba_batch_info;
+int sfmba_batch_create(const sfmba_input* inputs, size_t count, const sfmba_params* params,
+                       const sfmba_options* options, int32_t autotune_iterations,
+                       sfmba_batch** batch);
+int sfmba_batch_solve(sfmba_batch* batch, sfmba_output* outputs, sfmba_stats* stats);
+int sfmba_batch_get_info(const sfmba_batch* batch, sfmba_batch_info* info);
+void sfmba_batch_destroy(sfmba_batch* batch);
+
 /* ------------------------------------------------ caller solve policy
  * The solve part of the reference's caller, from the assembled problem on.
  *

@@ -141,6 +141,17 @@ class WindowSummary(C.Structure):
                 ("final_cost", C.c_double), ("iterations_run", C.c_uint64)]
 
 
+MAX_WINDOW_SHAPES = 8
+
+
+class BatchInfo(C.Structure):
+    _fields_ = [("threads_per_window", C.c_int32), ("pairs_in_flight", C.c_int32), ("tuned", C.c_int32),
+                ("num_shapes", C.c_int32), ("shape_threads", C.c_int32 * MAX_WINDOW_SHAPES),
+                ("shape_pairs", C.c_int32 * MAX_WINDOW_SHAPES), ("shape_seconds", C.c_double * MAX_WINDOW_SHAPES),
+                ("create_seconds", C.c_double), ("capture_seconds", C.c_double), ("num_windows", C.c_uint64),
+                ("num_large", C.c_uint64)]
+
+
 SCOPE_GLOBAL = 0
 SCOPE_LOCAL = 1
 LOCAL_BA_NUM_IMAGES = 6
@@ -154,6 +165,11 @@ SIGNATURES = {
                                          C.POINTER(Output), C.POINTER(Stats)]),
     "sfmba_bundle_adjust_batch": (C.c_int, [C.POINTER(Input), C.c_size_t, C.POINTER(Params),
                                             C.POINTER(Options), C.POINTER(Output), C.POINTER(Stats)]),
+    "sfmba_batch_create": (C.c_int, [C.POINTER(Input), C.c_size_t, C.POINTER(Params), C.POINTER(Options), C.c_int32,
+                                     C.POINTER(C.c_void_p)]),
+    "sfmba_batch_solve": (C.c_int, [C.c_void_p, C.POINTER(Output), C.POINTER(Stats)]),
+    "sfmba_batch_get_info": (C.c_int, [C.c_void_p, C.POINTER(BatchInfo)]),
+    "sfmba_batch_destroy": (None, [C.c_void_p]),
     "sfmba_problem_create": (C.c_int, [C.POINTER(Input), C.c_int32, C.POINTER(C.c_void_p)]),
     "sfmba_problem_solve": (C.c_int, [C.c_void_p, C.POINTER(Params), C.POINTER(Options), C.POINTER(Output),
                                       C.POINTER(Stats)]),

@@ -370,6 +370,69 @@ def bundle_adjust_batch(inputs: Sequence[BaInput], params: BaParams | None = Non
     return res
 
 
+class BatchPlan:
+    """A batch of independent problems prepared once and solved by replaying
+    a captured HIP graph (sfmba_batch_create / sfmba_batch_solve).  No
+    counterpart in the reference, whose callers loop over bundle_adjust
+    (crates/sfm-reconstruction/src/lib.rs:708-719).  autotune_iterations > 0
+    times every window-kernel launch shape on this batch (iterations capped
+    at that value) and keeps the fastest."""
+
+    def __init__(self, inputs: Sequence[BaInput], params: BaParams | None = None, *, precision: str = "fp64",
+                 device: int = 0, autotune_iterations: int = 0):
+        self._lib = N.load_engine()
+        self._h = C.c_void_p()
+        params = params or BaParams()
+        self._cis = [_CInput(i) for i in inputs]
+        arr_in = (N.Input * len(self._cis))(*[ci.c for ci in self._cis])
+        opt = _options(precision, "cholesky", device, False)
+        N.check(self._lib.sfmba_batch_create(arr_in, len(self._cis), C.byref(_c_params(params)), C.byref(opt),
+                                             int(autotune_iterations), C.byref(self._h)))
+
+    def info(self) -> dict:
+        inf = N.BatchInfo()
+        N.check(self._lib.sfmba_batch_get_info(self._h, C.byref(inf)))
+        n = inf.num_shapes
+        return {"threads_per_window": inf.threads_per_window, "pairs_in_flight": inf.pairs_in_flight,
+                "tuned": bool(inf.tuned),
+                "shapes": [{"threads": inf.shape_threads[k], "pairs": inf.shape_pairs[k],
+                            "seconds": inf.shape_seconds[k]} for k in range(n)],
+                "create_seconds": inf.create_seconds, "capture_seconds": inf.capture_seconds,
+                "num_windows": inf.num_windows, "num_large": inf.num_large}
+
+    def solve(self) -> list:
+        if not self._h:
+            raise ValueError("plan is closed")
+        outs = [_Outputs(ci) for ci in self._cis]
+        stats = [_Stats(256) for _ in self._cis]
+        arr_out = (N.Output * len(self._cis))(*[o.c for o in outs])
+        arr_st = (N.Stats * len(self._cis))(*[s.c for s in stats])
+        N.check(self._lib.sfmba_batch_solve(self._h, arr_out, arr_st))
+        res = []
+        for k, o in enumerate(outs):
+            stats[k].c = arr_st[k]
+            res.append(BaOutput(o.poses, o.points, o.cams, arr_out[k].initial_cost, arr_out[k].final_cost,
+                                int(arr_out[k].iterations_run), stats[k].to_dict()))
+        return res
+
+    def close(self) -> None:
+        if self._h:
+            self._lib.sfmba_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
 INTRINSICS = {"fixed": N.INTRINSICS_FIXED, "free": N.INTRINSICS_FREE, "free_guarded": N.INTRINSICS_FREE_GUARDED}
 
 

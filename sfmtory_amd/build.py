@@ -22,7 +22,7 @@ LIB = PKG / "lib"
 OBJ = PKG / "_build"
 
 HIP_SOURCES = ["engine_host.hip", "engine_kernels.hip", "dense_kernels.hip", "intrinsics_kernels.hip",
-               "window_kernels.hip", "pcg_kernels.hip"]
+               "window_kernels.hip", "window_kernels_512.hip", "window_kernels_256.hip", "pcg_kernels.hip"]
 
 # host-only translation units (no device code): g++, linked into libsfmba
 HOST_SOURCES = ["colmap_io.cpp", "scope.cpp"]
@@ -73,7 +73,7 @@ def build_engine() -> Path:
         obj = OBJ / (Path(src).stem + ".o")
         objs.append(obj)
         jobs.append(["g++", *HOST_FLAGS, "-c", str(CSRC / src), "-o", str(obj)])
-    with cf.ThreadPoolExecutor(max_workers=min(4, len(jobs))) as ex:
+    with cf.ThreadPoolExecutor(max_workers=max(1, min(8, os.cpu_count() or 1, len(jobs)))) as ex:
         list(ex.map(_run, jobs))
     out = LIB / "libsfmba.so"
     tmp = LIB / "libsfmba.so.tmp"

@@ -7,9 +7,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <exception>
+#include <map>
 #include <mutex>
 #include <thread>
 #include <cmath>
@@ -889,17 +891,39 @@ struct Planner {
   }
 };
 
+// A batch packed and resident on the device: the windows the kernel
+// solves (win) and those too large for it (big, solved by the
+// single-problem path).
 template <typename T>
-static void batch_solve(const sfmba_input* inputs, size_t count, const sfmba_params* prm,
-                        const sfmba_options& opt, sfmba_output* outs, sfmba_stats* stats) {
-  using clock = std::chrono::steady_clock;
-  using V4 = typename Vec4<T>::type;
-  const auto t0 = clock::now();
-  for (size_t w = 0; w < count; ++w) validate(&inputs[w]);
-  StreamGuard sg(opt.device);
+struct WindowBatch {
+  std::vector<Pack> packs;
+  std::vector<std::vector<sfmba_camera>> cams_in;
+  std::vector<size_t> win, big;
+  uint32_t max_dim = 1;
+  std::vector<WinPlan> plan;
+  std::vector<size_t> state_dbl;
+  size_t up_struct = 0, state_end = 0;
+  DevArena arena;
+  uint8_t* base = nullptr;
+  View<T>* d_views = nullptr;
+  WindowResult* d_res = nullptr;
+  double* d_oc = nullptr;
+  uint64_t max_it = 1;
+  WindowParams wp{};
+  std::vector<uint8_t> host;  // structure + initial state, as uploaded
+  uint64_t total_obs = 0;
+};
 
+template <typename T>
+static void batch_prepare(WindowBatch<T>& B, const sfmba_input* inputs, size_t count, const sfmba_params* prm,
+                          hipStream_t s) {
+  using V4 = typename Vec4<T>::type;
+  for (size_t w = 0; w < count; ++w) validate(&inputs[w]);
+  B.packs.resize(count);
+  B.cams_in.resize(count);
+  for (size_t w = 0; w < count; ++w)
+    B.cams_in[w].assign(inputs[w].cameras, inputs[w].cameras + inputs[w].num_cameras);
   // Host packing on worker threads.
-  std::vector<Pack> packs(count);
   {
     std::atomic<size_t> next{0};
     std::exception_ptr err;
@@ -910,7 +934,7 @@ static void batch_solve(const sfmba_input* inputs, size_t count, const sfmba_par
       th.emplace_back([&] {
         for (size_t w; (w = next++) < count;) {
           try {
-            build_pack(&inputs[w], packs[w], true, 0);
+            build_pack(&inputs[w], B.packs[w], true, 0);
           } catch (...) {
             std::lock_guard<std::mutex> g(err_mu);
             if (!err) err = std::current_exception();
@@ -920,26 +944,24 @@ static void batch_solve(const sfmba_input* inputs, size_t count, const sfmba_par
     for (auto& t : th) t.join();
     if (err) std::rethrow_exception(err);
   }
-  std::vector<size_t> win;   // solved by the window kernel
-  std::vector<size_t> big;   // too large for it
-  uint32_t max_dim = 1;
   for (size_t w = 0; w < count; ++w) {
-    if (packs[w].D > WINDOW_MAX_DIM) {
-      big.push_back(w);
+    if (B.packs[w].D > WINDOW_MAX_DIM) {
+      B.big.push_back(w);
       continue;
     }
-    win.push_back(w);
-    for (uint32_t g : packs[w].grp_dim) max_dim = std::max(max_dim, g);
+    B.win.push_back(w);
+    B.total_obs += B.packs[w].n_obs;
+    for (uint32_t g : B.packs[w].grp_dim) B.max_dim = std::max(B.max_dim, g);
   }
 
   // One allocation: uploaded structure and initial state first, then the
   // device-only work arrays.
-  const size_t nw = win.size();
-  std::vector<WinPlan> plan(nw);
+  const size_t nw = B.win.size();
+  B.plan.resize(nw);
   Planner up;
   for (size_t k = 0; k < nw; ++k) {
-    const Pack& P = packs[win[k]];
-    WinPlan& q = plan[k];
+    const Pack& P = B.packs[B.win[k]];
+    WinPlan& q = B.plan[k];
     q.obs_img = up.take(4 * (size_t)P.n_obs);
     q.obs_pt = up.take(4 * (size_t)P.n_obs);
     q.obs_xy = up.take(16 * (size_t)P.n_obs);
@@ -961,18 +983,18 @@ static void batch_solve(const sfmba_input* inputs, size_t count, const sfmba_par
     q.pairs = up.take(4 * P.pairs.size());
     q.chunks = up.take(16 * P.chunks.size());
   }
-  const size_t up_struct = up.n;
-  std::vector<size_t> state_dbl(nw);
+  B.up_struct = up.n;
+  B.state_dbl.resize(nw);
   for (size_t k = 0; k < nw; ++k) {
-    const Pack& P = packs[win[k]];
-    state_dbl[k] = 7 * (size_t)P.n_img + 8 * (size_t)P.n_cam + 3 * (size_t)P.n_pt;
-    plan[k].state = up.take(2 * 8 * state_dbl[k]);
+    const Pack& P = B.packs[B.win[k]];
+    B.state_dbl[k] = 7 * (size_t)P.n_img + 8 * (size_t)P.n_cam + 3 * (size_t)P.n_pt;
+    B.plan[k].state = up.take(2 * 8 * B.state_dbl[k]);
   }
-  const size_t state_end = up.n;
+  B.state_end = up.n;
   Planner& wk = up;  // work arrays follow in the same allocation
   for (size_t k = 0; k < nw; ++k) {
-    const Pack& P = packs[win[k]];
-    WinPlan& q = plan[k];
+    const Pack& P = B.packs[B.win[k]];
+    WinPlan& q = B.plan[k];
     q.J4 = wk.take(sizeof(V4) * (size_t)P.n_rows);
     q.E4 = wk.take(sizeof(V4) * (size_t)P.n_rows);
     q.F4 = wk.take(sizeof(V4) * (size_t)P.n_rows);
@@ -986,16 +1008,15 @@ static void batch_solve(const sfmba_input* inputs, size_t count, const sfmba_par
     q.S = wk.take(8 * ((size_t)P.D + 1) * P.D);
     q.delta = wk.take(8 * (size_t)P.D);
   }
-  DevArena arena;
-  uint8_t* base = arena.alloc<uint8_t>(std::max<size_t>(up.n, 1));
-  std::vector<uint8_t> host(state_end);
+  B.base = B.arena.template alloc<uint8_t>(std::max<size_t>(up.n, 1));
+  B.host.assign(B.state_end, 0);
   auto put = [&](size_t off, const void* src, size_t bytes) {
-    if (bytes) std::memcpy(host.data() + off, src, bytes);
+    if (bytes) std::memcpy(B.host.data() + off, src, bytes);
   };
   std::vector<View<T>> views(nw);
   for (size_t k = 0; k < nw; ++k) {
-    const Pack& P = packs[win[k]];
-    const WinPlan& q = plan[k];
+    const Pack& P = B.packs[B.win[k]];
+    const WinPlan& q = B.plan[k];
     put(q.obs_img, P.obs_img.data(), 4 * (size_t)P.n_obs);
     put(q.obs_pt, P.obs_pt.data(), 4 * (size_t)P.n_obs);
     put(q.obs_xy, P.obs_xy.data(), 16 * (size_t)P.n_obs);
@@ -1031,7 +1052,7 @@ static void batch_solve(const sfmba_input* inputs, size_t count, const sfmba_par
     v.D = P.D;
     v.Dp = P.D;  // rhs row right below the system
     v.ld = P.D + 1;
-    auto at = [&](size_t off) { return base + off; };
+    auto at = [&](size_t off) { return B.base + off; };
     v.obs_img = reinterpret_cast<const uint32_t*>(at(q.obs_img));
     v.obs_pt = reinterpret_cast<const uint32_t*>(at(q.obs_pt));
     v.obs_xy = reinterpret_cast<const double2*>(at(q.obs_xy));
@@ -1066,81 +1087,293 @@ static void batch_solve(const sfmba_input* inputs, size_t count, const sfmba_par
     v.delta = reinterpret_cast<double*>(at(q.delta));
     double* st = reinterpret_cast<double*>(at(q.state));
     for (int c = 0; c < 2; ++c) {
-      double* sb = st + (size_t)c * state_dbl[k];
+      double* sb = st + (size_t)c * B.state_dbl[k];
       v.poses[c] = sb;
       v.cams[c] = sb + 7 * (size_t)P.n_img;
       v.pts[c] = sb + 7 * (size_t)P.n_img + 8 * (size_t)P.n_cam;
     }
   }
-  (void)up_struct;
-  View<T>* d_views = arena.alloc<View<T>>(std::max<size_t>(nw, 1));
-  WindowResult* d_res = arena.alloc<WindowResult>(std::max<size_t>(nw, 1));
-  const uint64_t max_it = std::max<uint64_t>(prm->max_iterations, 1);
-  double* d_oc = arena.alloc<double>(std::max<size_t>(nw, 1) * max_it);
+  B.d_views = B.arena.template alloc<View<T>>(std::max<size_t>(nw, 1));
+  B.d_res = B.arena.template alloc<WindowResult>(std::max<size_t>(nw, 1));
+  B.max_it = std::max<uint64_t>(prm->max_iterations, 1);
+  B.d_oc = B.arena.template alloc<double>(std::max<size_t>(nw, 1) * B.max_it);
   if (nw) {
-    HIP_CHECK(hipMemcpyAsync(base, host.data(), state_end, hipMemcpyHostToDevice, sg.s));
-    HIP_CHECK(hipMemcpyAsync(d_views, views.data(), nw * sizeof(View<T>), hipMemcpyHostToDevice, sg.s));
+    HIP_CHECK(hipMemcpyAsync(B.base, B.host.data(), B.state_end, hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(B.d_views, views.data(), nw * sizeof(View<T>), hipMemcpyHostToDevice, s));
   }
-  HIP_CHECK(hipStreamSynchronize(sg.s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  B.wp = WindowParams{};
+  B.wp.max_iterations = prm->max_iterations;
+  B.wp.initial_lambda = prm->initial_lambda;
+  B.wp.loss_scale_px = prm->loss_scale_px;
+  B.wp.robust_loss = prm->robust_loss;
+}
+
+// Writes window k's solution (final state sb: poses, camera parameters,
+// points) and result into the caller's output and statistics.
+template <typename T>
+static void batch_unpack(const WindowBatch<T>& B, size_t k, const double* sb, const WindowResult& r,
+                         const double* oc, double kernel_s, double upload_s, sfmba_output* outs,
+                         sfmba_stats* stats) {
+  const size_t w = B.win[k];
+  const Pack& P = B.packs[w];
+  sfmba_output* out = &outs[w];
+  for (uint32_t i = 0; i < P.n_img; ++i) {
+    const double* q = sb + 7 * (size_t)i;
+    out->poses[i] = sfmba_pose{q[0], q[1], q[2], q[3], q[4], q[5], q[6]};
+  }
+  const double* cs = sb + 7 * (size_t)P.n_img;
+  for (uint32_t c = 0; c < P.n_cam; ++c) {
+    out->cameras[c] = B.cams_in[w][c];
+    const int kp = model_param_count_host(B.cams_in[w][c].model_id);
+    for (int j = 0; j < kp; ++j) out->cameras[c].params[j] = cs[8 * (size_t)c + j];
+  }
+  if (P.n_pt) std::memcpy(out->points, cs + 8 * (size_t)P.n_cam, 24 * (size_t)P.n_pt);
+  out->initial_cost = r.initial_cost;
+  out->final_cost = r.final_cost;
+  out->iterations_run = r.iterations_run;
+  if (stats) {
+    sfmba_stats& st = stats[w];
+    st.dim = P.D;
+    st.num_linearizations = r.num_linearizations;
+    st.num_factorizations = r.num_factorizations;
+    st.num_cholesky_failures = r.num_cholesky_failures;
+    st.num_lu_solves = r.num_lu_solves;
+    st.num_rejected_steps = r.num_rejected_steps;
+    st.num_schur_pairs = P.pairs.size() / 2;
+    for (uint64_t i = 0; i < r.num_outer; ++i) push_stat(st.outer_costs, st.outer_costs_capacity, &st.outer_costs_len, oc[i]);
+    st.time_total_s = kernel_s;
+    st.time_upload_s = upload_s;
+  }
+}
+
+// ---- launch-shape autotuner
+// Tuned shapes, per precision, destination block class and batch size
+// class (log2 of the window count and of the mean observations per
+// window), for the life of the process.
+static std::mutex g_tune_mu;
+static std::map<std::array<uint32_t, 4>, int> g_tuned_shape;
+
+template <typename T>
+static std::array<uint32_t, 4> tune_key(const WindowBatch<T>& B) {
+  auto lg = [](uint64_t x) {
+    uint32_t l = 0;
+    while ((1ull << l) < x) ++l;
+    return l;
+  };
+  const uint32_t ne = B.max_dim * B.max_dim;
+  const uint32_t nq = ne <= 64 ? 1 : ne <= 128 ? 2 : ne <= 192 ? 3 : 4;
+  const uint64_t nw = std::max<size_t>(B.win.size(), 1);
+  return {(uint32_t)sizeof(T), nq, lg(nw), lg(std::max<uint64_t>(B.total_obs / nw, 1))};
+}
+
+template <typename T>
+static int lookup_shape(const WindowBatch<T>& B) {
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  auto it = g_tuned_shape.find(tune_key(B));
+  return it == g_tuned_shape.end() ? window_default_shape<T>() : it->second;
+}
+
+}  // namespace sfmba
+
+// The batch plan of the C ABI: a batch resident on the device, its tuned
+// launch shape and one solve captured in a HIP graph.
+struct sfmba_batch {
+  int device = 0;
+  int precision = 0;
+  hipStream_t s = nullptr;
+  sfmba::WindowBatch<float> b32;
+  sfmba::WindowBatch<double> b64;
+  int shape = 0;
+  sfmba_batch_info info{};
+  sfmba::DevArena arena;
+  double* d_pristine = nullptr;  // copy of the state region at creation
+  double* d_out = nullptr;       // final states, compact
+  std::vector<uint64_t> out_off;
+  double* h_out = nullptr;       // pinned mirrors
+  sfmba::WindowResult* h_res = nullptr;
+  double* h_oc = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<std::unique_ptr<sfmba_problem>> big;  // windows above the kernel's limit
+  sfmba_params prm{};
+  sfmba_options opt{};
+  ~sfmba_batch();
+};
+
+namespace sfmba {
+
+// Restores the initial state of every window (both state buffers) from the
+// pristine copy: a copy kernel, so that graph captures hold no memset or
+// device-to-device copy node.
+template <typename T>
+static void batch_restore(const WindowBatch<T>& B, const double* pristine, hipStream_t s) {
+  if (B.win.empty()) return;
+  dense_copy(pristine, reinterpret_cast<double*>(B.base + B.up_struct), (B.state_end - B.up_struct) / 8, s);
+}
+
+// Times every launch shape on the batch with the iteration count capped at
+// tune_iterations (the initial state restored before each run, one
+// one-iteration warm-up launch per shape first) and records the fastest.
+template <typename T>
+static int tune_batch(WindowBatch<T>& B, const double* pristine, hipStream_t s, int tune_iterations,
+                      double* seconds) {
+  const uint32_t nw = (uint32_t)B.win.size();
+  WindowParams wp = B.wp;
+  wp.max_iterations = std::min<uint64_t>(B.wp.max_iterations, (uint64_t)std::max(tune_iterations, 1));
+  WindowParams w1 = B.wp;
+  w1.max_iterations = 1;
+  hipEvent_t e0, e1;
+  HIP_CHECK(hipEventCreate(&e0));
+  HIP_CHECK(hipEventCreate(&e1));
+  int best = window_default_shape<T>();
+  double best_t = 1e300;
+  for (int sh = 0; sh < N_WINDOW_SHAPES; ++sh) {
+    batch_restore(B, pristine, s);
+    launch_window_lm<T>(B.d_views, std::min<uint32_t>(nw, 1), B.max_dim, w1, B.d_res, B.d_oc, sh, s);
+    batch_restore(B, pristine, s);
+    HIP_CHECK(hipEventRecord(e0, s));
+    launch_window_lm<T>(B.d_views, nw, B.max_dim, wp, B.d_res, B.d_oc, sh, s);
+    HIP_CHECK(hipEventRecord(e1, s));
+    HIP_CHECK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    seconds[sh] = ms * 1e-3;
+    if (seconds[sh] < best_t) {
+      best_t = seconds[sh];
+      best = sh;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  batch_restore(B, pristine, s);
+  HIP_CHECK(hipStreamSynchronize(s));
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  g_tuned_shape[tune_key(B)] = best;
+  return best;
+}
+
+template <typename T>
+static void batch_solve(const sfmba_input* inputs, size_t count, const sfmba_params* prm,
+                        const sfmba_options& opt, sfmba_output* outs, sfmba_stats* stats) {
+  using clock = std::chrono::steady_clock;
+  const auto t0 = clock::now();
+  StreamGuard sg(opt.device);
+  WindowBatch<T> B;
+  batch_prepare(B, inputs, count, prm, sg.s);
+  const size_t nw = B.win.size();
   const auto t2 = clock::now();
-  WindowParams wp{};
-  wp.max_iterations = prm->max_iterations;
-  wp.initial_lambda = prm->initial_lambda;
-  wp.loss_scale_px = prm->loss_scale_px;
-  wp.robust_loss = prm->robust_loss;
-  launch_window_lm<T>(d_views, (uint32_t)nw, max_dim, wp, d_res, d_oc, sg.s);
+  launch_window_lm<T>(B.d_views, (uint32_t)nw, B.max_dim, B.wp, B.d_res, B.d_oc, lookup_shape(B), sg.s);
   HIP_CHECK(hipStreamSynchronize(sg.s));
   const auto t3 = clock::now();
   std::vector<WindowResult> res(nw);
-  std::vector<double> oc(nw * max_it);
+  std::vector<double> oc(nw * B.max_it);
   if (nw) {
-    HIP_CHECK(hipMemcpyAsync(res.data(), d_res, nw * sizeof(WindowResult), hipMemcpyDeviceToHost, sg.s));
-    HIP_CHECK(hipMemcpyAsync(oc.data(), d_oc, oc.size() * sizeof(double), hipMemcpyDeviceToHost, sg.s));
-    HIP_CHECK(hipMemcpyAsync(host.data() + up_struct, base + up_struct, state_end - up_struct,
+    HIP_CHECK(hipMemcpyAsync(res.data(), B.d_res, nw * sizeof(WindowResult), hipMemcpyDeviceToHost, sg.s));
+    HIP_CHECK(hipMemcpyAsync(oc.data(), B.d_oc, oc.size() * sizeof(double), hipMemcpyDeviceToHost, sg.s));
+    HIP_CHECK(hipMemcpyAsync(B.host.data() + B.up_struct, B.base + B.up_struct, B.state_end - B.up_struct,
                              hipMemcpyDeviceToHost, sg.s));
   }
   HIP_CHECK(hipStreamSynchronize(sg.s));
   const double kernel_s = std::chrono::duration<double>(t3 - t2).count();
   const double upload_s = std::chrono::duration<double>(t2 - t0).count();
   for (size_t k = 0; k < nw; ++k) {
-    const size_t w = win[k];
-    const Pack& P = packs[w];
-    const WindowResult& r = res[k];
-    const double* sb = reinterpret_cast<const double*>(host.data() + plan[k].state) + (size_t)r.cur * state_dbl[k];
-    sfmba_output* out = &outs[w];
-    for (uint32_t i = 0; i < P.n_img; ++i) {
-      const double* q = sb + 7 * (size_t)i;
-      out->poses[i] = sfmba_pose{q[0], q[1], q[2], q[3], q[4], q[5], q[6]};
-    }
-    const double* cs = sb + 7 * (size_t)P.n_img;
-    for (uint32_t c = 0; c < P.n_cam; ++c) {
-      out->cameras[c] = inputs[w].cameras[c];
-      const int kp = model_param_count_host(inputs[w].cameras[c].model_id);
-      for (int j = 0; j < kp; ++j) out->cameras[c].params[j] = cs[8 * (size_t)c + j];
-    }
-    if (P.n_pt) std::memcpy(out->points, cs + 8 * (size_t)P.n_cam, 24 * (size_t)P.n_pt);
-    out->initial_cost = r.initial_cost;
-    out->final_cost = r.final_cost;
-    out->iterations_run = r.iterations_run;
-    if (stats) {
-      sfmba_stats& st = stats[w];
-      st.dim = P.D;
-      st.num_linearizations = r.num_linearizations;
-      st.num_factorizations = r.num_factorizations;
-      st.num_cholesky_failures = r.num_cholesky_failures;
-      st.num_lu_solves = r.num_lu_solves;
-      st.num_rejected_steps = r.num_rejected_steps;
-      st.num_schur_pairs = P.pairs.size() / 2;
-      for (uint64_t i = 0; i < r.num_outer; ++i)
-        push_stat(st.outer_costs, st.outer_costs_capacity, &st.outer_costs_len, oc[k * max_it + i]);
-      st.time_total_s = kernel_s;
-      st.time_upload_s = upload_s;
-    }
+    const double* sb =
+        reinterpret_cast<const double*>(B.host.data() + B.plan[k].state) + (size_t)res[k].cur * B.state_dbl[k];
+    batch_unpack(B, k, sb, res[k], oc.data() + k * B.max_it, kernel_s, upload_s, outs, stats);
   }
   // Windows too large for one workgroup: the single-problem path.
-  for (size_t w : big) {
+  for (size_t w : B.big) {
     std::unique_ptr<sfmba_problem> H(create_problem(&inputs[w], opt.device));
     solve_problem(H.get(), prm, &opt, &outs[w], stats ? &stats[w] : nullptr);
+  }
+}
+
+// Plan creation: packing and upload, the pristine state copy, the
+// autotuner (when asked), then one solve captured as a graph: restore the
+// initial state, the window kernel, the gather of the final states, and
+// the copies of results, outer costs and states into pinned host memory.
+template <typename T>
+static void batch_plan_build(sfmba_batch& Q, WindowBatch<T>& B, const sfmba_input* inputs, size_t count,
+                             int autotune) {
+  using clock = std::chrono::steady_clock;
+  const auto t0 = clock::now();
+  hipStream_t s = Q.s;
+  batch_prepare(B, inputs, count, &Q.prm, s);
+  const size_t nw = B.win.size();
+  const size_t n_state = (B.state_end - B.up_struct) / 8;
+  Q.d_pristine = Q.arena.alloc<double>(std::max<size_t>(n_state, 1));
+  if (nw) dense_copy(reinterpret_cast<const double*>(B.base + B.up_struct), Q.d_pristine, n_state, s);
+  std::vector<uint64_t> st_off(nw), nd(nw);
+  Q.out_off.resize(nw);
+  uint64_t n_out = 0;
+  for (size_t k = 0; k < nw; ++k) {
+    st_off[k] = B.plan[k].state / 8;
+    nd[k] = B.state_dbl[k];
+    Q.out_off[k] = n_out;
+    n_out += nd[k];
+  }
+  uint64_t* d_meta = Q.arena.alloc<uint64_t>(std::max<size_t>(3 * nw, 1));
+  Q.d_out = Q.arena.alloc<double>(std::max<uint64_t>(n_out, 1));
+  std::vector<uint64_t> meta;
+  meta.insert(meta.end(), st_off.begin(), st_off.end());
+  meta.insert(meta.end(), nd.begin(), nd.end());
+  meta.insert(meta.end(), Q.out_off.begin(), Q.out_off.end());
+  if (nw) HIP_CHECK(hipMemcpyAsync(d_meta, meta.data(), meta.size() * 8, hipMemcpyHostToDevice, s));
+  HIP_CHECK(hipHostMalloc((void**)&Q.h_out, std::max<uint64_t>(n_out, 1) * 8, hipHostMallocDefault));
+  HIP_CHECK(hipHostMalloc((void**)&Q.h_res, std::max<size_t>(nw, 1) * sizeof(WindowResult), hipHostMallocDefault));
+  HIP_CHECK(hipHostMalloc((void**)&Q.h_oc, std::max<size_t>(nw, 1) * B.max_it * 8, hipHostMallocDefault));
+  HIP_CHECK(hipStreamSynchronize(s));
+  Q.info.create_seconds = std::chrono::duration<double>(clock::now() - t0).count();
+  Q.info.num_windows = nw;
+  Q.info.num_large = B.big.size();
+  Q.info.num_shapes = N_WINDOW_SHAPES;
+  for (int k = 0; k < N_WINDOW_SHAPES; ++k) {
+    Q.info.shape_threads[k] = WINDOW_SHAPES[k].threads;
+    Q.info.shape_pairs[k] = WINDOW_SHAPES[k].pairs;
+  }
+  if (autotune && nw) {
+    Q.shape = tune_batch(B, Q.d_pristine, s, autotune, Q.info.shape_seconds);
+    Q.info.tuned = 1;
+  } else {
+    Q.shape = lookup_shape(B);
+  }
+  Q.info.threads_per_window = WINDOW_SHAPES[Q.shape].threads;
+  Q.info.pairs_in_flight = WINDOW_SHAPES[Q.shape].pairs;
+  // capture (a batch without windows for the kernel has no graph)
+  const auto t1 = clock::now();
+  if (nw) {
+    HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    batch_restore(B, Q.d_pristine, s);
+    launch_window_lm<T>(B.d_views, (uint32_t)nw, B.max_dim, B.wp, B.d_res, B.d_oc, Q.shape, s);
+    launch_window_gather((uint32_t)nw, B.d_res, d_meta, d_meta + nw, d_meta + 2 * nw,
+                         reinterpret_cast<const double*>(B.base), Q.d_out, s);
+    HIP_CHECK(hipMemcpyAsync(Q.h_res, B.d_res, nw * sizeof(WindowResult), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(Q.h_oc, B.d_oc, nw * B.max_it * 8, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(Q.h_out, Q.d_out, n_out * 8, hipMemcpyDeviceToHost, s));
+    const hipError_t ce = hipStreamEndCapture(s, &Q.graph);
+    if (ce != hipSuccess) throw Error(SFMBA_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(ce));
+    HIP_CHECK(hipGraphInstantiate(&Q.exec, Q.graph, nullptr, nullptr, 0));
+  }
+  Q.info.capture_seconds = std::chrono::duration<double>(clock::now() - t1).count();
+  for (size_t w : B.big) Q.big.emplace_back(create_problem(&inputs[w], Q.device));
+}
+
+template <typename T>
+static void batch_plan_solve(sfmba_batch& Q, WindowBatch<T>& B, sfmba_output* outs, sfmba_stats* stats) {
+  using clock = std::chrono::steady_clock;
+  HIP_CHECK(hipSetDevice(Q.device));
+  const auto t0 = clock::now();
+  if (Q.exec) {
+    HIP_CHECK(hipGraphLaunch(Q.exec, Q.s));
+    HIP_CHECK(hipStreamSynchronize(Q.s));
+  }
+  const double graph_s = std::chrono::duration<double>(clock::now() - t0).count();
+  for (size_t k = 0; k < B.win.size(); ++k)
+    batch_unpack(B, k, Q.h_out + Q.out_off[k], Q.h_res[k], Q.h_oc + k * B.max_it, graph_s, 0.0, outs, stats);
+  for (size_t j = 0; j < B.big.size(); ++j) {
+    const size_t w = B.big[j];
+    solve_problem(Q.big[j].get(), &Q.prm, &Q.opt, &outs[w], stats ? &stats[w] : nullptr);
   }
 }
 
@@ -1814,4 +2047,84 @@ int sfmba_bundle_adjust_batch(const sfmba_input* inputs, size_t count, const sfm
   });
 }
 
+int sfmba_batch_create(const sfmba_input* inputs, size_t count, const sfmba_params* params,
+                       const sfmba_options* options, int32_t autotune_iterations, sfmba_batch** batch) {
+  return guarded([&] {
+    if (!batch) throw Error(SFMBA_ERR_INVALID_ARGUMENT, "NULL argument");
+    *batch = nullptr;
+    if ((count && !inputs) || !params) throw Error(SFMBA_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (autotune_iterations < 0) throw Error(SFMBA_ERR_INVALID_ARGUMENT, "autotune_iterations is negative");
+    sfmba_options o;
+    sfmba_options_default(&o);
+    if (options) o = *options;
+    if (o.precision != SFMBA_PRECISION_FP64 && o.precision != SFMBA_PRECISION_MIXED)
+      throw Error(SFMBA_ERR_INVALID_ARGUMENT, "unknown precision");
+    if (o.solver == SFMBA_SOLVER_PCG)
+      throw Error(SFMBA_ERR_UNSUPPORTED, "batched windows are solved by Cholesky only");
+    std::unique_ptr<sfmba_batch> Q(new sfmba_batch());
+    {
+      StreamGuard probe(o.device);  // device checks
+    }
+    Q->device = o.device;
+    Q->precision = o.precision;
+    Q->prm = *params;
+    Q->opt = o;
+    HIP_CHECK(hipSetDevice(o.device));
+    HIP_CHECK(hipStreamCreateWithFlags(&Q->s, hipStreamNonBlocking));
+    if (o.precision == SFMBA_PRECISION_MIXED)
+      batch_plan_build(*Q, Q->b32, inputs, count, autotune_iterations);
+    else
+      batch_plan_build(*Q, Q->b64, inputs, count, autotune_iterations);
+    *batch = Q.release();
+  });
+}
+
+int sfmba_batch_solve(sfmba_batch* batch, sfmba_output* outputs, sfmba_stats* stats) {
+  return guarded([&] {
+    if (!batch) throw Error(SFMBA_ERR_INVALID_ARGUMENT, "NULL argument");
+    const size_t count = batch->precision == SFMBA_PRECISION_MIXED ? batch->b32.packs.size() : batch->b64.packs.size();
+    if (count && !outputs) throw Error(SFMBA_ERR_INVALID_ARGUMENT, "NULL argument");
+    for (size_t i = 0; i < count; ++i) {
+      const sfmba::Pack& P = batch->precision == SFMBA_PRECISION_MIXED ? batch->b32.packs[i] : batch->b64.packs[i];
+      const sfmba_output& out = outputs[i];
+      if ((P.n_img && !out.poses) || (P.n_pt && !out.points) || (P.n_cam && !out.cameras))
+        throw Error(SFMBA_ERR_INVALID_ARGUMENT, "problem " + std::to_string(i) + ": output arrays are NULL");
+      if (stats) {
+        sfmba_stats z{};
+        z.outer_costs = stats[i].outer_costs;
+        z.outer_costs_capacity = stats[i].outer_costs_capacity;
+        z.trial_costs = stats[i].trial_costs;
+        z.trial_costs_capacity = stats[i].trial_costs_capacity;
+        stats[i] = z;
+      }
+    }
+    if (batch->precision == SFMBA_PRECISION_MIXED)
+      batch_plan_solve(*batch, batch->b32, outputs, stats);
+    else
+      batch_plan_solve(*batch, batch->b64, outputs, stats);
+  });
+}
+
+int sfmba_batch_get_info(const sfmba_batch* batch, sfmba_batch_info* info) {
+  return guarded([&] {
+    if (!batch || !info) throw Error(SFMBA_ERR_INVALID_ARGUMENT, "NULL argument");
+    *info = batch->info;
+  });
+}
+
+void sfmba_batch_destroy(sfmba_batch* batch) { delete batch; }
+
 }  // extern "C"
+
+sfmba_batch::~sfmba_batch() {
+  if (device >= 0) (void)hipSetDevice(device);
+  if (s) (void)hipStreamSynchronize(s);
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  if (h_out) (void)hipHostFree(h_out);
+  if (h_res) (void)hipHostFree(h_res);
+  if (h_oc) (void)hipHostFree(h_oc);
+  big.clear();
+  if (s) (void)hipStreamDestroy(s);
+}
+

@@ -31,11 +31,37 @@ struct WindowResult {
 // keeps the solution in 16 registers per lane).
 constexpr uint32_t WINDOW_MAX_DIM = 1024;
 
+// Launch shapes of the window kernel: threads per window (one workgroup)
+// and Schur point pairs in flight per wavefront.  The autotuner
+// (engine_host.hip) times them on a batch; without a tuned choice the
+// default per precision is used (the best of a measured sweep over 256
+// windows of 10-40 cameras on gfx950: FP32 1024/8, FP64 512/4).
+struct WindowShape {
+  int threads, pairs;
+};
+constexpr int N_WINDOW_SHAPES = 6;
+constexpr WindowShape WINDOW_SHAPES[N_WINDOW_SHAPES] = {{1024, 8}, {1024, 4}, {512, 8},
+                                                        {512, 4},  {256, 8},  {256, 4}};
+template <typename T>
+constexpr int window_default_shape() {
+  return sizeof(T) == 4 ? 0 : 3;
+}
+
 // One workgroup per problem; outer_costs holds max_iterations doubles per
-// problem.
+// problem; shape indexes WINDOW_SHAPES.
 template <typename T>
 void launch_window_lm(const View<T>* views, uint32_t n_win, uint32_t max_grp_dim,
-                      const WindowParams& prm, WindowResult* res, double* outer_costs,
+                      const WindowParams& prm, WindowResult* res, double* outer_costs, int shape,
                       hipStream_t s);
+
+template <typename T, int WGS, int NP>
+void launch_window_shape(const View<T>* views, uint32_t n_win, uint32_t max_grp_dim, const WindowParams& prm,
+                         WindowResult* res, double* outer_costs, hipStream_t s);
+
+// Copies each window's final state (buffer res[w].cur of its two state
+// buffers, nd[w] doubles each, starting at state[w]) to out + out_off[w].
+void launch_window_gather(uint32_t n_win, const WindowResult* res, const uint64_t* state,
+                          const uint64_t* nd, const uint64_t* out_off, const double* base, double* out,
+                          hipStream_t s);
 
 }  // namespace sfmba
