@@ -44,17 +44,18 @@ def log(*a):
 
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_syrk_update_hbm_traffic.txt")
+ASSEMBLY_TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "assembly_hbm_traffic.txt")
 
 
-def measured_traffic():
-    """HBM bytes per k_syrk_update launch of this workload, from the counter
-    runs summarised in profiles/k_syrk_update_hbm_traffic.txt (FETCH_SIZE +
-    WRITE_SIZE per dispatch, each scaled by the same counter's reading for a
-    1 GiB copy); None when that summary is absent."""
+def measured_traffic(path=TRAFFIC_PROFILE, key="traffic_bytes_per_launch"):
+    """HBM bytes per launch of this workload from a committed counter-run
+    summary (FETCH_SIZE + WRITE_SIZE per dispatch, each scaled by the same
+    counter's reading for a 1 GiB copy): k_syrk_update by default, the
+    assembly kernels by key; None when the summary is absent."""
     try:
-        with open(TRAFFIC_PROFILE) as f:
+        with open(path) as f:
             for line in f:
-                if line.startswith("traffic_bytes_per_launch:"):
+                if line.startswith(key + ":"):
                     return float(line.split(":", 1)[1])
     except OSError:
         pass
@@ -328,15 +329,17 @@ def main() -> int:
     mixed_costs = prof_mixed["outer_costs"]
     reach = next((i + 1 for i, c in enumerate(mixed_costs) if c <= fp64_final), None)
     reach_t = None
+    q_reach = None
     if reach is not None:
         p_reach = ba.BaParams(max_iterations=reach)
         prob.solve(p_reach, precision="mixed")
         tr = []
         for _ in range(max(1, args.steps)):
             t = time.perf_counter()
-            prob.solve(p_reach, precision="mixed")
+            out_reach = prob.solve(p_reach, precision="mixed")
             tr.append(time.perf_counter() - t)
         reach_t = float(np.mean(tr))
+        q_reach = residual_stats(ba, inp, out_reach)
         log(f"mixed reaches the FP64 final cost at iteration {reach}: {reach_t:.3f}s")
     D = prof_fp64["dim"]
     q_mixed = residual_stats(ba, inp, mixed_outs[-1])
@@ -380,7 +383,20 @@ def main() -> int:
     t_as = r_mixed["schur_assembly_ms"] * 1e-3
     roof_schur = {"bound": "hbm", "kernel": "Schur assembly (mixed)", "achieved": bytes_schur / t_as / 1e9,
                   "peak": PEAK_HBM / 1e9, "unit": "GB/s", "frac": bytes_schur / t_as / PEAK_HBM,
-                  "traffic": None, "algorithmic_bytes": bytes_schur}
+                  "traffic": measured_traffic(ASSEMBLY_TRAFFIC_PROFILE, "schur_traffic_bytes"),
+                  "algorithmic_bytes": bytes_schur}
+    if roof_schur["traffic"]:
+        roof_schur["measured_GB_per_s"] = roof_schur["traffic"] / t_as / 1e9
+    # Jacobian stage, mixed: 112 + 8 k_f algorithmic bytes per observation
+    # (k_f = 2 free intrinsics per PINHOLE camera).
+    bytes_lin = (112 + 8 * 2) * n_obs
+    t_lin = r_mixed["linearize_ms"] * 1e-3
+    roof_lin = {"bound": "hbm", "kernel": "Jacobian stage: k_linearize + k_point_accumulate (mixed)", "achieved": bytes_lin / t_lin / 1e9,
+                "peak": PEAK_HBM / 1e9, "unit": "GB/s", "frac": bytes_lin / t_lin / PEAK_HBM,
+                "traffic": measured_traffic(ASSEMBLY_TRAFFIC_PROFILE, "linearize_traffic_bytes"),
+                "algorithmic_bytes": bytes_lin}
+    if roof_lin["traffic"]:
+        roof_lin["measured_GB_per_s"] = roof_lin["traffic"] / t_lin / 1e9
 
     windows = None
     if not args.no_windows:
@@ -430,6 +446,9 @@ def main() -> int:
         "problem_create_s": create_s,
         "rms_px": {"mixed": rms_mixed, "fp64": rms_fp64, "diff": abs(rms_mixed - rms_fp64)},
         "residuals": {"mixed": q_mixed, "fp64": q_fp64},
+        "rms_px_at_fp64_cost": None if q_reach is None else
+            {"mixed": q_reach["rms_px"], "fp64": rms_fp64, "diff": abs(q_reach["rms_px"] - rms_fp64),
+             "inlier_rms_mixed": q_reach["inlier_rms_px"], "inlier_rms_fp64": q_fp64["inlier_rms_px"]},
         "pose_error": pe,
         "final_cost": {"mixed": mixed_outs[-1].final_cost, "fp64": fp64_outs[-1].final_cost},
         "mixed": r_mixed,
@@ -437,6 +456,7 @@ def main() -> int:
         "roofline": roof,
         "roofline_factorization": roof_fact,
         "roofline_schur": roof_schur,
+        "roofline_linearize": roof_lin,
         "cpu_baseline": cpu,
         "windows": windows,
         "large_scene": large,
