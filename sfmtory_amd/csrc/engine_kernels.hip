@@ -45,10 +45,25 @@ __global__ __launch_bounds__(256) void k_compute_F(View<T> v) {
 // The Schur reduction on the matrix cores, one wavefront per chunk, chunks
 // in destination-tile order: the wavefronts in flight work on neighbouring
 // destinations and share their F and E rows in L2 and the MALL.
+// Pairs in flight per wavefront and the occupancy asked for, measured on
+// the 1,000-camera scene: FP32 two pairs at eight waves per SIMD (64
+// VGPRs; 2.32 -> 2.02 ms per assembly against four pairs at five waves),
+// FP64 four pairs at the compiler's choice (four waves; eight waves spill).
+// The gathers' latency, not bandwidth or issue, bounds this kernel, so the
+// pairs in flight per SIMD are what count.
 template <typename T>
-__global__ __launch_bounds__(256) void k_schur_mfma(View<T> v, double lambda, double* __restrict__ parts) {
+constexpr int schur_pairs() {
+  return sizeof(T) == 4 ? 2 : 4;
+}
+template <typename T>
+constexpr int schur_min_waves() {
+  return sizeof(T) == 4 ? 8 : 1;
+}
+template <typename T>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(schur_min_waves<T>()))) void k_schur_mfma(
+    View<T> v, double lambda, double* __restrict__ parts) {
   const uint32_t ci = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
-  if (ci < v.n_chunks) schur_chunk_mfma<T, 4>(v, v.chunks[ci], lambda, parts);
+  if (ci < v.n_chunks) schur_chunk_mfma<T, schur_pairs<T>()>(v, v.chunks[ci], lambda, parts);
 }
 
 // Destinations split over several chunks: sum the partials in chunk order.
