@@ -248,13 +248,19 @@ __device__ inline void linearize_obs(const View<T>& v, uint32_t o, int cur, int 
 }
 
 // ------------------------------------------------------- point accumulation
-template <typename T>
-__device__ inline void point_accumulate(const View<T>& v, uint32_t p) {
+// Point p's 3x3 block and gradient from its observations' terms.  With
+// LPP > 1, LPP consecutive lanes share the point: lane r sums observations
+// r, r + LPP, ... (so neighbouring lanes read neighbouring observations'
+// terms), then the lanes' sums are combined by a fixed butterfly and lane 0
+// writes (LPP a power of two <= 64, the lanes of one point in one
+// wavefront).
+template <typename T, int LPP = 1>
+__device__ inline void point_accumulate(const View<T>& v, uint32_t p, uint32_t r = 0) {
   double a[9];
 #pragma unroll
   for (int k = 0; k < 9; ++k) a[k] = 0.0;
-  uint8_t active = 0;
-  for (uint32_t o = v.pt_ptr[p]; o < v.pt_ptr[p + 1]; ++o) {
+  int active = 0;
+  for (uint32_t o = v.pt_ptr[p] + r; o < v.pt_ptr[p + 1]; o += LPP) {
     if (!v.obs_valid[o]) continue;
     if (v.obs_inc[2 * o] != NONE) active = 1;
     const T* c = v.obs_pc + 9 * (size_t)o;
@@ -262,10 +268,17 @@ __device__ inline void point_accumulate(const View<T>& v, uint32_t p) {
     for (int k = 0; k < 9; ++k) a[k] += (double)c[k];
   }
 #pragma unroll
+  for (int m = 1; m < LPP; m <<= 1) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) a[k] += __shfl_xor(a[k], m);
+    active |= __shfl_xor(active, m);
+  }
+  if (r != 0) return;
+#pragma unroll
   for (int k = 0; k < 6; ++k) v.pt_C[6 * (size_t)p + k] = a[k];
 #pragma unroll
   for (int k = 0; k < 3; ++k) v.pt_g[3 * (size_t)p + k] = a[6 + k];
-  v.pt_active[p] = active;
+  v.pt_active[p] = (uint8_t)active;
 }
 
 // Damped point block (C + lambda diag C)^-1 by the adjugate; a point without

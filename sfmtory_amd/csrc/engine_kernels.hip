@@ -21,10 +21,13 @@ __global__ __launch_bounds__(256) void k_linearize(View<T> v, int cur, int loss,
   if (o < v.n_obs) linearize_obs<T, LPO>(v, o, cur, loss, scale, t % LPO);
 }
 
+// LPP lanes per point (neighbouring lanes read neighbouring observations).
+constexpr int ACCUMULATE_LANES = 4;
 template <typename T>
 __global__ __launch_bounds__(256) void k_point_accumulate(View<T> v) {
-  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p < v.n_pt) point_accumulate(v, p);
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t p = t / ACCUMULATE_LANES;
+  if (p < v.n_pt) point_accumulate<T, ACCUMULATE_LANES>(v, p, t % ACCUMULATE_LANES);
 }
 
 template <typename T>
@@ -48,12 +51,13 @@ __global__ __launch_bounds__(256) void k_compute_F(View<T> v) {
 // Pairs in flight per wavefront and the occupancy asked for, measured on
 // the 1,000-camera scene: FP32 two pairs at eight waves per SIMD (64
 // VGPRs; 2.32 -> 2.02 ms per assembly against four pairs at five waves),
-// FP64 four pairs at the compiler's choice (four waves; eight waves spill).
-// The gathers' latency, not bandwidth or issue, bounds this kernel, so the
-// pairs in flight per SIMD are what count.
+// FP64 two pairs at the compiler's choice (66 VGPRs; 3.20 -> 3.07 ms
+// against four pairs; eight waves spill).  The gathers' latency, not
+// bandwidth or issue, bounds this kernel, so the pairs in flight per SIMD
+// are what count.
 template <typename T>
 constexpr int schur_pairs() {
-  return sizeof(T) == 4 ? 2 : 4;
+  return 2;
 }
 template <typename T>
 constexpr int schur_min_waves() {
@@ -386,7 +390,7 @@ void launch_linearize(const Layout& L, Work<T>& W, int cur, int loss, double sca
 template <typename T>
 void launch_point_accumulate(const Layout& L, Work<T>& W, hipStream_t s) {
   if (L.n_pt == 0) return;
-  hipLaunchKernelGGL(k_point_accumulate<T>, dim3(blocks_for(L.n_pt, 256)), dim3(256), 0, s,
+  hipLaunchKernelGGL(k_point_accumulate<T>, dim3(blocks_for((uint64_t)L.n_pt * ACCUMULATE_LANES, 256)), dim3(256), 0, s,
                      make_view(L, W));
   HIP_CHECK(hipGetLastError());
 }
