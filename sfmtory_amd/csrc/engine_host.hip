@@ -550,7 +550,6 @@ static void alloc_pcg(sfmba_problem& H) {
   B.z = A.alloc<double>(n);
   B.p = A.alloc<double>(n);
   B.q = A.alloc<double>(n);
-  B.y2 = A.alloc<double>(n);
   B.symv_part = A.alloc<double>(std::max<size_t>(pcg_symv_part_doubles(L.D), 1));
   B.dot_part = A.alloc<double>(pcg_dot_part_doubles(L.D));
   B.scal = A.alloc<double>(4);
@@ -1344,13 +1343,20 @@ static void batch_plan_build(sfmba_batch& Q, WindowBatch<T>& B, const sfmba_inpu
   const auto t1 = clock::now();
   if (nw) {
     HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    batch_restore(B, Q.d_pristine, s);
-    launch_window_lm<T>(B.d_views, (uint32_t)nw, B.max_dim, B.wp, B.d_res, B.d_oc, Q.shape, s);
-    launch_window_gather((uint32_t)nw, B.d_res, d_meta, d_meta + nw, d_meta + 2 * nw,
-                         reinterpret_cast<const double*>(B.base), Q.d_out, s);
-    HIP_CHECK(hipMemcpyAsync(Q.h_res, B.d_res, nw * sizeof(WindowResult), hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipMemcpyAsync(Q.h_oc, B.d_oc, nw * B.max_it * 8, hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipMemcpyAsync(Q.h_out, Q.d_out, n_out * 8, hipMemcpyDeviceToHost, s));
+    try {
+      batch_restore(B, Q.d_pristine, s);
+      launch_window_lm<T>(B.d_views, (uint32_t)nw, B.max_dim, B.wp, B.d_res, B.d_oc, Q.shape, s);
+      launch_window_gather((uint32_t)nw, B.d_res, d_meta, d_meta + nw, d_meta + 2 * nw,
+                           reinterpret_cast<const double*>(B.base), Q.d_out, s);
+      HIP_CHECK(hipMemcpyAsync(Q.h_res, B.d_res, nw * sizeof(WindowResult), hipMemcpyDeviceToHost, s));
+      HIP_CHECK(hipMemcpyAsync(Q.h_oc, B.d_oc, nw * B.max_it * 8, hipMemcpyDeviceToHost, s));
+      HIP_CHECK(hipMemcpyAsync(Q.h_out, Q.d_out, n_out * 8, hipMemcpyDeviceToHost, s));
+    } catch (...) {
+      // leave capture mode before the error propagates
+      hipGraph_t partial = nullptr;
+      if (hipStreamEndCapture(s, &partial) == hipSuccess && partial) (void)hipGraphDestroy(partial);
+      throw;
+    }
     const hipError_t ce = hipStreamEndCapture(s, &Q.graph);
     if (ce != hipSuccess) throw Error(SFMBA_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(ce));
     HIP_CHECK(hipGraphInstantiate(&Q.exec, Q.graph, nullptr, nullptr, 0));

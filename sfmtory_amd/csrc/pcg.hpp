@@ -12,7 +12,7 @@ struct PcgBuffers {
   double* Minv = nullptr;      // inverses of the diagonal parameter blocks (sum of d^2)
   uint32_t* moff = nullptr;    // per block: offset into Minv
   uint32_t* row_grp = nullptr; // per unknown: its parameter block
-  double *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr, *y2 = nullptr;
+  double *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
   double* symv_part = nullptr; // pcg_symv_part_doubles(D)
   double* dot_part = nullptr;  // pcg_dot_part_doubles(D)
   double* scal = nullptr;      // 2 device scalars

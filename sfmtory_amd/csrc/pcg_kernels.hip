@@ -8,11 +8,15 @@
 //                    (Gauss-Jordan, one wavefront per block, no pivoting: the
 //                    blocks of a damped normal matrix are positive definite;
 //                    a pivot that is not > 0 flags a breakdown)
-//   k_symv_lower     y1 = L x with L the lower triangle (diagonal included):
-//                    256-row blocks x 1024-column chunks, per-chunk partials
-//   k_symv_upper     y2 = (L - diag)^T x: one wavefront per column (a
-//                    column of the lower triangle is contiguous)
-//   k_symv_finish    y = y1 + y2, partials summed in chunk order
+//   k_symv_fused     one pass over the lower triangle L: a workgroup reads
+//                    a 256-row x 1024-column tile once and forms both its
+//                    row products (L x, per-chunk partials) and the column
+//                    products of its strictly lower part ((L - diag)^T x,
+//                    one wavefront reduction per column, the four
+//                    wavefronts combined in a fixed order, per-row-block
+//                    partials)
+//   k_symv_finish    y = the row partials in chunk order + the column
+//                    partials in row-block order
 //   k_dot2 / k_dot2_final
 //                    two dot products, deterministic two-level reduction
 // Every reduction has a fixed order, so a solve is reproducible.
@@ -84,44 +88,89 @@ __global__ __launch_bounds__(256) void k_bj_apply(uint32_t D, const uint32_t* __
   z[i] = s;
 }
 
-__global__ __launch_bounds__(SYMV_ROWS) void k_symv_lower(uint32_t D, const double* __restrict__ S, size_t ld,
+// prow[cc * D + i]: row i's products over column chunk cc (columns <= i);
+// pcol[rb * D + j]: column j's products over the rows > j of row block rb.
+// Tiles right of the diagonal are skipped; every partial the finish reads
+// belongs to a tile that is not.
+__global__ __launch_bounds__(SYMV_ROWS) void k_symv_fused(uint32_t D, const double* __restrict__ S, size_t ld,
                                                           const double* __restrict__ x,
-                                                          double* __restrict__ part) {
+                                                          double* __restrict__ prow, double* __restrict__ pcol) {
   __shared__ double xs[SYMV_COLS];
+  __shared__ double cs[SYMV_ROWS / 64][SYMV_COLS];
   const uint32_t rb = blockIdx.x, cc = blockIdx.y;
-  const uint32_t c0 = cc * SYMV_COLS;
-  if (c0 > rb * SYMV_ROWS + SYMV_ROWS - 1) return;  // chunk right of the diagonal: nothing
+  const uint32_t r0 = rb * SYMV_ROWS, c0 = cc * SYMV_COLS;
+  if (c0 > r0 + SYMV_ROWS - 1) return;  // chunk right of the diagonal: nothing
   const uint32_t c1 = min(D, c0 + SYMV_COLS);
-  for (uint32_t j = threadIdx.x; j < c1 - c0; j += SYMV_ROWS) xs[j] = x[c0 + j];
+  const uint32_t nc = c1 - c0;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  for (uint32_t j = t; j < nc; j += SYMV_ROWS) xs[j] = x[c0 + j];
+  for (uint32_t j = t; j < SYMV_COLS; j += SYMV_ROWS)
+#pragma unroll
+    for (int q = 0; q < SYMV_ROWS / 64; ++q) cs[q][j] = 0.0;
   __syncthreads();
-  const uint32_t i = rb * SYMV_ROWS + threadIdx.x;
-  if (i >= D) return;
-  const uint32_t jend = min(c1, i + 1);
+  const uint32_t i = r0 + t;
+  const bool row_ok = i < D;
+  const double xi = row_ok ? x[i] : 0.0;
+  // columns past this wavefront's last row hold no lower-triangle entries
+  const uint32_t wlast = r0 + 64 * w + 63;
+  const uint32_t jend = min(c1, wlast + 1);
   double acc = 0.0;
-  for (uint32_t j = c0; j < jend; ++j) acc += S[(size_t)j * ld + i] * xs[j - c0];
-  part[(size_t)cc * D + i] = acc;
+  // Eight columns at a time: eight independent loads, then the column sums
+  // of the eight by recursive halving (lanes l and l ^ 32 / 16 / 8 split
+  // and add the eight values, then xor 4 / 2 / 1 finish the sum): lane l
+  // ends with column 4 b5 + 2 b4 + b3 (bits of l) summed over the wavefront.
+  for (uint32_t j0 = c0; j0 < jend; j0 += 8) {
+    double v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const uint32_t j = j0 + k;
+      const double a = (row_ok && j < jend && j <= i) ? S[(size_t)j * ld + i] : 0.0;
+      acc += a * (j < jend ? xs[j - c0] : 0.0);
+      v[k] = (j < i) ? a * xi : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const bool up = lane & 32;
+      const double send = up ? v[k] : v[k + 4], keep = up ? v[k + 4] : v[k];
+      v[k] = keep + __shfl_xor(send, 32);
+    }
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const bool up = lane & 16;
+      const double send = up ? v[k] : v[k + 2], keep = up ? v[k + 2] : v[k];
+      v[k] = keep + __shfl_xor(send, 16);
+    }
+    {
+      const bool up = lane & 8;
+      const double send = up ? v[0] : v[1], keep = up ? v[1] : v[0];
+      v[0] = keep + __shfl_xor(send, 8);
+    }
+    v[0] += __shfl_xor(v[0], 4);
+    v[0] += __shfl_xor(v[0], 2);
+    v[0] += __shfl_xor(v[0], 1);
+    const uint32_t jc = j0 + ((lane >> 5) & 1) * 4 + ((lane >> 4) & 1) * 2 + ((lane >> 3) & 1);
+    if ((lane & 7) == 0 && jc < jend) cs[w][jc - c0] = v[0];
+  }
+  if (row_ok) prow[(size_t)cc * D + i] = acc;
+  __syncthreads();
+  for (uint32_t j = t; j < nc; j += SYMV_ROWS) {
+    double v = 0.0;
+#pragma unroll
+    for (int q = 0; q < SYMV_ROWS / 64; ++q) v += cs[q][j];
+    pcol[(size_t)rb * D + c0 + j] = v;
+  }
 }
 
-__global__ __launch_bounds__(256) void k_symv_upper(uint32_t D, const double* __restrict__ S, size_t ld,
-                                                    const double* __restrict__ x, double* __restrict__ y2) {
-  const uint32_t lane = threadIdx.x & 63;
-  const uint32_t c = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (c >= D) return;
-  const double* col = S + (size_t)c * ld;
-  double s = 0.0;
-  for (uint32_t r = c + 1 + lane; r < D; r += 64) s += col[r] * x[r];
-  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-  if (lane == 0) y2[c] = s;
-}
-
-__global__ __launch_bounds__(256) void k_symv_finish(uint32_t D, const double* __restrict__ part,
-                                                     const double* __restrict__ y2, double* __restrict__ y) {
+__global__ __launch_bounds__(256) void k_symv_finish(uint32_t D, const double* __restrict__ prow,
+                                                     const double* __restrict__ pcol, double* __restrict__ y) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= D) return;
   const uint32_t ncc = i / SYMV_COLS + 1;  // chunks left of / at the diagonal
   double s = 0.0;
-  for (uint32_t cc = 0; cc < ncc; ++cc) s += part[(size_t)cc * D + i];
-  y[i] = s + y2[i];
+  for (uint32_t cc = 0; cc < ncc; ++cc) s += prow[(size_t)cc * D + i];
+  const uint32_t nrb = (D + SYMV_ROWS - 1) / SYMV_ROWS;
+  for (uint32_t rb = i / SYMV_ROWS; rb < nrb; ++rb) s += pcol[(size_t)rb * D + i];
+  y[i] = s;
 }
 
 // part[2 b], part[2 b + 1] = block b's sums of a.b and c.d.
@@ -207,15 +256,16 @@ __global__ __launch_bounds__(256) void k_pcg_init(uint32_t n, const double* __re
 
 static inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
-size_t pcg_symv_part_doubles(uint32_t D) { return (size_t)cdiv(D, SYMV_COLS) * D; }
+size_t pcg_symv_part_doubles(uint32_t D) { return ((size_t)cdiv(D, SYMV_COLS) + cdiv(D, SYMV_ROWS)) * D; }
 size_t pcg_dot_part_doubles(uint32_t D) { return 2 * (size_t)cdiv(D, DOT_BLOCK * DOT_ITEMS) + 2; }
 
 static void symv(const PcgBuffers& B, const double* S, size_t ld, uint32_t D, const double* x, double* y,
                  hipStream_t s) {
-  hipLaunchKernelGGL(k_symv_lower, dim3(cdiv(D, SYMV_ROWS), cdiv(D, SYMV_COLS)), dim3(SYMV_ROWS), 0, s, D,
-                     S, ld, x, B.symv_part);
-  hipLaunchKernelGGL(k_symv_upper, dim3(cdiv(D, 4)), dim3(256), 0, s, D, S, ld, x, B.y2);
-  hipLaunchKernelGGL(k_symv_finish, dim3(cdiv(D, 256)), dim3(256), 0, s, D, B.symv_part, B.y2, y);
+  double* prow = B.symv_part;
+  double* pcol = B.symv_part + (size_t)cdiv(D, SYMV_COLS) * D;
+  hipLaunchKernelGGL(k_symv_fused, dim3(cdiv(D, SYMV_ROWS), cdiv(D, SYMV_COLS)), dim3(SYMV_ROWS), 0, s, D, S,
+                     ld, x, prow, pcol);
+  hipLaunchKernelGGL(k_symv_finish, dim3(cdiv(D, 256)), dim3(256), 0, s, D, prow, pcol, y);
 }
 
 static void dot2(const PcgBuffers& B, uint32_t n, const double* a, const double* b, const double* c,
