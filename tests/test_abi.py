@@ -81,3 +81,45 @@ def test_no_device_fails_loudly():
     with pytest.raises(N.EngineError) as e:
         ba.bundle_adjust(sc.inp)
     assert e.value.code == N.SFMBA_ERR_DEVICE
+
+
+# ctypes mirror -> C struct of include/sfmba.h
+MIRRORS = {
+    "Camera": "sfmba_camera", "Pose": "sfmba_pose", "Observation": "sfmba_observation", "Input": "sfmba_input",
+    "Params": "sfmba_params", "Output": "sfmba_output", "Options": "sfmba_options", "Stats": "sfmba_stats",
+    "Policy": "sfmba_policy", "PolicyReport": "sfmba_policy_report", "IntrinsicsParams": "sfmba_intrinsics_params",
+    "ColmapRefineOptions": "sfmba_colmap_refine_options", "ColmapRefineReport": "sfmba_colmap_refine_report",
+    "Reconstruction": "sfmba_reconstruction", "WindowSummary": "sfmba_window_summary",
+    "BatchInfo": "sfmba_batch_info",
+}
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    """Every ctypes mirror has the size, and every field the offset, that a
+    C compiler gives the header's struct (field names as in the header)."""
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    lines = ["#include <stdio.h>", "#include <stddef.h>", f'#include "{HEADER}"', "int main(void) {"]
+    for py, c in MIRRORS.items():
+        lines.append(f'  printf("{py} size %zu\\n", sizeof({c}));')
+        for name, *_ in getattr(N, py)._fields_:
+            lines.append(f'  printf("{py} {name} %zu\\n", offsetof({c}, {name}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-std=c11", "-o", str(exe), str(src)], check=True, capture_output=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {}
+    for line in out:
+        if line:
+            py, field, val = line.split()
+            got[(py, field)] = int(val)
+    for py in MIRRORS:
+        st = getattr(N, py)
+        assert got[(py, "size")] == C.sizeof(st), py
+        for name, *_ in st._fields_:
+            assert got[(py, name)] == getattr(st, name).offset, (py, name)
