@@ -81,6 +81,24 @@ def test_no_device_fails_loudly():
     with pytest.raises(N.EngineError) as e:
         ba.bundle_adjust(sc.inp)
     assert e.value.code == N.SFMBA_ERR_DEVICE
+    with pytest.raises(N.EngineError) as e:
+        ba.BatchPlan([sc.inp], ba.BaParams(max_iterations=2), autotune_iterations=1)
+    assert e.value.code == N.SFMBA_ERR_DEVICE
+
+
+def test_batch_plan_argument_errors_need_no_device():
+    lib = N.load_engine()
+    h = C.c_void_p()
+    p = N.Params()
+    lib.sfmba_params_default(C.byref(p))
+    # NULL output handle, negative tuning iterations, NULL params
+    assert lib.sfmba_batch_create(None, 0, C.byref(p), None, 0, None) == N.SFMBA_ERR_INVALID_ARGUMENT
+    assert lib.sfmba_batch_create(None, 0, C.byref(p), None, -1, C.byref(h)) == N.SFMBA_ERR_INVALID_ARGUMENT
+    assert lib.sfmba_batch_create(None, 0, None, None, 0, C.byref(h)) == N.SFMBA_ERR_INVALID_ARGUMENT
+    assert not h
+    assert lib.sfmba_batch_solve(None, None, None) == N.SFMBA_ERR_INVALID_ARGUMENT
+    assert lib.sfmba_batch_get_info(None, None) == N.SFMBA_ERR_INVALID_ARGUMENT
+    lib.sfmba_batch_destroy(None)
 
 
 # ctypes mirror -> C struct of include/sfmba.h
