@@ -607,6 +607,138 @@ __device__ inline void schur_chunk_mfma(const View<T>& v, const uint4 ch, double
   }
 }
 
+// schur_chunk_mfma for blocks of at most 8x8 (da, db <= 8), two pairs per
+// MFMA: rows 0-7 of both operands carry the even pair of a step, rows 8-15
+// the odd pair, so the 16x16 tile's two diagonal quadrants sum the even and
+// the odd pairs of the chunk and its off-diagonal quadrants (cross terms)
+// are discarded.  Lane l loads row l % 8 of pair 2u + (l / 8) % 2,
+// component l / 16: per load instruction twice the pairs of the one-pair
+// form, at the same registers.  The quadrants are added at the end (even +
+// odd); the FP32 instantiation flushes after every U steps, so no more than
+// U terms are summed in FP32 per quadrant.
+template <typename T, int U>
+__device__ inline void schur_chunk_mfma8(const View<T>& v, const uint4 ch, double lambda, double* parts) {
+  using M = Mfma<T>;
+  using acc_t = typename M::acc_t;
+  constexpr uint32_t HI = 0x80000000u, OFF = 0x7FFFFFFFu;
+  static_assert(PAIR_CHUNK <= 128, "a chunk's pair words are held in two registers per lane");
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t i = lane & 7, h = (lane >> 3) & 1, k = lane >> 4;
+  const uint32_t ga = ch.x >> 16, gb = ch.x & 0xFFFFu;
+  const uint32_t da = v.grp_dim[ga], db = v.grp_dim[gb];
+  const bool diag = ga == gb;
+  const bool fa_on = i < da && k < 3, eb_on = i < db && k < 3;
+  const bool ja_on = i < da && k < 2, jb_on = i < db && k < 2;
+  const T* F = reinterpret_cast<const T*>(v.F4);
+  const T* E = reinterpret_cast<const T*>(v.E4);
+  const T* J = reinterpret_cast<const T*>(v.J4);
+  const uint32_t n = ch.z - ch.y;
+  const uint2 w0 = lane < n ? v.pairs[ch.y + lane] : make_uint2(0u, 0u);
+  const uint2 w1 = lane + 64 < n ? v.pairs[ch.y + 64 + lane] : make_uint2(0u, 0u);
+  // Pair q's words (zero for q >= n).
+  auto word = [&](uint32_t q, uint32_t& x, uint32_t& y) {
+    x = 0u;
+    y = 0u;
+    if (q < n) {
+      const uint32_t sl = q & 63;
+      x = __builtin_amdgcn_readlane(q < 64 ? w0.x : w1.x, sl);
+      y = __builtin_amdgcn_readlane(q < 64 ? w0.y : w1.y, sl);
+    }
+  };
+  struct Stage {
+    T fa[U], eb[U], ja[U], jb[U];
+  };
+  // Operands of U steps (2U pairs) starting at pair q.
+  auto load = [&](uint32_t q, Stage& st) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      uint32_t x0, y0, x1, y1;
+      word(q + 2 * u, x0, y0);
+      word(q + 2 * u + 1, x1, y1);
+      const bool ok = q + 2 * u + h < n;
+      const uint32_t px = h ? x1 : x0, py = h ? y1 : y0;
+      const uint32_t ra = px & OFF, rb = py & OFF;
+      st.fa[u] = (ok && fa_on) ? F[4 * (size_t)(ra + i) + k] : T(0);
+      st.eb[u] = (ok && eb_on) ? E[4 * (size_t)(rb + i) + k] : T(0);
+      const bool same = ok && (px & HI);
+      st.ja[u] = (same && ja_on) ? J[4 * (size_t)(ra + i) + 2 + k] : T(0);
+      st.jb[u] = (same && jb_on) ? J[4 * (size_t)(rb + i) + k] : T(0);
+    }
+  };
+  acc_t s = {T(0), T(0), T(0), T(0)}, b = {T(0), T(0), T(0), T(0)};
+  double ss[4] = {0.0, 0.0, 0.0, 0.0}, bs[4] = {0.0, 0.0, 0.0, 0.0};
+  Acc<T> accR;
+  const bool rlane = diag && lane < da;
+  Stage cur, nxt;
+  load(0, cur);
+  for (uint32_t q = 0; q < n; q += 2 * U) {
+    if (q + 2 * U < n) load(q + 2 * U, nxt);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (q + 2 * u >= n) break;
+      uint32_t x0, y0, x1, y1;
+      word(q + 2 * u, x0, y0);
+      word(q + 2 * u + 1, x1, y1);
+      s = M::step(cur.fa[u], cur.eb[u], s);
+      if ((x0 | x1) & HI) b = M::step(cur.ja[u], cur.jb[u], b);
+      // right-hand side in pair order
+      if (rlane && (y0 & HI)) accR.add(v.F4[(x0 & OFF) + lane].w - v.E4[(y0 & OFF) + lane].w);
+      if (rlane && (y1 & HI)) accR.add(v.F4[(x1 & OFF) + lane].w - v.E4[(y1 & OFF) + lane].w);
+    }
+    if (sizeof(T) == 4) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        ss[c] += (double)s[c];
+        bs[c] += (double)b[c];
+        s[c] = T(0);
+        b[c] = T(0);
+      }
+    }
+    cur = nxt;
+  }
+  if (sizeof(T) == 8) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      ss[c] = (double)s[c];
+      bs[c] = (double)b[c];
+    }
+  }
+  // Quadrant (8 + r, 8 + c) sits 40 lanes up in the same register (FP32
+  // layout, row 4 (l / 16) + v) or 8 lanes up two registers on (FP64
+  // layout, row l / 16 + 4 v).
+  // Only registers holding rows 0-7 are combined (FP64: registers 0 and 1).
+#pragma unroll
+  for (int q = 0; q < (sizeof(T) == 4 ? 4 : 2); ++q) {
+    const int src = sizeof(T) == 4 ? q : q + 2;
+    const int dl = sizeof(T) == 4 ? 40 : 8;
+    ss[q] += __shfl(ss[src], (int)((lane + dl) & 63));
+    bs[q] += __shfl(bs[src], (int)((lane + dl) & 63));
+  }
+  const uint32_t oa = v.grp_off[ga], ob = v.grp_off[gb];
+  const uint32_t c = lane & 15;
+  if (ch.w == NONE) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const uint32_t r = M::row(lane, q);
+      if (r >= da || c >= db || (diag && r < c)) continue;
+      double val = bs[q] - ss[q];
+      if (diag && r == c) val = (bs[q] + bs[q] * lambda) - ss[q];
+      v.S[(size_t)(ob + c) * v.ld + oa + r] = val;
+    }
+    if (rlane) v.S[(size_t)(oa + lane) * v.ld + v.Dp] = accR.value();
+  } else {
+    double* pp = parts + (size_t)ch.w * PART_STRIDE;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const uint32_t r = M::row(lane, q);
+      if (r >= da || c >= db) continue;
+      pp[r + c * da] = bs[q];
+      pp[196 + r + c * da] = ss[q];
+    }
+    if (rlane) pp[392 + lane] = accR.value();
+  }
+}
+
 // ------------------------------------------------- back-substitution + update
 template <typename T>
 __device__ inline void backsub_point(const View<T>& v, uint32_t p, int cur) {

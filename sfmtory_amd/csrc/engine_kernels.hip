@@ -48,26 +48,37 @@ __global__ __launch_bounds__(256) void k_compute_F(View<T> v) {
 // The Schur reduction on the matrix cores, one wavefront per chunk, chunks
 // in destination-tile order: the wavefronts in flight work on neighbouring
 // destinations and share their F and E rows in L2 and the MALL.
-// Pairs in flight per wavefront and the occupancy asked for, measured on
-// the 1,000-camera scene: FP32 two pairs at eight waves per SIMD (64
-// VGPRs; 2.32 -> 2.02 ms per assembly against four pairs at five waves),
-// FP64 two pairs at the compiler's choice (66 VGPRs; 3.20 -> 3.07 ms
-// against four pairs; eight waves spill).  The gathers' latency, not
-// bandwidth or issue, bounds this kernel, so the pairs in flight per SIMD
-// are what count.
+// Steps in flight per wavefront and the occupancy asked for, measured on
+// the 1,000-camera scene (every block 8x8, so the packed form: two pairs
+// per step):
+//   FP32  two steps at seven waves per SIMD (68 VGPRs): 1.72 ms per
+//         assembly; eight waves spill (2.00 ms), one step at eight waves
+//         1.80 ms, four steps at four waves 1.88 ms;
+//   FP64  two steps at six waves (77 VGPRs): 2.10 ms; the compiler's choice
+//         (five waves) 2.31 ms, four steps 2.45 ms.
+// The one-pair form took 2.02 ms (FP32, eight waves) and 3.07 ms (FP64,
+// five waves) on the same scene; it keeps those settings for blocks above
+// 8x8.  The gathers' latency, not bandwidth or issue, bounds this kernel,
+// so the pairs in flight per SIMD are what count.
 template <typename T>
 constexpr int schur_pairs() {
   return 2;
 }
-template <typename T>
+template <typename T, bool PACK>
 constexpr int schur_min_waves() {
-  return sizeof(T) == 4 ? 8 : 1;
+  return PACK ? (sizeof(T) == 4 ? 7 : 6) : (sizeof(T) == 4 ? 8 : 1);
 }
-template <typename T>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(schur_min_waves<T>()))) void k_schur_mfma(
+// PACK: every block is at most 8x8 (the largest parameter block of the
+// problem has at most 8 unknowns), two pairs per MFMA.
+template <typename T, bool PACK>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(schur_min_waves<T, PACK>()))) void k_schur_mfma(
     View<T> v, double lambda, double* __restrict__ parts) {
   const uint32_t ci = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
-  if (ci < v.n_chunks) schur_chunk_mfma<T, schur_pairs<T>()>(v, v.chunks[ci], lambda, parts);
+  if (ci >= v.n_chunks) return;
+  if (PACK)
+    schur_chunk_mfma8<T, schur_pairs<T>()>(v, v.chunks[ci], lambda, parts);
+  else
+    schur_chunk_mfma<T, schur_pairs<T>()>(v, v.chunks[ci], lambda, parts);
 }
 
 // Destinations split over several chunks: sum the partials in chunk order.
@@ -418,7 +429,11 @@ void launch_schur(const Layout& L, Work<T>& W, double lambda, hipStream_t s) {
   const View<T> v = make_view(L, W);
   if (L.n_chunks) {
     if (L.max_grp_dim > 16) throw Error(SFMBA_ERR_INVALID_ARGUMENT, "group dimension above 16");
-    hipLaunchKernelGGL(k_schur_mfma<T>, dim3(blocks_for(L.n_chunks, 4)), dim3(256), 0, s, v, lambda, W.parts);
+    const dim3 grid(blocks_for(L.n_chunks, 4));
+    if (L.max_grp_dim <= 8)
+      hipLaunchKernelGGL((k_schur_mfma<T, true>), grid, dim3(256), 0, s, v, lambda, W.parts);
+    else
+      hipLaunchKernelGGL((k_schur_mfma<T, false>), grid, dim3(256), 0, s, v, lambda, W.parts);
     HIP_CHECK(hipGetLastError());
   }
   if (L.n_multi) {

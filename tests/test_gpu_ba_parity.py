@@ -63,6 +63,19 @@ def test_multi_block_reduced_system(n_img):
     assert_parity(sc.inp, ba.BaParams(max_iterations=15))
 
 
+@pytest.mark.parametrize("n_free", [1, 2, 3, 4])
+def test_parameter_block_sizes_on_both_schur_forms(n_free):
+    # A pose block (6) and its own camera's free SIMPLE_RADIAL parameters
+    # form one block: 7 or 8 unknowns take the Schur assembly's two-pairs-
+    # per-MFMA form (every block at most 8x8), 9 or 10 the one-pair form.
+    sc = scenes.generate("c0", num_points=400, seed=13 + n_free)
+    inp = sc.inp
+    inp.fixed_camera_params = [[k >= n_free for k in range(4)] for _ in range(len(inp.cameras))]
+    out, _ = assert_parity(inp, ba.BaParams(max_iterations=12))
+    # pose 0 is fixed, so its camera is a block of its own
+    assert out.stats["dim"] == 15 * (6 + n_free) + n_free
+
+
 def test_opencv_c1_shaped_small():
     sc = scenes.generate("c1", num_images=60, num_points=4000)
     assert_parity(sc.inp, ba.BaParams(max_iterations=8))
