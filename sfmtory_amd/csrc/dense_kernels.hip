@@ -142,6 +142,97 @@ __device__ inline void tile_mma(double4_t acc[4][4], const double* __restrict__ 
   }
 }
 
+// The diagonal tile's update, lower triangle only: its 36 16x16 tiles
+// (row tile ti >= column tile tj) split nine per wavefront.  Wavefront W
+// takes row tiles 7 - W (column tiles 0..7-W, accumulators 0..7-W) and W
+// (column tiles 0..W, accumulators 8-W..8), so one K step needs at most
+// eight column fragments and two row fragments of the panel.  tile_mma's
+// 64x64 split gives one wavefront 16 tiles, and the diagonal tile's MFMA
+// chain is on the factorization's critical path.  Element v of lane l of
+// tile (ti, tj) is C[ti*16 + l%16][tj*16 + l/16 + 4v]: the same products
+// in the same K order as tile_mma.  W is a template argument so that every
+// accumulator index is a constant (nine accumulators, no dynamic indexing);
+// callers switch on the wavefront index.
+template <int W>
+struct DiagSplit {
+  static constexpr int NH = 8 - W, NL = W + 1;
+  static __device__ __attribute__((always_inline)) inline int row_tile(int e) { return e < NH ? 7 - W : W; }
+  static __device__ __attribute__((always_inline)) inline int col_tile(int e) { return e < NH ? e : e - NH; }
+  // one K step of four: row points at the step's panel row for this lane
+  static __device__ __attribute__((always_inline)) inline void step(double4_t acc[9], const double* row) {
+    double a[NH];
+#pragma unroll
+    for (int q = 0; q < NH; ++q) a[q] = -row[q * 16];
+    const double bh = row[(7 - W) * 16], bl = row[W * 16];
+#pragma unroll
+    for (int q = 0; q < NH; ++q) acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[q], bh, acc[q], 0, 0, 0);
+#pragma unroll
+    for (int q = 0; q < NL; ++q)
+      acc[NH + q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[q], bl, acc[NH + q], 0, 0, 0);
+  }
+  static __device__ __attribute__((always_inline)) inline void load(double4_t acc[9], const double* C, size_t ldc) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int e = 0; e < 9; ++e)
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        acc[e][v] = C[(size_t)(col_tile(e) * 16 + (lane >> 4) + 4 * v) * ldc + row_tile(e) * 16 + (lane & 15)];
+  }
+  // every entry of the lower 16x16 tiles into the tiled LDS block, the
+  // identity on padding
+  static __device__ __attribute__((always_inline)) inline void to_lds(const double4_t acc[9], double* L, uint32_t base,
+                                                                      uint32_t D) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int e = 0; e < 9; ++e)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = row_tile(e) * 16 + (lane & 15), c = col_tile(e) * 16 + (lane >> 4) + 4 * v;
+        const bool pad = base + r >= D || base + c >= D;
+        L[tile_index(row_tile(e), col_tile(e)) * 256 + 64 * v + lane] = pad ? (r == c ? 1.0 : 0.0) : acc[e][v];
+      }
+  }
+};
+
+#define SFMBA_DIAG_SWITCH(w, CALL) \
+  switch (w) {                      \
+    case 0: DiagSplit<0>::CALL; break; \
+    case 1: DiagSplit<1>::CALL; break; \
+    case 2: DiagSplit<2>::CALL; break; \
+    default: DiagSplit<3>::CALL; break; \
+  }
+
+// The diagonal workgroup's update of its tile (C -= A A^T over K columns of
+// the panel A) and the result into the tiled LDS block.  The panel's first
+// chunk was issued by the caller (tile_mma_begin with A = B); staging and
+// barriers are tile_mma's.
+__device__ inline void diag_tile_update(const double* __restrict__ C, size_t ldc, const double* __restrict__ A,
+                                        size_t lda, int K, double* As, uint32_t base, uint32_t D) {
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  double4_t acc[9];
+  SFMBA_DIAG_SWITCH(w, load(acc, C, ldc))
+  __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  __syncthreads();
+  int buf = 0;
+  for (int k0 = 0; k0 < K; k0 += KC) {
+    if (k0 + KC < K) {
+      double* An = As + (buf ^ 1) * 2 * KC * KS;
+      chunk_to_lds(A, lda, A, lda, k0 + KC, An, An + KC * KS);
+    }
+    const double* Ac = As + buf * 2 * KC * KS;
+#pragma unroll
+    for (int kk = 0; kk < KC; kk += 4) {
+      const double* row = Ac + (kk + (lane >> 4)) * KS + (lane & 15);
+      SFMBA_DIAG_SWITCH(w, step(acc, row))
+    }
+    __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    buf ^= 1;
+  }
+  SFMBA_DIAG_SWITCH(w, to_lds(acc, As, base, D))
+}
+
 // The trailing-update tiles are computed transposed, acc = C^T: tile_mma
 // is given the column panel as A and the row panel as B, so element
 // (m, n, v) of lane l is C[r][c] with r = wc*64 + n*16 + l%16 and
@@ -182,24 +273,6 @@ __device__ inline void tile_store_t(const double4_t acc[4][4], double* C, size_t
         const uint32_t g = row0 + r;
         if ((!lower_only || r >= c) && (g < D || g == Dp) && col0 + c < D)
           C[(size_t)c * ldc + r] = acc[m][n][v];
-      }
-}
-
-// The (transposed) accumulator of a diagonal tile into the tiled LDS block,
-// lower triangle, with the identity on padding rows/columns (global index
-// >= D).
-__device__ inline void tile_to_lds(const double4_t acc[4][4], double* L, uint32_t base, uint32_t D) {
-  const int lane = threadIdx.x & 63;
-#pragma unroll
-  for (int m = 0; m < 4; ++m)
-#pragma unroll
-    for (int n = 0; n < 4; ++n)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int r = acc_row(n, lane), c = acc_col(m, v, lane);
-        if ((r >> 4) < (c >> 4)) continue;
-        const bool pad = base + r >= D || base + c >= D;
-        L[doff(r, c)] = pad ? (r == c ? 1.0 : 0.0) : acc[m][n][v];
       }
 }
 
@@ -549,8 +622,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   // C - sum L L^T).
   const uint32_t lin = blockIdx.y * gridDim.x + blockIdx.x;
   const bool late = (lin >> 8) & 1u;
+  // The diagonal workgroup runs tile_mma with K = 0 (it only waits for the
+  // first staged chunk) and updates its tile's lower triangle with
+  // diag_tile_update instead: 7.43 -> 7.30 ms per factorization at
+  // D = 7,994, same results bit for bit.
+  const bool dwg = i == j0 && j == j0;
   tile_mma_begin(Aj, ld, Ai, ld, smem);
-  if (!late) {
+  if (!late && !dwg) {
     tile_load_t(acc, C, ld);
   } else {
 #pragma unroll
@@ -558,7 +636,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 #pragma unroll
       for (int n = 0; n < 4; ++n) acc[m][n] = double4_t{0.0, 0.0, 0.0, 0.0};
   }
-  tile_mma(acc, Aj, ld, Ai, ld, K, -1.0, smem, smem + KC * KS);
+  tile_mma(acc, Aj, ld, Ai, ld, dwg ? 0 : K, -1.0, smem, smem + KC * KS);
   if (late) {
     const int lane = threadIdx.x & 63;
 #pragma unroll
@@ -574,9 +652,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         for (int v = 0; v < 4; ++v) acc[m][n][v] += cm[n][v];
     }
   }
-  if (i == j0 && j == j0) {
-    __syncthreads();
-    tile_to_lds(acc, smem, (uint32_t)j * NB, D);
+  if (dwg) {
+    diag_tile_update(C, ld, Aj, ld, K, smem, (uint32_t)j * NB, D);
     diag_factor(smem, S, ld, j0, D, Top, flags);
     return;
   }
