@@ -324,27 +324,53 @@ def main() -> int:
     mixed_t = float(np.mean(mixed_walls))
     fp64_t = float(np.mean(fp64_walls))
     # Like-for-like: the mixed solve stopped at the iteration where it first
-    # reaches the FP64 solve's final cost (same trajectory, cut short).
+    # reaches the FP64 solve's final cost (same trajectory, cut short), and
+    # then, if its RMS reprojection error there is not yet within 1e-4 px of
+    # the FP64 solve's, at the first later iteration where it is (searched up
+    # to two iterations past the FP64 solve's count).  The plain RMS drifts by
+    # a few 1e-4 px per iteration near convergence (the Huber objective trades
+    # outlier residuals), so equal cost alone does not pin it.
     fp64_final = fp64_outs[-1].final_cost
+    q_fp64 = residual_stats(ba, inp, fp64_outs[-1])
+    rms_fp64 = q_fp64["rms_px"]
     mixed_costs = prof_mixed["outer_costs"]
     reach = next((i + 1 for i, c in enumerate(mixed_costs) if c <= fp64_final), None)
     reach_t = None
     q_reach = None
-    if reach is not None:
-        p_reach = ba.BaParams(max_iterations=reach)
-        prob.solve(p_reach, precision="mixed")
+    cut, q_cut, cut_t, cut_search = None, None, None, []
+
+    def timed_cut(k):
+        p_k = ba.BaParams(max_iterations=k)
+        prob.solve(p_k, precision="mixed")
         tr = []
         for _ in range(max(1, args.steps)):
             t = time.perf_counter()
-            out_reach = prob.solve(p_reach, precision="mixed")
+            out_k = prob.solve(p_k, precision="mixed")
             tr.append(time.perf_counter() - t)
-        reach_t = float(np.mean(tr))
+        return float(np.mean(tr)), out_k
+
+    if reach is not None:
+        reach_t, out_reach = timed_cut(reach)
         q_reach = residual_stats(ba, inp, out_reach)
         log(f"mixed reaches the FP64 final cost at iteration {reach}: {reach_t:.3f}s")
+        last = max(reach, fp64_outs[-1].iterations_run + 2)
+        for k in range(reach, last + 1):
+            if k == reach:
+                q_k, c_k = q_reach, out_reach.final_cost
+            else:
+                o_k = prob.solve(ba.BaParams(max_iterations=k), precision="mixed")
+                q_k, c_k = residual_stats(ba, inp, o_k), o_k.final_cost
+            d_k = abs(q_k["rms_px"] - rms_fp64)
+            cut_search.append({"iterations": k, "cost": c_k, "rms_px": q_k["rms_px"], "rms_diff_px": d_k})
+            if d_k <= 1e-4:
+                cut, q_cut = k, q_k
+                break
+        if cut is not None:
+            cut_t = reach_t if cut == reach else timed_cut(cut)[0]
+            log(f"mixed within 1e-4 px of the FP64 RMS at iteration {cut}: {cut_t:.3f}s")
     D = prof_fp64["dim"]
     q_mixed = residual_stats(ba, inp, mixed_outs[-1])
-    q_fp64 = residual_stats(ba, inp, fp64_outs[-1])
-    rms_mixed, rms_fp64 = q_mixed["rms_px"], q_fp64["rms_px"]
+    rms_mixed = q_mixed["rms_px"]
     pe = {"mixed": pose_error(mixed_outs[-1].poses, sc.poses_true),
           "fp64": pose_error(fp64_outs[-1].poses, sc.poses_true),
           "initial": pose_error(inp.poses, sc.poses_true)}
@@ -414,10 +440,13 @@ def main() -> int:
         cpu = cpu_baseline(inp, params, prof_fp64, threads)
         log(f"cpu baseline estimate {cpu['value']:.1f}s")
 
-    # value: the mixed form's time to converge to the FP64 plain solve (its
-    # final cost), a timed solve cut at the first iteration that reaches it;
-    # the mixed solve run to the reference's own stopping rule is beside it.
-    value = reach_t if reach_t is not None else mixed_t
+    # value: the mixed form's time to converge to the FP64 plain solve: a
+    # timed solve cut at the first iteration that reaches its final cost and
+    # has its RMS reprojection error within 1e-4 px (the equal-cost cut if no
+    # iteration in the search meets the RMS bound; then rms_criterion_met is
+    # false); the mixed solve run to the reference's own stopping rule is
+    # beside it.
+    value = cut_t if cut_t is not None else (reach_t if reach_t is not None else mixed_t)
     line = {
         "metric": "BA time-to-converge (s) + Jac/Schur obs/s, mixed vs FP64 plain, 1000-cam scene",
         "value": value,
@@ -434,7 +463,10 @@ def main() -> int:
         "config": {"workload": "1000 PINHOLE cameras, 500k points, %d observations (mean track 6), 1%% outliers, "
                                "Huber 2px, pose 0 fixed, principal points pinned, D=%d" % (n_obs, D),
                    "scene": args.scene, "max_iterations": 50},
-        "value_definition": "mixed-precision solve time to reach the FP64 plain solve's final cost",
+        "value_definition": "mixed-precision solve time to reach the FP64 plain solve's final cost with its "
+                            "RMS reprojection error within 1e-4 px of the FP64 solve's",
+        "rms_criterion_met": cut is not None,
+        "mixed_iterations_at_value": cut if cut is not None else reach,
         "fp64_time_to_converge_s": fp64_t,
         "mixed_time_to_stopping_rule_s": mixed_t,
         "mixed_speedup_vs_fp64": fp64_t / value,
@@ -449,6 +481,10 @@ def main() -> int:
         "rms_px_at_fp64_cost": None if q_reach is None else
             {"mixed": q_reach["rms_px"], "fp64": rms_fp64, "diff": abs(q_reach["rms_px"] - rms_fp64),
              "inlier_rms_mixed": q_reach["inlier_rms_px"], "inlier_rms_fp64": q_fp64["inlier_rms_px"]},
+        "rms_px_at_value": None if q_cut is None else
+            {"mixed": q_cut["rms_px"], "fp64": rms_fp64, "diff": abs(q_cut["rms_px"] - rms_fp64),
+             "inlier_rms_mixed": q_cut["inlier_rms_px"], "inlier_rms_fp64": q_fp64["inlier_rms_px"]},
+        "rms_cut_search": cut_search,
         "pose_error": pe,
         "final_cost": {"mixed": mixed_outs[-1].final_cost, "fp64": fp64_outs[-1].final_cost},
         "mixed": r_mixed,
