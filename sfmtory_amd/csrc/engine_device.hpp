@@ -616,7 +616,9 @@ __device__ inline void schur_chunk_mfma(const View<T>& v, const uint4 ch, double
 // form, at the same registers.  The quadrants are added at the end (even +
 // odd); the FP32 instantiation flushes after every U steps, so no more than
 // U terms are summed in FP32 per quadrant.
-template <typename T, int U>
+// LONG: chunks of any length (the window solver's one chunk per
+// destination), taken 128 pairs at a time; otherwise at most PAIR_CHUNK.
+template <typename T, int U, bool LONG = false>
 __device__ inline void schur_chunk_mfma8(const View<T>& v, const uint4 ch, double lambda, double* parts) {
   using M = Mfma<T>;
   using acc_t = typename M::acc_t;
@@ -632,70 +634,74 @@ __device__ inline void schur_chunk_mfma8(const View<T>& v, const uint4 ch, doubl
   const T* F = reinterpret_cast<const T*>(v.F4);
   const T* E = reinterpret_cast<const T*>(v.E4);
   const T* J = reinterpret_cast<const T*>(v.J4);
-  const uint32_t n = ch.z - ch.y;
-  const uint2 w0 = lane < n ? v.pairs[ch.y + lane] : make_uint2(0u, 0u);
-  const uint2 w1 = lane + 64 < n ? v.pairs[ch.y + 64 + lane] : make_uint2(0u, 0u);
-  // Pair q's words (zero for q >= n).
-  auto word = [&](uint32_t q, uint32_t& x, uint32_t& y) {
-    x = 0u;
-    y = 0u;
-    if (q < n) {
-      const uint32_t sl = q & 63;
-      x = __builtin_amdgcn_readlane(q < 64 ? w0.x : w1.x, sl);
-      y = __builtin_amdgcn_readlane(q < 64 ? w0.y : w1.y, sl);
-    }
-  };
-  struct Stage {
-    T fa[U], eb[U], ja[U], jb[U];
-  };
-  // Operands of U steps (2U pairs) starting at pair q.
-  auto load = [&](uint32_t q, Stage& st) {
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      uint32_t x0, y0, x1, y1;
-      word(q + 2 * u, x0, y0);
-      word(q + 2 * u + 1, x1, y1);
-      const bool ok = q + 2 * u + h < n;
-      const uint32_t px = h ? x1 : x0, py = h ? y1 : y0;
-      const uint32_t ra = px & OFF, rb = py & OFF;
-      st.fa[u] = (ok && fa_on) ? F[4 * (size_t)(ra + i) + k] : T(0);
-      st.eb[u] = (ok && eb_on) ? E[4 * (size_t)(rb + i) + k] : T(0);
-      const bool same = ok && (px & HI);
-      st.ja[u] = (same && ja_on) ? J[4 * (size_t)(ra + i) + 2 + k] : T(0);
-      st.jb[u] = (same && jb_on) ? J[4 * (size_t)(rb + i) + k] : T(0);
-    }
-  };
+  const uint32_t n_all = ch.z - ch.y;
   acc_t s = {T(0), T(0), T(0), T(0)}, b = {T(0), T(0), T(0), T(0)};
   double ss[4] = {0.0, 0.0, 0.0, 0.0}, bs[4] = {0.0, 0.0, 0.0, 0.0};
   Acc<T> accR;
   const bool rlane = diag && lane < da;
-  Stage cur, nxt;
-  load(0, cur);
-  for (uint32_t q = 0; q < n; q += 2 * U) {
-    if (q + 2 * U < n) load(q + 2 * U, nxt);
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      if (q + 2 * u >= n) break;
-      uint32_t x0, y0, x1, y1;
-      word(q + 2 * u, x0, y0);
-      word(q + 2 * u + 1, x1, y1);
-      s = M::step(cur.fa[u], cur.eb[u], s);
-      if ((x0 | x1) & HI) b = M::step(cur.ja[u], cur.jb[u], b);
-      // right-hand side in pair order
-      if (rlane && (y0 & HI)) accR.add(v.F4[(x0 & OFF) + lane].w - v.E4[(y0 & OFF) + lane].w);
-      if (rlane && (y1 & HI)) accR.add(v.F4[(x1 & OFF) + lane].w - v.E4[(y1 & OFF) + lane].w);
-    }
-    if (sizeof(T) == 4) {
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        ss[c] += (double)s[c];
-        bs[c] += (double)b[c];
-        s[c] = T(0);
-        b[c] = T(0);
+  for (uint32_t base = 0; base < (LONG ? n_all : 1u); base += 128) {
+    const uint32_t n = LONG ? min(128u, n_all - base) : n_all;
+    const uint32_t p0 = ch.y + base;
+    const uint2 w0 = lane < n ? v.pairs[p0 + lane] : make_uint2(0u, 0u);
+    const uint2 w1 = lane + 64 < n ? v.pairs[p0 + 64 + lane] : make_uint2(0u, 0u);
+    // Pair q's words (zero for q >= n).
+    auto word = [&](uint32_t q, uint32_t& x, uint32_t& y) {
+      x = 0u;
+      y = 0u;
+      if (q < n) {
+        const uint32_t sl = q & 63;
+        x = __builtin_amdgcn_readlane(q < 64 ? w0.x : w1.x, sl);
+        y = __builtin_amdgcn_readlane(q < 64 ? w0.y : w1.y, sl);
       }
+    };
+    struct Stage {
+      T fa[U], eb[U], ja[U], jb[U];
+    };
+    // Operands of U steps (2U pairs) starting at pair q.
+    auto load = [&](uint32_t q, Stage& st) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        uint32_t x0, y0, x1, y1;
+        word(q + 2 * u, x0, y0);
+        word(q + 2 * u + 1, x1, y1);
+        const bool ok = q + 2 * u + h < n;
+        const uint32_t px = h ? x1 : x0, py = h ? y1 : y0;
+        const uint32_t ra = px & OFF, rb = py & OFF;
+        st.fa[u] = (ok && fa_on) ? F[4 * (size_t)(ra + i) + k] : T(0);
+        st.eb[u] = (ok && eb_on) ? E[4 * (size_t)(rb + i) + k] : T(0);
+        const bool same = ok && (px & HI);
+        st.ja[u] = (same && ja_on) ? J[4 * (size_t)(ra + i) + 2 + k] : T(0);
+        st.jb[u] = (same && jb_on) ? J[4 * (size_t)(rb + i) + k] : T(0);
+      }
+    };
+    Stage cur, nxt;
+    load(0, cur);
+    for (uint32_t q = 0; q < n; q += 2 * U) {
+      if (q + 2 * U < n) load(q + 2 * U, nxt);
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (q + 2 * u >= n) break;
+        uint32_t x0, y0, x1, y1;
+        word(q + 2 * u, x0, y0);
+        word(q + 2 * u + 1, x1, y1);
+        s = M::step(cur.fa[u], cur.eb[u], s);
+        if ((x0 | x1) & HI) b = M::step(cur.ja[u], cur.jb[u], b);
+        // right-hand side in pair order
+        if (rlane && (y0 & HI)) accR.add(v.F4[(x0 & OFF) + lane].w - v.E4[(y0 & OFF) + lane].w);
+        if (rlane && (y1 & HI)) accR.add(v.F4[(x1 & OFF) + lane].w - v.E4[(y1 & OFF) + lane].w);
+      }
+      if (sizeof(T) == 4) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          ss[c] += (double)s[c];
+          bs[c] += (double)b[c];
+          s[c] = T(0);
+          b[c] = T(0);
+        }
+      }
+      cur = nxt;
     }
-    cur = nxt;
-  }
+  }  // 128-pair blocks
   if (sizeof(T) == 8) {
 #pragma unroll
     for (int c = 0; c < 4; ++c) {

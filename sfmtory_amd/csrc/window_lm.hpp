@@ -305,8 +305,15 @@ __device__ inline void wg_assemble(const View<T>& v, double lam) {
   __syncthreads();
   for (uint32_t a = threadIdx.x; a < v.n_inc; a += WG) compute_F(v, a);
   __syncthreads();
-  for (uint32_t ci = threadIdx.x >> 6; ci < v.n_chunks; ci += NWAVE)
-    schur_chunk<T, NQ, NP>(v, v.chunks[ci], lam, nullptr);
+  // Blocks of at most 8x8 (NQ == 1: at most 64 entries): the matrix-core
+  // form with two pairs per MFMA (chunks of any length); otherwise the
+  // vector form with NP pairs in flight per wavefront.
+  for (uint32_t ci = threadIdx.x >> 6; ci < v.n_chunks; ci += NWAVE) {
+    if constexpr (NQ == 1)
+      schur_chunk_mfma8<T, 2, true>(v, v.chunks[ci], lam, nullptr);
+    else
+      schur_chunk<T, NQ, NP>(v, v.chunks[ci], lam, nullptr);
+  }
   __syncthreads();
 }
 
