@@ -3,6 +3,8 @@ window) against the oracle: per window, the FP64 per-iteration costs within
 1e-9 relative and the same iteration count as the reference algorithm;
 camera models, losses, fixed cameras, the LU fallback inside the kernel, and
 problems too large for one workgroup routed to the single-problem path."""
+from collections import Counter
+
 import numpy as np
 import pytest
 
@@ -85,6 +87,27 @@ def test_mixed_windows_close_to_fp64():
         ra = np.sqrt(2 * a.final_cost / len(w.inp.observations))
         rb = np.sqrt(2 * b.final_cost / len(w.inp.observations))
         assert abs(ra - rb) < 1e-4
+
+
+def test_long_destination_chunks():
+    # Four images and 700 points: every image pair shares 400-560 points, so
+    # the window solver's one chunk per destination block spans several
+    # 128-pair blocks of the matrix-core assembly (blocks of 8 unknowns).
+    wins = [scenes.generate("c3w", seed=4200 + k, num_images=4, num_points=700) for k in range(2)]
+    for w in wins:
+        obs = w.inp.observations
+        seen = {}
+        for img, pt in zip(obs["image_idx"], obs["point_idx"]):
+            seen.setdefault(int(pt), set()).add(int(img))
+        shared = Counter((a, b) for s in seen.values() for a in s for b in s if a > b)
+        assert min(shared.values()) > 256
+    p = ba.BaParams(max_iterations=10)
+    f64 = ba.bundle_adjust_batch([w.inp for w in wins], p)
+    for w, o in zip(wins, f64):
+        check(w.inp, o, p)
+    mix = ba.bundle_adjust_batch([w.inp for w in wins], p, precision="mixed")
+    for a, b in zip(f64, mix):
+        assert b.final_cost == pytest.approx(a.final_cost, rel=1e-5)
 
 
 def test_large_problem_in_a_batch_takes_the_single_problem_path():
