@@ -8,7 +8,8 @@
 //                      and the per-observation terms of hot loop B, 880-909
 //   point_accumulate   c_diag / c_rhs sums, 885-886; the points_to_obs test, 960
 //   point_prepare      damped point-block inverse, 958-965
-//   compute_F / schur_chunk
+//   compute_F / schur_chunk (schur_chunk_mfma, schur_chunk_mfma8: the same
+//   reduction on the matrix cores)
 //                      damped diagonal and cross blocks, 923-956, and the
 //                      point elimination S -= E C^-1 E^T, rhs += E C^-1 g of
 //                      accumulate_point_schur, 112-211 / 967-1043
