@@ -53,9 +53,11 @@ __global__ __launch_bounds__(256) void k_compute_F(View<T> v) {
 // per step):
 //   FP32  two steps at seven waves per SIMD (68 VGPRs): 1.72 ms per
 //         assembly; eight waves spill (2.00 ms), one step at eight waves
-//         1.80 ms, four steps at four waves 1.88 ms;
+//         1.80 ms, three steps at six waves 1.78 ms, four steps at four
+//         waves 1.88 ms;
 //   FP64  two steps at six waves (77 VGPRs): 2.10 ms; the compiler's choice
-//         (five waves) 2.31 ms, four steps 2.45 ms.
+//         (five waves) 2.31 ms, three steps at five waves 2.22 ms, four
+//         steps 2.45 ms.
 // The one-pair form took 2.02 ms (FP32, eight waves) and 3.07 ms (FP64,
 // five waves) on the same scene; it keeps those settings for blocks above
 // 8x8.  The gathers' latency, not bandwidth or issue, bounds this kernel,
