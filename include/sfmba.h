@@ -141,9 +141,13 @@ int sfmba_bundle_adjust(const sfmba_input* input, const sfmba_params* params,
 /* Engine options (no counterpart in the reference: the precision and solver
  * choices of this engine). */
 #define SFMBA_PRECISION_FP64 0  /* everything in FP64 (parity mode) */
-#define SFMBA_PRECISION_MIXED 1 /* FP32 residual/Jacobian/Schur assembly,
-                                   FP64 point inverses, FP64 Cholesky and
-                                   FP64 iterative refinement of the step */
+#define SFMBA_PRECISION_MIXED 1 /* FP32 Jacobian and Schur assembly while
+                                   the Marquardt damping is >= 1e-7 (the
+                                   resolution of an FP32-assembled system),
+                                   FP64 assembly from then on; FP64 point
+                                   inverses, residuals, cost and Cholesky
+                                   throughout; optional FP64 iterative
+                                   refinement of the step */
 #define SFMBA_SOLVER_CHOLESKY 0 /* dense FP64 MFMA Cholesky (LU fallback) */
 #define SFMBA_SOLVER_PCG 1      /* block-Jacobi PCG on the reduced system */
 
@@ -152,7 +156,8 @@ typedef struct sfmba_options {
   int32_t solver;     /* SFMBA_SOLVER_* */
   int32_t device;     /* HIP device ordinal */
   int32_t profile;    /* 1: time each phase with HIP events into stats */
-  int32_t refine_steps; /* mixed precision: FP64 iterative-refinement sweeps of the step (default 0) */
+  int32_t refine_steps; /* mixed precision, FP32-assembled tries: FP64 iterative-refinement
+                           sweeps of the step against the assembled system (default 0) */
   int32_t pcg_max_iterations; /* PCG: iteration cap (default 500) */
   double pcg_tolerance;       /* PCG: relative residual target (default 1e-10) */
 } sfmba_options;
@@ -193,7 +198,23 @@ typedef struct sfmba_stats {
   uint64_t count_trailing_update;
   double flop_trailing_update;
   uint64_t num_pcg_iterations; /* solver = PCG: iterations over all tries */
+  /* Every damping try in order (lib.rs:1076-1221): the Marquardt damping
+   * its reduced system was built with, and its outcome (SFMBA_TRY_* flags).
+   * Both arrays hold tries_capacity entries (either may be NULL). */
+  double* try_lambdas;
+  uint8_t* try_outcomes;
+  uint64_t tries_capacity;
+  uint64_t tries_len;
+  uint64_t num_fp32_tries;   /* mixed: tries whose reduced system was assembled in FP32 */
 } sfmba_stats;
+
+/* Outcome of one damping try (sfmba_stats.try_outcomes): the low two bits
+ * are the LM decision, the others how the step was formed. */
+#define SFMBA_TRY_ACCEPTED 0   /* finite trial cost below the current cost */
+#define SFMBA_TRY_REJECTED 1   /* trial cost not lower (or not finite) */
+#define SFMBA_TRY_NO_STEP 2    /* neither Cholesky nor LU gave a step */
+#define SFMBA_TRY_LU 4         /* the step came from the LU fallback */
+#define SFMBA_TRY_FP32 8       /* mixed: the reduced system was assembled in FP32 */
 
 /* bundle_adjust with engine options and diagnostics (stats may be NULL). */
 int sfmba_bundle_adjust_ex(const sfmba_input* input, const sfmba_params* params,

@@ -94,7 +94,16 @@ class Stats(C.Structure):
         ("count_factor", C.c_uint64), ("time_trailing_update_s", C.c_double),
         ("count_trailing_update", C.c_uint64), ("flop_trailing_update", C.c_double),
         ("num_pcg_iterations", C.c_uint64),
+        ("try_lambdas", C.c_void_p), ("try_outcomes", C.c_void_p), ("tries_capacity", C.c_uint64),
+        ("tries_len", C.c_uint64), ("num_fp32_tries", C.c_uint64),
     ]
+
+
+TRY_ACCEPTED = 0
+TRY_REJECTED = 1
+TRY_NO_STEP = 2
+TRY_LU = 4
+TRY_FP32 = 8
 
 
 class Policy(C.Structure):

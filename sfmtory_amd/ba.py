@@ -271,19 +271,32 @@ class _Stats:
     def __init__(self, cap: int = 4096):
         self.outer = np.zeros(cap)
         self.trial = np.zeros(cap)
+        self.lambdas = np.zeros(cap)
+        self.outcomes = np.zeros(cap, dtype=np.uint8)
         s = N.Stats()
         s.outer_costs = self.outer.ctypes.data
         s.outer_costs_capacity = cap
         s.trial_costs = self.trial.ctypes.data
         s.trial_costs_capacity = cap
+        s.try_lambdas = self.lambdas.ctypes.data
+        s.try_outcomes = self.outcomes.ctypes.data
+        s.tries_capacity = cap
         self.c = s
 
+    _ARRAYS = ("outer_costs", "trial_costs", "outer_costs_capacity", "trial_costs_capacity", "try_lambdas",
+               "try_outcomes", "tries_capacity")
+
     def to_dict(self) -> dict:
+        """Counters and times, plus the traces: ``outer_costs``,
+        ``trial_costs`` and, per damping try, ``try_lambdas`` and
+        ``try_outcomes`` (SFMBA_TRY_* flags)."""
         s = self.c
-        d = {name: getattr(s, name) for name, _ in N.Stats._fields_
-             if name not in ("outer_costs", "trial_costs", "outer_costs_capacity", "trial_costs_capacity")}
+        d = {name: getattr(s, name) for name, _ in N.Stats._fields_ if name not in self._ARRAYS}
         d["outer_costs"] = self.outer[: min(s.outer_costs_len, len(self.outer))].copy()
         d["trial_costs"] = self.trial[: min(s.trial_costs_len, len(self.trial))].copy()
+        n = min(s.tries_len, len(self.lambdas))
+        d["try_lambdas"] = self.lambdas[:n].copy()
+        d["try_outcomes"] = self.outcomes[:n].copy()
         return d
 
 

@@ -38,6 +38,15 @@ constexpr size_t TOP_BLOCK = 36 * 4 * 64;
 inline size_t dense_workspace_doubles(uint32_t Dp) {
   return (size_t)(Dp / NB) * (LINV_BLOCK + TOP_BLOCK);
 }
+// Mixed form: the smallest Marquardt damping at which a reduced system is
+// assembled in FP32.  An FP32-assembled system is resolved only to about
+// 1e-7 of its diagonal, so below that damping its step no longer follows
+// the FP64 step, and rounding alone can make it indefinite (on the
+// 1,000-camera scene the first FP32 factorization fails at lambda = 5e-8).
+// The single-problem solver continues in FP64 from there; the window
+// solver, whose precision is fixed per launch, damps FP32 systems at least
+// this much (the LM schedule itself is unchanged).
+constexpr double MIXED_MIN_LAMBDA = 1e-7;
 constexpr int PAIR_CHUNK = 128;   // max point pairs reduced by one wavefront
 constexpr uint64_t SCHUR_TILE = 64; // destination tile edge (parameter blocks) of the chunk order
 

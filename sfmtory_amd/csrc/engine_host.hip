@@ -465,6 +465,31 @@ static void push_stat(double* arr, uint64_t cap, uint64_t* len, double v) {
   ++*len;
 }
 
+// Zero a caller's stats, keeping its array pointers and capacities.
+static void reset_stats(sfmba_stats* st) {
+  sfmba_stats z{};
+  z.outer_costs = st->outer_costs;
+  z.outer_costs_capacity = st->outer_costs_capacity;
+  z.trial_costs = st->trial_costs;
+  z.trial_costs_capacity = st->trial_costs_capacity;
+  z.try_lambdas = st->try_lambdas;
+  z.try_outcomes = st->try_outcomes;
+  z.tries_capacity = st->tries_capacity;
+  *st = z;
+}
+
+// One damping try's damping and outcome (SFMBA_TRY_*) into the trace.
+static void push_try(sfmba_stats* st, double lambda, int outcome) {
+  if (!st) return;
+  if (st->tries_len < st->tries_capacity) {
+    if (st->try_lambdas) st->try_lambdas[st->tries_len] = lambda;
+    if (st->try_outcomes) st->try_outcomes[st->tries_len] = (uint8_t)outcome;
+  }
+  ++st->tries_len;
+  if (outcome & SFMBA_TRY_FP32) ++st->num_fp32_tries;
+}
+
+
 }  // namespace sfmba
 
 // A problem resident on one device: packed structure, initial state and the
@@ -558,8 +583,11 @@ static void alloc_pcg(sfmba_problem& H) {
   H.have_pcg = true;
 }
 
+// Work buffers of one precision.  The reduced system S and the factor
+// workspace are FP64 in both precisions and never used by both at once, so
+// a second precision's work shares them (shared_S / shared_Linv).
 template <typename T>
-static void alloc_work(sfmba_problem& H, Work<T>& W) {
+static void alloc_work(sfmba_problem& H, Work<T>& W, double* shared_S = nullptr, double* shared_Linv = nullptr) {
   const Layout& L = H.pr.L;
   DevArena& A = H.pr.arena;
   hipStream_t s = H.s;
@@ -586,13 +614,19 @@ static void alloc_work(sfmba_problem& H, Work<T>& W) {
   W.pt_cinv = A.alloc<double>(9 * (size_t)L.n_pt);
   W.pt_ok = A.alloc<uint8_t>(L.n_pt);
   const size_t s_elems = (size_t)L.ld * L.Dp;
-  W.S = A.alloc<double>(s_elems);
-  W.Linv = A.alloc<double>(dense_workspace_doubles(L.Dp));
   W.delta = A.alloc<double>(L.Dp);
   W.vec = A.alloc<double>(L.Dp);
   W.rhs = A.alloc<double>(L.Dp);
   W.parts = A.alloc<double>((size_t)std::max<uint32_t>(L.n_parts, 1) * (2 * 196 + 16));
   W.inc_part = A.alloc<double>(3 * (size_t)std::max<uint32_t>(L.n_inc, 1));
+  if (shared_S) {
+    W.S = shared_S;
+    W.Linv = shared_Linv;
+    HIP_CHECK(hipStreamSynchronize(s));
+    return;
+  }
+  W.S = A.alloc<double>(s_elems);
+  W.Linv = A.alloc<double>(dense_workspace_doubles(L.Dp));
   // S: zero once, identity on the padding diagonal; the kernels rewrite the
   // D x D lower triangle and the rhs row on every try and never touch the
   // padding.
@@ -603,22 +637,168 @@ static void alloc_work(sfmba_problem& H, Work<T>& W) {
   HIP_CHECK(hipStreamSynchronize(s));
 }
 
+// The stages of an LM solve in one precision T (the precision of the
+// residual Jacobians and the Schur assembly; the point inverses, the
+// factorization and the cost are FP64 in both).  The current state lives in
+// W.poses/cams/pts[cur]; a trial step writes the other buffer.
 template <typename T>
-static void solve(sfmba_problem& H, Work<T>& W, const sfmba_params* prm, const sfmba_options* opt,
-                  sfmba_output* out, sfmba_stats* st) {
+struct Stages {
+  sfmba_problem& H;
+  Work<T>& W;
+  const sfmba_options* opt;
+  sfmba_stats* st;
+  Profiler& prof;
+  int loss;
+  double scale;
+  bool refine, pcg;
+
+  Stages(sfmba_problem& h, Work<T>& w, const sfmba_params* prm, const sfmba_options* o, sfmba_stats* s_,
+         Profiler& p)
+      : H(h), W(w), opt(o), st(s_), prof(p), loss(prm->robust_loss), scale(prm->loss_scale_px) {
+    const Layout& L = H.pr.L;
+    refine = sizeof(T) == 4 && opt->refine_steps > 0 && L.D;
+    if (refine && !W.S_copy) W.S_copy = H.pr.arena.alloc<double>((size_t)L.ld * L.Dp);
+    pcg = opt->solver == SFMBA_SOLVER_PCG;
+  }
+  // State buffer 0 <- the given device arrays.
+  void load(const double* poses, const double* cams, const double* pts) {
+    const Layout& L = H.pr.L;
+    HIP_CHECK(hipMemcpyAsync(W.poses[0], poses, 7 * (size_t)L.n_img * sizeof(double), hipMemcpyDeviceToDevice, H.s));
+    HIP_CHECK(hipMemcpyAsync(W.cams[0], cams, 8 * (size_t)L.n_cam * sizeof(double), hipMemcpyDeviceToDevice, H.s));
+    HIP_CHECK(hipMemcpyAsync(W.pts[0], pts, 3 * (size_t)L.n_pt * sizeof(double), hipMemcpyDeviceToDevice, H.s));
+  }
+  // total_cost and the in-front count of state buffer b (synchronizes).
+  void cost(int b, double& c, double& front) {
+    const Layout& L = H.pr.L;
+    launch_cost(L, W.poses[b], W.cams[b], W.pts[b], loss, scale, W.cost_part, W.cost, 0, H.s);
+    HIP_CHECK(hipMemcpyAsync(H.pinned, W.cost, 2 * sizeof(double), hipMemcpyDeviceToHost, H.s));
+    HIP_CHECK(hipStreamSynchronize(H.s));
+    c = H.pinned[0];
+    front = H.pinned[1];
+  }
+  // Hot loops A and B (lib.rs:816-909) at state cur.
+  void linearize(int cur) {
+    prof.begin(0);
+    launch_linearize<T>(H.pr.L, W, cur, loss, scale, H.s);
+    launch_point_accumulate<T>(H.pr.L, W, H.s);
+    prof.end();
+    if (st) st->num_linearizations++;
+  }
+  // Trial state from W.delta and its cost; one host synchronization.
+  void step_and_cost(int cur, double& trial, double& front) {
+    const Layout& L = H.pr.L;
+    double* h = H.pinned;
+    prof.begin(4);
+    launch_backsub_update<T>(L, W, cur, H.s);
+    launch_cost(L, W.poses[1 - cur], W.cams[1 - cur], W.pts[1 - cur], loss, scale, W.cost_part, W.cost, 0, H.s);
+    prof.end();
+    HIP_CHECK(hipMemcpyAsync(h, W.cost, 2 * sizeof(double), hipMemcpyDeviceToHost, H.s));
+    HIP_CHECK(hipMemcpyAsync(h + 4, W.flags, 4 * sizeof(int), hipMemcpyDeviceToHost, H.s));
+    HIP_CHECK(hipStreamSynchronize(H.s));
+    trial = h[0];
+    front = h[1];
+    if (reinterpret_cast<const int*>(h + 4)[2] != 0)
+      throw Error(SFMBA_ERR_DEVICE, "back substitution hand-over timed out");
+  }
+  // Reduced system at damping lam and its factorization.  Returns false when
+  // the Cholesky failed (flags[0], nalgebra's test) or PCG broke down.
+  bool assemble_factor(double lam) {
+    const Layout& L = H.pr.L;
+    hipStream_t s = H.s;
+    prof.begin(1);
+    launch_point_prepare<T>(L, W, lam, s);
+    launch_schur<T>(L, W, lam, s);
+    prof.end();
+    HIP_CHECK(hipMemsetAsync(W.flags, 0, 4 * sizeof(int), s));
+    if (pcg) {
+      // Iterative solve of the damped reduced system (no factorization).
+      prof.begin(3);
+      const PcgResult pr = pcg_solve(H.pcg, W.S, L.ld, L.D, L.Dp, L.n_grp, L.grp_off, L.grp_dim,
+                                     opt->pcg_max_iterations, opt->pcg_tolerance, W.delta, W.flags, H.pcg.host, s);
+      prof.end();
+      if (st) st->num_pcg_iterations += pr.iterations;
+      return pr.ok;
+    }
+    if (refine) {
+      dense_mirror_copy(W.S, W.S_copy, L.Dp, L.ld, s);
+      dense_extract_row(W.S, L.Dp, L.ld, L.Dp, W.rhs, s);
+    }
+    prof.begin(2);
+    dense_cholesky(W.S, W.Linv, L.Dp, L.ld, L.D, W.flags, s, prof.on ? &prof.trailing : nullptr);
+    prof.end();
+    HIP_CHECK(hipMemcpyAsync(H.pinned + 4, W.flags, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    return reinterpret_cast<const int*>(H.pinned + 4)[0] == 0;
+  }
+  // The step from the factor (L^T delta = L^-1 rhs), with optional FP64
+  // iterative refinement against the assembled system.
+  void solve_factored() {
+    const Layout& L = H.pr.L;
+    hipStream_t s = H.s;
+    prof.begin(3);
+    dense_extract_row(W.S, L.Dp, L.ld, L.Dp, W.vec, s);
+    dense_backsolve(W.S, W.Linv, L.Dp, L.ld, W.vec, W.delta, W.chain, next_epoch(W.chain_epoch), W.flags + 2, s);
+    for (int it = 0; refine && it < opt->refine_steps; ++it) {
+      // r = rhs - S delta, delta += S^-1 r with the same factor
+      dense_symv_residual(W.S_copy, L.Dp, L.ld, W.delta, W.rhs, W.vec, s);
+      dense_forwardsolve(W.S, W.Linv, L.Dp, L.ld, W.vec, H.vec2, s);
+      dense_backsolve(W.S, W.Linv, L.Dp, L.ld, H.vec2, H.vec3, W.chain, next_epoch(W.chain_epoch), W.flags + 2, s);
+      dense_vec_add(W.delta, H.vec3, L.Dp, s);
+    }
+    prof.end();
+  }
+  // One damping try (lib.rs:1076-1176): Cholesky, then (FP64 only) LU on
+  // the rebuilt system.  Returns false when no step was formed; bits
+  // receives SFMBA_TRY_LU when the LU fallback gave the step.
+  bool try_step(int cur, double lam, double& trial, double& front, int& bits) {
+    const Layout& L = H.pr.L;
+    if (assemble_factor(lam)) {
+      if (!pcg) solve_factored();
+      step_and_cost(cur, trial, front);
+      return true;
+    }
+    if (st) st->num_cholesky_failures++;
+    if (pcg || sizeof(T) == 4) return false;
+    // Not positive definite: rebuild S and solve by LU (lib.rs:1083-1093).
+    bits |= SFMBA_TRY_LU;
+    launch_point_prepare<T>(L, W, lam, H.s);
+    launch_schur<T>(L, W, lam, H.s);
+    dense_lu_solve(W.S, L.Dp, L.ld, L.D, W.delta, H.piv, W.flags, H.s);
+    HIP_CHECK(hipMemcpyAsync(H.pinned + 4, W.flags, 4 * sizeof(int), hipMemcpyDeviceToHost, H.s));
+    HIP_CHECK(hipStreamSynchronize(H.s));
+    if (st) st->num_lu_solves++;
+    if (reinterpret_cast<const int*>(H.pinned + 4)[1] != 0) return false;
+    step_and_cost(cur, trial, front);
+    return true;
+  }
+};
+
+
+// Bundle adjustment on a resident problem (lib.rs:747-1232).  FP64: every
+// stage in FP64, the reference's arithmetic.  Mixed: Jacobians and the
+// Schur assembly in FP32 while the Marquardt damping is at least
+// MIXED_MIN_LAMBDA; once it falls below, or an FP32-assembled system fails
+// to factor, the solve continues from its current state with FP64
+// assembly.  The LM schedule (lambda, nu, accept/reject, stopping rule) is
+// the same in both forms.
+static void solve(sfmba_problem& H, const sfmba_params* prm, const sfmba_options* opt, sfmba_output* out,
+                  sfmba_stats* st) {
   using clock = std::chrono::steady_clock;
   HIP_CHECK(hipSetDevice(H.device));
   const Layout& L = H.pr.L;
   hipStream_t s = H.s;
-  const bool mixed = sizeof(T) == 4;
-  if (mixed && opt->refine_steps > 0 && L.D && !W.S_copy)
-    W.S_copy = H.pr.arena.alloc<double>((size_t)L.ld * L.Dp);
-  const bool refine = mixed && opt->refine_steps > 0 && L.D;
-  const bool pcg = opt->solver == SFMBA_SOLVER_PCG;
-  if (pcg && L.D) alloc_pcg(H);
-  double* h = H.pinned;
-  const int loss = prm->robust_loss;
-  const double scale = prm->loss_scale_px;
+  const bool mixed = opt->precision == SFMBA_PRECISION_MIXED;
+  if (opt->solver == SFMBA_SOLVER_PCG && L.D) alloc_pcg(H);
+  // FP64 work always (the mixed form finishes in FP64); FP32 work for the
+  // mixed form, sharing S and the factor workspace.
+  if (!H.have64) {
+    alloc_work<double>(H, H.w64);
+    H.have64 = true;
+  }
+  if (mixed && !H.have32) {
+    alloc_work<float>(H, H.w32, H.w64.S, H.w64.Linv);
+    H.have32 = true;
+  }
   Profiler prof;
   prof.on = opt->profile != 0;
   prof.s = s;
@@ -627,156 +807,82 @@ static void solve(sfmba_problem& H, Work<T>& W, const sfmba_params* prm, const s
     st->num_schur_pairs = L.n_pairs;
     st->time_upload_s = H.pack_s + H.upload_s;
   }
+  Stages<double> s64(H, H.w64, prm, opt, st, prof);
+  Stages<float> s32(H, H.w32, prm, opt, st, prof);
+  bool fp32 = mixed;
   // Every solve starts from the problem's initial state.
-  HIP_CHECK(hipMemcpyAsync(W.poses[0], H.init_poses, 7 * (size_t)L.n_img * sizeof(double), hipMemcpyDeviceToDevice, s));
-  HIP_CHECK(hipMemcpyAsync(W.cams[0], H.init_cams, 8 * (size_t)L.n_cam * sizeof(double), hipMemcpyDeviceToDevice, s));
-  HIP_CHECK(hipMemcpyAsync(W.pts[0], H.init_pts, 3 * (size_t)L.n_pt * sizeof(double), hipMemcpyDeviceToDevice, s));
+  if (fp32) s32.load(H.init_poses, H.init_cams, H.init_pts);
+  else s64.load(H.init_poses, H.init_cams, H.init_pts);
   HIP_CHECK(hipStreamSynchronize(s));
   const auto t1 = clock::now();
 
   int cur = 0;
-  launch_cost(L, W.poses[0], W.cams[0], W.pts[0], loss, scale, W.cost_part, W.cost, 0, s);
-  HIP_CHECK(hipMemcpyAsync(h, W.cost, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
-  HIP_CHECK(hipStreamSynchronize(s));
-  const double initial_cost = h[0];
-  double n_front = h[1];
+  double initial_cost, n_front;
+  if (fp32) s32.cost(0, initial_cost, n_front);
+  else s64.cost(0, initial_cost, n_front);
   double cost = initial_cost;
   size_t iterations_run = 0;
+  // The mixed form's switch to FP64 assembly: the current state moves to
+  // the FP64 buffers and is linearized again there.
+  auto escalate = [&]() {
+    s64.load(H.w32.poses[cur], H.w32.cams[cur], H.w32.pts[cur]);
+    cur = 0;
+    fp32 = false;
+    s64.linearize(cur);
+  };
 
   if (L.D != 0) {
     double lambda = prm->initial_lambda;
     double nu = 2.0;
-    // One damping try: reduced system, factorization, step, trial state and
-    // its cost, with one host synchronization.  Returns false when neither
-    // Cholesky nor LU gives a step.
-    // One damping try: reduced system at damping lam, factorization, step,
-    // trial state and its cost, with one host synchronization.  Returns 0
-    // with the trial cost, 1 when the factorization failed (flags[0]).
-    auto assemble_factor_step = [&](double lam, double& trial) {
-      prof.begin(1);
-      launch_point_prepare<T>(L, W, lam, s);
-      launch_schur<T>(L, W, lam, s);
-      prof.end();
-      if (refine) {
-        dense_mirror_copy(W.S, W.S_copy, L.Dp, L.ld, s);
-        dense_extract_row(W.S, L.Dp, L.ld, L.Dp, W.rhs, s);
-      }
-      if (pcg) {
-        // Iterative solve of the damped reduced system (no factorization).
-        prof.begin(3);
-        const PcgResult pr = pcg_solve(H.pcg, W.S, L.ld, L.D, L.Dp, L.n_grp, L.grp_off, L.grp_dim,
-                                       opt->pcg_max_iterations, opt->pcg_tolerance, W.delta, W.flags,
-                                       H.pcg.host, s);
-        prof.end();
-        if (st) st->num_pcg_iterations += pr.iterations;
-        if (!pr.ok) return true;
-        prof.begin(4);
-        launch_backsub_update<T>(L, W, cur, s);
-        launch_cost(L, W.poses[1 - cur], W.cams[1 - cur], W.pts[1 - cur], loss, scale, W.cost_part, W.cost, 0, s);
-        prof.end();
-        HIP_CHECK(hipMemcpyAsync(h, W.cost, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
-        HIP_CHECK(hipStreamSynchronize(s));
-        trial = h[0];
-        return false;
-      }
-      prof.begin(2);
-      HIP_CHECK(hipMemsetAsync(W.flags, 0, 4 * sizeof(int), s));
-      dense_cholesky(W.S, W.Linv, L.Dp, L.ld, L.D, W.flags, s, prof.on ? &prof.trailing : nullptr);
-      prof.end();
-      prof.begin(3);
-      dense_extract_row(W.S, L.Dp, L.ld, L.Dp, W.vec, s);
-      dense_backsolve(W.S, W.Linv, L.Dp, L.ld, W.vec, W.delta, W.chain, next_epoch(W.chain_epoch), W.flags + 2, s);
-      for (int it = 0; refine && it < opt->refine_steps; ++it) {
-        // FP64 iterative refinement of the step: r = rhs - S delta,
-        // delta += S^{-1} r with the same factor.
-        dense_symv_residual(W.S_copy, L.Dp, L.ld, W.delta, W.rhs, W.vec, s);
-        dense_forwardsolve(W.S, W.Linv, L.Dp, L.ld, W.vec, H.vec2, s);
-        dense_backsolve(W.S, W.Linv, L.Dp, L.ld, H.vec2, H.vec3, W.chain, next_epoch(W.chain_epoch), W.flags + 2, s);
-        dense_vec_add(W.delta, H.vec3, L.Dp, s);
-      }
-      prof.end();
-      prof.begin(4);
-      launch_backsub_update<T>(L, W, cur, s);
-      launch_cost(L, W.poses[1 - cur], W.cams[1 - cur], W.pts[1 - cur], loss, scale, W.cost_part, W.cost, 0, s);
-      prof.end();
-      HIP_CHECK(hipMemcpyAsync(h, W.cost, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
-      HIP_CHECK(hipMemcpyAsync(h + 4, W.flags, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
-      HIP_CHECK(hipStreamSynchronize(s));
-      trial = h[0];
-      if (reinterpret_cast<const int*>(h + 4)[2] != 0)
-        throw Error(SFMBA_ERR_DEVICE, "back substitution hand-over timed out");
-      return reinterpret_cast<const int*>(h + 4)[0] != 0;
-    };
-    // Returns false when no step could be formed (the try counts as rejected).
-    // Mixed form only: FP32 assembly resolves the reduced system to ~1e-7 of
-    // its diagonal, so a tiny Marquardt damping can leave it indefinite by
-    // rounding alone (the free scale gauge leaves an eigenvalue of order
-    // lambda).  After a failed factorization the mixed form refactors with a
-    // growing damping floor, which it then keeps for the rest of the solve;
-    // the step stays an LM step for that damping and acceptance still uses
-    // the FP64 cost.  An LU solve of an indefinite FP32-assembled system
-    // would carry no information, so the mixed form does not fall back to it.
-    double mixed_floor = 0.0;
-    auto try_step = [&](double lam_lm, double& trial) {
-      const double lam = mixed ? std::max(lam_lm, mixed_floor) : lam_lm;
-      if (!assemble_factor_step(lam, trial)) return true;
-      if (st) st->num_cholesky_failures++;
-      if (pcg) return false;  // breakdown: the damping bump takes over
-      if (mixed) {
-        for (double shift : {1e-6, 1e-5, 1e-4, 1e-3, 1e-2}) {
-          if (shift <= lam) continue;
-          mixed_floor = shift;
-          if (!assemble_factor_step(shift, trial)) return true;
-          if (st) st->num_cholesky_failures++;
-        }
-        return false;
-      }
-      // Not positive definite: rebuild S and solve by LU (lib.rs:1083-1093).
-      launch_point_prepare<T>(L, W, lam, s);
-      launch_schur<T>(L, W, lam, s);
-      dense_lu_solve(W.S, L.Dp, L.ld, L.D, W.delta, H.piv, W.flags, s);
-      HIP_CHECK(hipMemcpyAsync(h + 4, W.flags, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
-      HIP_CHECK(hipStreamSynchronize(s));
-      if (st) st->num_lu_solves++;
-      if (reinterpret_cast<const int*>(h + 4)[1] != 0) return false;
-      launch_backsub_update<T>(L, W, cur, s);
-      launch_cost(L, W.poses[1 - cur], W.cams[1 - cur], W.pts[1 - cur], loss, scale, W.cost_part, W.cost, 0, s);
-      HIP_CHECK(hipMemcpyAsync(h, W.cost, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
-      HIP_CHECK(hipStreamSynchronize(s));
-      trial = h[0];
-      return true;
-    };
-
     const double FUNCTION_TOLERANCE = 1e-6;
     for (uint64_t outer = 0; outer < prm->max_iterations; ++outer) {
       ++iterations_run;
       const double cost_before = cost;
       if (n_front == 0.0) break;  // no observation in front of its camera (lib.rs:855-857)
-      prof.begin(0);
-      launch_linearize<T>(L, W, cur, loss, scale, s);
-      launch_point_accumulate<T>(L, W, s);
-      prof.end();
-      if (st) st->num_linearizations++;
+      if (fp32 && lambda < MIXED_MIN_LAMBDA) escalate();
+      else if (fp32) s32.linearize(cur);
+      else s64.linearize(cur);
       bool accepted = false;
       for (int inner = 0; inner < 8; ++inner) {
-        double trial = 0.0;
-        const bool ok = try_step(lambda, trial);
-        if (st) st->num_factorizations++;
+        if (fp32 && lambda < MIXED_MIN_LAMBDA) escalate();
+        double trial = 0.0, front = 0.0;
+        int bits = 0;
+        bool ok;
+        if (fp32) {
+          bits = SFMBA_TRY_FP32;
+          ok = s32.try_step(cur, lambda, trial, front, bits);
+          if (st) st->num_factorizations++;
+          if (!ok && opt->solver != SFMBA_SOLVER_PCG) {
+            // the FP32-assembled system did not factor: this damping again
+            // with FP64 assembly
+            push_try(st, lambda, SFMBA_TRY_NO_STEP | bits);
+            escalate();
+            bits = 0;
+            ok = s64.try_step(cur, lambda, trial, front, bits);
+            if (st) st->num_factorizations++;
+          }
+        } else {
+          ok = s64.try_step(cur, lambda, trial, front, bits);
+          if (st) st->num_factorizations++;
+        }
         if (!ok) {
+          push_try(st, lambda, SFMBA_TRY_NO_STEP | bits);
           lambda *= nu;
           nu *= 2.0;
           continue;
         }
         if (st) push_stat(st->trial_costs, st->trial_costs_capacity, &st->trial_costs_len, trial);
         if (std::isfinite(trial) && trial < cost) {
+          push_try(st, lambda, SFMBA_TRY_ACCEPTED | bits);
           cur = 1 - cur;
           cost = trial;
-          n_front = h[1];
+          n_front = front;
           lambda = std::max(lambda / 3.0, 1e-12);
           nu = 2.0;
           accepted = true;
           break;
         }
+        push_try(st, lambda, SFMBA_TRY_REJECTED | bits);
         if (st) st->num_rejected_steps++;
         lambda *= nu;
         nu *= 2.0;
@@ -792,11 +898,14 @@ static void solve(sfmba_problem& H, Work<T>& W, const sfmba_params* prm, const s
   prof.collect(st);
 
   // Results back to the caller's arrays.
+  const double* rp = fp32 ? H.w32.poses[cur] : H.w64.poses[cur];
+  const double* rc = fp32 ? H.w32.cams[cur] : H.w64.cams[cur];
+  const double* rx = fp32 ? H.w32.pts[cur] : H.w64.pts[cur];
   std::vector<double> hp(7 * (size_t)L.n_img), hc(8 * (size_t)L.n_cam);
-  HIP_CHECK(hipMemcpyAsync(hp.data(), W.poses[cur], hp.size() * sizeof(double), hipMemcpyDeviceToHost, s));
-  HIP_CHECK(hipMemcpyAsync(hc.data(), W.cams[cur], hc.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipMemcpyAsync(hp.data(), rp, hp.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipMemcpyAsync(hc.data(), rc, hc.size() * sizeof(double), hipMemcpyDeviceToHost, s));
   if (L.n_pt)
-    HIP_CHECK(hipMemcpyAsync(out->points, W.pts[cur], 3 * (size_t)L.n_pt * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(out->points, rx, 3 * (size_t)L.n_pt * sizeof(double), hipMemcpyDeviceToHost, s));
   HIP_CHECK(hipStreamSynchronize(s));
   for (uint32_t i = 0; i < L.n_img; ++i) {
     sfmba_pose& q = out->poses[i];
@@ -823,34 +932,14 @@ static void solve_problem(sfmba_problem* H, const sfmba_params* params, const sf
   sfmba_options o;
   sfmba_options_default(&o);
   if (options) o = *options;
-  if (stats) {
-    // keep the caller's array pointers and capacities, reset the rest
-    sfmba_stats z{};
-    z.outer_costs = stats->outer_costs;
-    z.outer_costs_capacity = stats->outer_costs_capacity;
-    z.trial_costs = stats->trial_costs;
-    z.trial_costs_capacity = stats->trial_costs_capacity;
-    *stats = z;
-  }
+  if (stats) reset_stats(stats);  // keeps the caller's array pointers and capacities
   if (o.solver != SFMBA_SOLVER_CHOLESKY && o.solver != SFMBA_SOLVER_PCG)
     throw Error(SFMBA_ERR_INVALID_ARGUMENT, "unknown solver");
   if (o.solver == SFMBA_SOLVER_PCG && !(o.pcg_max_iterations > 0 && o.pcg_tolerance >= 0.0))
     throw Error(SFMBA_ERR_INVALID_ARGUMENT, "PCG needs pcg_max_iterations > 0 and pcg_tolerance >= 0");
-  if (o.precision == SFMBA_PRECISION_MIXED) {
-    if (!H->have32) {
-      alloc_work<float>(*H, H->w32);
-      H->have32 = true;
-    }
-    solve<float>(*H, H->w32, params, &o, out, stats);
-  } else if (o.precision == SFMBA_PRECISION_FP64) {
-    if (!H->have64) {
-      alloc_work<double>(*H, H->w64);
-      H->have64 = true;
-    }
-    solve<double>(*H, H->w64, params, &o, out, stats);
-  } else {
+  if (o.precision != SFMBA_PRECISION_MIXED && o.precision != SFMBA_PRECISION_FP64)
     throw Error(SFMBA_ERR_INVALID_ARGUMENT, "unknown precision");
-  }
+  solve(*H, params, &o, out, stats);
 }
 
 // Layout without pair lists for the cost / Jacobian entry points.
@@ -2038,12 +2127,7 @@ int sfmba_bundle_adjust_batch(const sfmba_input* inputs, size_t count, const sfm
           (inputs[i].num_cameras && !out.cameras))
         throw Error(SFMBA_ERR_INVALID_ARGUMENT, "problem " + std::to_string(i) + ": output arrays are NULL");
       if (stats) {
-        sfmba_stats z{};
-        z.outer_costs = stats[i].outer_costs;
-        z.outer_costs_capacity = stats[i].outer_costs_capacity;
-        z.trial_costs = stats[i].trial_costs;
-        z.trial_costs_capacity = stats[i].trial_costs_capacity;
-        stats[i] = z;
+        reset_stats(&stats[i]);
       }
     }
     if (o.precision == SFMBA_PRECISION_MIXED)
@@ -2096,12 +2180,7 @@ int sfmba_batch_solve(sfmba_batch* batch, sfmba_output* outputs, sfmba_stats* st
       if ((P.n_img && !out.poses) || (P.n_pt && !out.points) || (P.n_cam && !out.cameras))
         throw Error(SFMBA_ERR_INVALID_ARGUMENT, "problem " + std::to_string(i) + ": output arrays are NULL");
       if (stats) {
-        sfmba_stats z{};
-        z.outer_costs = stats[i].outer_costs;
-        z.outer_costs_capacity = stats[i].outer_costs_capacity;
-        z.trial_costs = stats[i].trial_costs;
-        z.trial_costs_capacity = stats[i].trial_costs_capacity;
-        stats[i] = z;
+        reset_stats(&stats[i]);
       }
     }
     if (batch->precision == SFMBA_PRECISION_MIXED)

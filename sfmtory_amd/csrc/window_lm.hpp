@@ -348,7 +348,7 @@ __global__ __launch_bounds__(WGS) void k_window_lm(const View<T>* __restrict__ v
   uint64_t iterations = 0, n_lin = 0, n_fact = 0, n_chol_fail = 0, n_lu = 0, n_rej = 0, n_outer = 0;
 
   if (v.D != 0) {
-    double lambda = prm.initial_lambda, nu = 2.0, mixed_floor = 0.0;
+    double lambda = prm.initial_lambda, nu = 2.0;
     for (uint64_t outer = 0; outer < prm.max_iterations; ++outer) {
       ++iterations;
       const double cost_before = cost;
@@ -363,23 +363,17 @@ __global__ __launch_bounds__(WGS) void k_window_lm(const View<T>* __restrict__ v
         ++n_fact;
         double trial = 0.0, trial_front = 0.0;
         bool have_step = false, stepped = false;
-        const double lam0 = mixed ? fmax(lambda, mixed_floor) : lambda;
+        // FP32-assembled systems are damped at least at their resolution
+        const double lam0 = mixed ? fmax(lambda, MIXED_MIN_LAMBDA) : lambda;
         wg_assemble<T, NQ, NP>(v, lam0);
         if (wg_cholesky(v.S, v.D, panel, sh)) {
           have_step = true;
         } else {
           ++n_chol_fail;
           if (mixed) {
-            // FP32-assembled systems: refactor with a growing damping floor
-            // that is kept for the rest of the solve (no LU).
-            const double shifts[5] = {1e-6, 1e-5, 1e-4, 1e-3, 1e-2};
-            for (int q = 0; q < 5 && !have_step; ++q) {
-              if (shifts[q] <= lam0) continue;
-              mixed_floor = shifts[q];
-              wg_assemble<T, NQ, NP>(v, shifts[q]);
-              if (wg_cholesky(v.S, v.D, panel, sh)) have_step = true;
-              else ++n_chol_fail;
-            }
+            // no step: the damping bump takes over (an LU solve of an
+            // FP32-assembled system that rounding made indefinite would
+            // carry no information)
           } else {
             // not positive definite: rebuild S and solve by LU (lib.rs:1083-1093)
             wg_assemble<T, NQ, NP>(v, lam0);
