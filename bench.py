@@ -2,22 +2,22 @@
 mixed precision vs FP64 plain, on one MI355X; and the 256-window batch.
 
 A step is one full Levenberg-Marquardt solve to convergence (the reference's
-stopping rule) of the 1,000-PINHOLE-camera / 500k-point / ~3M-observation
-scene (1% outliers, Huber 2 px), starting from the same perturbed state,
-with the problem already resident in HBM (created once, untimed).
+stopping rule, crates/sfm-ba/src/lib.rs:1188-1221) of the 1,000-PINHOLE-camera
+/ 500k-point / ~3M-observation scene (1% outliers, Huber 2 px), starting
+from the same perturbed state, with the problem already resident in HBM
+(created once, untimed).
 
-Prints one JSON line.  ``value`` is the mixed-precision time to converge to
-the FP64 plain solve, in seconds (lower is better): the mixed solve cut at
-the first iteration whose cost reaches the FP64 solve's final cost, timed
-as a solve of its own.  The mixed solve run to the reference's stopping
-rule, the FP64 plain time, the speed-up, Jacobian and Schur-assembly
-throughput (observations/s), residual statistics and pose error of both
-forms, the roofline of the dominant kernel (the factorization's trailing
-update, timed per launch with HIP events) with the whole factorization and
-the Schur assembly beside it, a CPU baseline from the oracle, the
-256-window batched solve (windows/s: a tuned, graph-replayed batch plan and
-one-off batched calls) and the 5,000-camera scene solved by
-PCG and by Cholesky are reported beside it.
+Prints one JSON line.  ``value`` is the mixed-precision solve's time to
+converge by that rule, in seconds (lower is better), the mean over the
+timed steps.  Beside it: the FP64 plain solve timed the same way, the
+speed-up, per-try damping traces of both, Jacobian and Schur-assembly
+throughput (observations/s), residual statistics (RMS from the residuals,
+mean |r|, inlier RMS) and pose error of both forms, the roofline of the
+dominant kernel (the factorization's trailing update, timed per launch
+with HIP events on its stream) with the whole factorization, the Schur
+assembly and the Jacobian stage beside it, a CPU baseline from the oracle
+(all host cores), the 256-window batched solve (windows/s) and the
+5,000-camera scene solved by PCG and by Cholesky.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--scene c2] [--no-cpu-baseline] [--no-windows]
                   [--no-large] [--copy-calibration]
@@ -60,6 +60,22 @@ def measured_traffic(path=TRAFFIC_PROFILE, key="traffic_bytes_per_launch"):
     except OSError:
         pass
     return None
+
+
+def traffic_source(path=TRAFFIC_PROFILE):
+    """Where a roofline's `traffic` comes from: the committed counter-run
+    summary (not this run) and the commit it was measured at."""
+    commit = None
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("measured_at_commit:"):
+                    commit = line.split(":", 1)[1].strip()
+    except OSError:
+        return None
+    rel = os.path.relpath(path, ROOT)
+    return (f"{rel}: FETCH_SIZE and WRITE_SIZE counter runs of this bench command (separate rocprofv3 runs, "
+            f"not this run)" + (f", measured at commit {commit}" if commit else ""))
 
 
 def copy_calibration():
@@ -235,34 +251,77 @@ def large_section(ba, scenes):
     return res
 
 
+def trace(st):
+    """Per-try damping and outcome of a solve: A accepted, R rejected, N no
+    step; suffix 3 for a reduced system assembled in FP32, L for the LU
+    fallback."""
+    return {"lambdas": [float(x) for x in st["try_lambdas"]],
+            "outcomes": " ".join("ARN?"[o & 3] + ("L" if o & 4 else "") + ("3" if o & 8 else "")
+                                 for o in (int(x) for x in st["try_outcomes"])),
+            "fp32_tries": int(st["num_fp32_tries"])}
+
+
 def per_unit(stats, key, count_key):
     n = stats[count_key]
     return stats[key] / n if n else float("nan")
 
 
+def host_info():
+    """Host cores this process may use, CPU model and memory."""
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    mem = None
+    try:
+        with open("/proc/meminfo") as f:
+            for line in f:
+                if line.startswith("MemTotal:"):
+                    mem = int(line.split()[1]) * 1024
+                    break
+    except OSError:
+        pass
+    return {"nproc": cores, "os_cpu_count": os.cpu_count(), "cpu_model": model, "mem_total_bytes": mem}
+
+
 def cpu_baseline(inp, params, fp64_stats, threads):
-    """Oracle (CPU FP64 restatement of the reference) on a bounded sample:
-    one outer iteration's stages on the full scene with the reference's
-    threading, plus a 1500 x 1500 Cholesky scaled by D^3; combined with the
-    GPU FP64 run's iteration and try counts into an estimated CPU
-    time-to-converge."""
+    """Oracle (the C++ FP64 restatement of the reference, threaded as the
+    reference is: parallel Jacobians, serial accumulation, per-thread D x D
+    Schur accumulators) timed on this host at full size: one outer
+    iteration's stages on the full scene with `threads` threads, and one
+    single-threaded Cholesky factorization + solve of the reference-layout
+    reduced system's size D (nalgebra's Cholesky::new is single-threaded,
+    lib.rs:1081).  Combined with the GPU FP64 solve's linearization and try
+    counts (the same trajectory) into the CPU time-to-converge."""
     import oracle
 
     t = oracle.stage_times(inp, params, nthreads=threads)
-    n_chol = 1500
-    tc = oracle.cholesky_seconds(n_chol)
-    D = t["dim"]
-    t_chol = tc * (D / n_chol) ** 3
+    D = int(t["dim"])
+    t0 = time.perf_counter()
+    t_chol = oracle.cholesky_seconds(D)
+    chol_wall = time.perf_counter() - t0
     n_lin = fp64_stats["num_linearizations"]
     n_try = fp64_stats["num_factorizations"]
-    est = n_lin * (t["jacobians"] + t["accumulate"]) + n_try * (t["schur"] + t_chol + t["cost"])
-    sample = (f"oracle stages of one outer iteration on the full scene ({threads} threads: Jacobians "
-              f"{t['jacobians']:.2f}s, serial accumulation {t['accumulate']:.2f}s, Schur assembly {t['schur']:.2f}s, "
-              f"cost {t['cost']:.2f}s) + single-threaded {n_chol}^2 Cholesky {tc:.2f}s scaled by (D={D}/{n_chol})^3; "
-              f"combined with the GPU FP64 run's {n_lin} linearizations and {n_try} tries")
+    est = t["cost"] + n_lin * (t["jacobians"] + t["accumulate"]) + n_try * (t["schur"] + t_chol + t["cost"])
+    sample = (f"oracle on the full scene, timed on this host: one outer iteration's stages with {threads} "
+              f"threads (Jacobians {t['jacobians']:.2f}s, serial accumulation {t['accumulate']:.2f}s, "
+              f"Schur assembly {t['schur']:.2f}s, cost {t['cost']:.2f}s) and one single-threaded D={D} "
+              f"Cholesky + solve ({t_chol:.1f}s); time-to-converge = cost + {n_lin} x (Jacobians + accumulation) "
+              f"+ {n_try} x (Schur + Cholesky + cost), the GPU FP64 solve's linearization and try counts")
     return {"value": est, "unit": "s", "cores": threads, "kind": "port", "sample": sample,
             "stage_seconds": {k: float(v) for k, v in t.items() if k != "dim"},
-            "cholesky_seconds_estimate": t_chol}
+            "cholesky_seconds": t_chol, "cholesky_dim": D, "timed_seconds": sum(float(v) for k, v in t.items()
+                                                                                if k != "dim") + chol_wall,
+            "host": host_info()}
 
 
 def main() -> int:
@@ -323,51 +382,8 @@ def main() -> int:
 
     mixed_t = float(np.mean(mixed_walls))
     fp64_t = float(np.mean(fp64_walls))
-    # Like-for-like: the mixed solve stopped at the iteration where it first
-    # reaches the FP64 solve's final cost (same trajectory, cut short), and
-    # then, if its RMS reprojection error there is not yet within 1e-4 px of
-    # the FP64 solve's, at the first later iteration where it is (searched up
-    # to two iterations past the FP64 solve's count).  The plain RMS drifts by
-    # a few 1e-4 px per iteration near convergence (the Huber objective trades
-    # outlier residuals), so equal cost alone does not pin it.
-    fp64_final = fp64_outs[-1].final_cost
     q_fp64 = residual_stats(ba, inp, fp64_outs[-1])
     rms_fp64 = q_fp64["rms_px"]
-    mixed_costs = prof_mixed["outer_costs"]
-    reach = next((i + 1 for i, c in enumerate(mixed_costs) if c <= fp64_final), None)
-    reach_t = None
-    q_reach = None
-    cut, q_cut, cut_t, cut_search = None, None, None, []
-
-    def timed_cut(k):
-        p_k = ba.BaParams(max_iterations=k)
-        prob.solve(p_k, precision="mixed")
-        tr = []
-        for _ in range(max(1, args.steps)):
-            t = time.perf_counter()
-            out_k = prob.solve(p_k, precision="mixed")
-            tr.append(time.perf_counter() - t)
-        return float(np.mean(tr)), out_k
-
-    if reach is not None:
-        reach_t, out_reach = timed_cut(reach)
-        q_reach = residual_stats(ba, inp, out_reach)
-        log(f"mixed reaches the FP64 final cost at iteration {reach}: {reach_t:.3f}s")
-        last = max(reach, fp64_outs[-1].iterations_run + 2)
-        for k in range(reach, last + 1):
-            if k == reach:
-                q_k, c_k = q_reach, out_reach.final_cost
-            else:
-                o_k = prob.solve(ba.BaParams(max_iterations=k), precision="mixed")
-                q_k, c_k = residual_stats(ba, inp, o_k), o_k.final_cost
-            d_k = abs(q_k["rms_px"] - rms_fp64)
-            cut_search.append({"iterations": k, "cost": c_k, "rms_px": q_k["rms_px"], "rms_diff_px": d_k})
-            if d_k <= 1e-4:
-                cut, q_cut = k, q_k
-                break
-        if cut is not None:
-            cut_t = reach_t if cut == reach else timed_cut(cut)[0]
-            log(f"mixed within 1e-4 px of the FP64 RMS at iteration {cut}: {cut_t:.3f}s")
     D = prof_fp64["dim"]
     q_mixed = residual_stats(ba, inp, mixed_outs[-1])
     rms_mixed = q_mixed["rms_px"]
@@ -397,6 +413,7 @@ def main() -> int:
     roof = {"bound": "mfma", "kernel": "k_syrk_update (FP64 trailing update + lookahead diagonal factorization)",
             "achieved": f_tu / t_tu / 1e12, "peak": PEAK_FP64_MFMA / 1e12, "unit": "TFLOP/s",
             "frac": f_tu / t_tu / PEAK_FP64_MFMA, "traffic": measured_traffic(),
+            "traffic_source": traffic_source(),
             "flop_per_launch": f_tu, "avg_launch_us": t_tu * 1e6, "launches": n_tu}
     # The whole factorization (D^3/3 + 2 D^2 FP64 FLOP) as a kernel group.
     flop = D ** 3 / 3.0 + 2.0 * D ** 2
@@ -410,6 +427,7 @@ def main() -> int:
     roof_schur = {"bound": "hbm", "kernel": "Schur assembly (mixed)", "achieved": bytes_schur / t_as / 1e9,
                   "peak": PEAK_HBM / 1e9, "unit": "GB/s", "frac": bytes_schur / t_as / PEAK_HBM,
                   "traffic": measured_traffic(ASSEMBLY_TRAFFIC_PROFILE, "schur_traffic_bytes"),
+                  "traffic_source": traffic_source(ASSEMBLY_TRAFFIC_PROFILE),
                   "algorithmic_bytes": bytes_schur}
     if roof_schur["traffic"]:
         roof_schur["measured_GB_per_s"] = roof_schur["traffic"] / t_as / 1e9
@@ -420,13 +438,14 @@ def main() -> int:
     roof_lin = {"bound": "hbm", "kernel": "Jacobian stage: k_linearize + k_point_accumulate (mixed)", "achieved": bytes_lin / t_lin / 1e9,
                 "peak": PEAK_HBM / 1e9, "unit": "GB/s", "frac": bytes_lin / t_lin / PEAK_HBM,
                 "traffic": measured_traffic(ASSEMBLY_TRAFFIC_PROFILE, "linearize_traffic_bytes"),
+                "traffic_source": traffic_source(ASSEMBLY_TRAFFIC_PROFILE),
                 "algorithmic_bytes": bytes_lin}
     if roof_lin["traffic"]:
         roof_lin["measured_GB_per_s"] = roof_lin["traffic"] / t_lin / 1e9
 
     windows = None
     if not args.no_windows:
-        windows = windows_section(ba, scenes, min(16, os.cpu_count() or 1), cpu=not args.no_cpu_baseline)
+        windows = windows_section(ba, scenes, host_info()["nproc"], cpu=not args.no_cpu_baseline)
 
     large = None
     if not args.no_large:
@@ -435,18 +454,14 @@ def main() -> int:
 
     cpu = None
     if not args.no_cpu_baseline:
-        threads = min(16, os.cpu_count() or 1)
-        log(f"cpu baseline: oracle stages with {threads} threads ...")
+        threads = host_info()["nproc"]
+        log(f"cpu baseline: oracle stages with {threads} threads and one full-size Cholesky ...")
         cpu = cpu_baseline(inp, params, prof_fp64, threads)
-        log(f"cpu baseline estimate {cpu['value']:.1f}s")
+        log(f"cpu baseline {cpu['value']:.1f}s (Cholesky D={cpu['cholesky_dim']}: {cpu['cholesky_seconds']:.1f}s)")
 
-    # value: the mixed form's time to converge to the FP64 plain solve: a
-    # timed solve cut at the first iteration that reaches its final cost and
-    # has its RMS reprojection error within 1e-4 px (the equal-cost cut if no
-    # iteration in the search meets the RMS bound; then rms_criterion_met is
-    # false); the mixed solve run to the reference's own stopping rule is
-    # beside it.
-    value = cut_t if cut_t is not None else (reach_t if reach_t is not None else mixed_t)
+    # value: the mixed form's time to converge by the reference's stopping
+    # rule (mean of the timed steps); the FP64 plain solve beside it.
+    value = mixed_t
     line = {
         "metric": "BA time-to-converge (s) + Jac/Schur obs/s, mixed vs FP64 plain, 1000-cam scene",
         "value": value,
@@ -458,33 +473,26 @@ def main() -> int:
         "higher_is_better": False,
         "scaling": "strong",
         "vs_baseline": None,
-        "dtype": "f32 assembly / f64 solve (mixed); f64 (plain)",
+        "dtype": "f32 assembly while lambda >= 1e-7, then f64; f64 point inverses, cost and Cholesky (mixed); "
+                 "f64 (plain)",
         "data": "synthetic (seeded scene generator)",
         "config": {"workload": "1000 PINHOLE cameras, 500k points, %d observations (mean track 6), 1%% outliers, "
                                "Huber 2px, pose 0 fixed, principal points pinned, D=%d" % (n_obs, D),
                    "scene": args.scene, "max_iterations": 50},
-        "value_definition": "mixed-precision solve time to reach the FP64 plain solve's final cost with its "
-                            "RMS reprojection error within 1e-4 px of the FP64 solve's",
-        "rms_criterion_met": cut is not None,
-        "mixed_iterations_at_value": cut if cut is not None else reach,
+        "value_definition": "mixed-precision solve, device-resident problem, run to the reference's stopping "
+                            "rule (relative cost decrease < 1e-6, no accepted step in 8 tries, or 50 "
+                            "iterations); wall-clock mean over the timed steps",
         "fp64_time_to_converge_s": fp64_t,
-        "mixed_time_to_stopping_rule_s": mixed_t,
         "mixed_speedup_vs_fp64": fp64_t / value,
-        "mixed_time_to_fp64_final_cost_s": reach_t,
-        "mixed_iterations_to_fp64_final_cost": reach,
+        "rms_criterion_met": abs(rms_mixed - rms_fp64) <= 1e-4,
+        "step_walls_s": {"mixed": mixed_walls, "fp64": fp64_walls},
+        "traces": {"mixed": trace(prof_mixed), "fp64": trace(prof_fp64)},
         "iterations": {"mixed": mixed_outs[-1].iterations_run, "fp64": fp64_outs[-1].iterations_run},
         "device_resident_solve_s": {"mixed": float(np.mean([o.stats["time_total_s"] for o in mixed_outs])),
                                     "fp64": float(np.mean([o.stats["time_total_s"] for o in fp64_outs]))},
         "problem_create_s": create_s,
         "rms_px": {"mixed": rms_mixed, "fp64": rms_fp64, "diff": abs(rms_mixed - rms_fp64)},
         "residuals": {"mixed": q_mixed, "fp64": q_fp64},
-        "rms_px_at_fp64_cost": None if q_reach is None else
-            {"mixed": q_reach["rms_px"], "fp64": rms_fp64, "diff": abs(q_reach["rms_px"] - rms_fp64),
-             "inlier_rms_mixed": q_reach["inlier_rms_px"], "inlier_rms_fp64": q_fp64["inlier_rms_px"]},
-        "rms_px_at_value": None if q_cut is None else
-            {"mixed": q_cut["rms_px"], "fp64": rms_fp64, "diff": abs(q_cut["rms_px"] - rms_fp64),
-             "inlier_rms_mixed": q_cut["inlier_rms_px"], "inlier_rms_fp64": q_fp64["inlier_rms_px"]},
-        "rms_cut_search": cut_search,
         "pose_error": pe,
         "final_cost": {"mixed": mixed_outs[-1].final_cost, "fp64": fp64_outs[-1].final_cost},
         "mixed": r_mixed,

@@ -44,6 +44,7 @@ def load() -> C.CDLL:
             "oracle_numeric_jacobians": (C.c_int, [P, P, P, d, d, d, P, P, P, P]),
             "oracle_total_cost": (C.c_int, [P, P, P]),
             "oracle_mean_reprojection_error": (C.c_int, [P, P]),
+            "oracle_residual_norms": (C.c_int, [P, P, P]),
             "oracle_bundle_adjust": (C.c_int, [P, P, P, C.c_int, P, C.c_size_t, P, P, C.c_size_t, P]),
             "oracle_default_fixed_params_mask": (C.c_int, [P, C.c_size_t, P, P]),
             "oracle_refine_intrinsics": (C.c_int, [P, C.c_size_t, P, C.c_size_t, P, P, C.c_size_t, P,
@@ -154,6 +155,24 @@ def mean_reprojection_error(inp: BaInput) -> float:
     if load().oracle_mean_reprojection_error(C.addressof(ci.c), C.addressof(out)):
         raise ValueError("oracle: invalid input")
     return out.value
+
+
+def residual_norms(inp: BaInput):
+    """|reprojection residual| per observation and its in-front flag
+    (lib.rs:249-261)."""
+    ci = _CInput(inp)
+    n = len(inp.observations)
+    norm = np.zeros(n)
+    front = np.zeros(n, dtype=np.uint8)
+    if load().oracle_residual_norms(C.addressof(ci.c), norm.ctypes.data, front.ctypes.data):
+        raise ValueError("oracle: invalid input")
+    return norm, front.astype(bool)
+
+
+def rms_reprojection_error(inp: BaInput) -> float:
+    """Plain RMS of |r| over the observations in front of their camera."""
+    norm, front = residual_norms(inp)
+    return float(np.sqrt(np.mean(norm[front] ** 2))) if front.any() else 0.0
 
 
 def bundle_adjust(inp: BaInput, params: BaParams | None = None, nthreads: int = 1) -> dict:

@@ -1129,6 +1129,23 @@ int oracle_total_cost(const sfmba_input* in, const sfmba_params* prm, double* ou
   return 0;
 }
 
+// |reprojection_residual| of every observation (lib.rs:249-261) and its
+// in-front flag (0: behind the camera, norm 0): the plain RMS and mean
+// reprojection statistics of a state.
+int oracle_residual_norms(const sfmba_input* in, double* norm, uint8_t* in_front) {
+  if (!validate(in)) return SFMBA_ERR_INVALID_ARGUMENT;
+  for (size_t o = 0; o < in->num_observations; ++o) {
+    const sfmba_observation& ob = in->observations[o];
+    double r[2];
+    const double* X = &in->points[3 * ob.point_idx];
+    const bool ok = reprojection_residual(in->poses[ob.image_idx], in->cameras[in->camera_of_image[ob.image_idx]],
+                                          V3{X[0], X[1], X[2]}, ob.x, ob.y, r);
+    in_front[o] = ok ? 1 : 0;
+    norm[o] = ok ? norm2(r) : 0.0;
+  }
+  return 0;
+}
+
 // lib.rs:708-725
 int oracle_mean_reprojection_error(const sfmba_input* in, double* out) {
   if (!validate(in)) return SFMBA_ERR_INVALID_ARGUMENT;

@@ -84,8 +84,11 @@ def test_mixed_windows_close_to_fp64():
     mix = ba.bundle_adjust_batch([w.inp for w in wins], p, precision="mixed")
     for a, b, w in zip(f64, mix, wins):
         assert b.final_cost == pytest.approx(a.final_cost, rel=1e-5)
-        ra = np.sqrt(2 * a.final_cost / len(w.inp.observations))
-        rb = np.sqrt(2 * b.final_cost / len(w.inp.observations))
+        # plain RMS reprojection error from the residuals, by the oracle
+        ra = oracle.rms_reprojection_error(ba.BaInput(w.inp.camera_of_image, a.cameras, a.poses, a.points,
+                                                      w.inp.observations))
+        rb = oracle.rms_reprojection_error(ba.BaInput(w.inp.camera_of_image, b.cameras, b.poses, b.points,
+                                                      w.inp.observations))
         assert abs(ra - rb) < 1e-4
 
 

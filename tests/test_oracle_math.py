@@ -116,3 +116,33 @@ def test_cholesky_fails_on_indefinite_lu_does_not():
     assert oracle.cholesky_solve(A, b) is None
     assert np.allclose(oracle.lu_solve(A, b), [1, -0.5, 1 / 3])
     assert oracle.lu_solve(np.zeros((2, 2)), np.ones(2)) is None
+
+
+def test_residual_norms_pinned():
+    """oracle.residual_norms: its mean over in-front observations is the
+    reference's mean_reprojection_error (lib.rs:708-725); on a PINHOLE scene
+    every norm equals |K (R X + t) / z - x| computed here in numpy from the
+    closed form; noiseless ground truth gives zero."""
+    from sfmtory_amd import scenes
+    from sfmtory_amd.ba import BaInput
+    sc = scenes.generate("c2", num_images=12, num_points=300, seed=77)
+    inp = sc.inp
+    norm, front = oracle.residual_norms(inp)
+    assert front.all()
+    assert norm.mean() == pytest.approx(oracle.mean_reprojection_error(inp), rel=1e-12)
+    obs = inp.observations
+    P = np.asarray(inp.poses)[obs["image_idx"].astype(int)]
+    X = np.asarray(inp.points)[obs["point_idx"].astype(int)]
+    w, x, y, z = P[:, 0], P[:, 1], P[:, 2], P[:, 3]
+    R = np.stack([np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)], -1),
+                  np.stack([2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)], -1),
+                  np.stack([2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)], -1)], -2)
+    pc = np.einsum("nij,nj->ni", R, X) + P[:, 4:7]
+    cam = inp.cameras["params"][np.asarray(inp.camera_of_image)[obs["image_idx"].astype(int)].astype(int)]
+    u = cam[:, 0] * pc[:, 0] / pc[:, 2] + cam[:, 2]
+    v = cam[:, 1] * pc[:, 1] / pc[:, 2] + cam[:, 3]
+    assert np.allclose(norm, np.hypot(u - obs["x"], v - obs["y"]), rtol=1e-9, atol=1e-9)
+    clean = scenes.generate("c2", num_images=12, num_points=300, seed=77, pixel_sigma=0.0, outlier_frac=0.0)
+    truth = BaInput(clean.inp.camera_of_image, clean.cameras_true, clean.poses_true, clean.points_true,
+                    clean.inp.observations)
+    assert oracle.rms_reprojection_error(truth) < 1e-9
