@@ -353,6 +353,28 @@ int sfmba_mean_reprojection_error(const sfmba_input* input, double* out);
 int sfmba_total_cost(const sfmba_input* input, const sfmba_params* params,
                      double* out);
 
+/* sfmba_total_cost of a state already in device memory, allocating no
+ * device memory.  No counterpart in the reference: for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).
+ *  - device_input is a host struct whose arrays (observations,
+ *    camera_of_image, cameras, poses, points) are device pointers on the
+ *    current HIP device, in this header's layouts; the fixed_* fields are
+ *    ignored.  Observations are summed in the caller's order.
+ *  - stream is a hipStream_t (NULL: the null stream); the call enqueues
+ *    its kernels there and synchronizes it before returning *out (host).
+ *  - Partial sums live in one static device array of the library, of
+ *    SFMBA_NOALLOC_MAX_OBSERVATIONS / 2048 + 2 doubles (256 KB) per GPU;
+ *    more observations than SFMBA_NOALLOC_MAX_OBSERVATIONS are rejected
+ *    (SFMBA_ERR_INVALID_ARGUMENT).  Every call on one GPU shares that GPU's
+ *    array: calls that overlap on one GPU race.
+ *  - An out-of-range index fails with SFMBA_ERR_INVALID_ARGUMENT, as in
+ *    sfmba_total_cost (checked on the device; nothing is read out of
+ *    range). */
+#define SFMBA_NOALLOC_MAX_OBSERVATIONS (1ull << 26)
+int sfmba_total_cost_noalloc(const sfmba_input* device_input, const sfmba_params* params, void* stream,
+                             double* out);
+
 /* ------------------------------------------------ fixed-parameter mask
  * Replaces: pub fn default_fixed_params_mask(&[CameraModel]) -> Vec<Vec<bool>>
  * (crates/sfm-ba/src/intrinsics.rs:33-61).  Writes num_cameras masks of 8

@@ -17,6 +17,8 @@ SFMBA_ERR_DEVICE = -2
 SFMBA_ERR_OUT_OF_MEMORY = -3
 SFMBA_ERR_UNSUPPORTED = -4
 
+NOALLOC_MAX_OBSERVATIONS = 1 << 26
+
 PRECISION_FP64 = 0
 PRECISION_MIXED = 1
 SOLVER_CHOLESKY = 0
@@ -185,6 +187,8 @@ SIGNATURES = {
     "sfmba_problem_destroy": (None, [C.c_void_p]),
     "sfmba_mean_reprojection_error": (C.c_int, [C.POINTER(Input), C.POINTER(C.c_double)]),
     "sfmba_total_cost": (C.c_int, [C.POINTER(Input), C.POINTER(Params), C.POINTER(C.c_double)]),
+    "sfmba_total_cost_noalloc": (C.c_int, [C.POINTER(Input), C.POINTER(Params), C.c_void_p,
+                                           C.POINTER(C.c_double)]),
     "sfmba_default_fixed_params_mask": (C.c_int, [C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p]),
     "sfmba_intrinsics_params_default": (None, [C.POINTER(IntrinsicsParams)]),
     "sfmba_refine_intrinsics": (C.c_int, [C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_void_p,

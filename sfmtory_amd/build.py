@@ -22,7 +22,8 @@ LIB = PKG / "lib"
 OBJ = PKG / "_build"
 
 HIP_SOURCES = ["engine_host.hip", "engine_kernels.hip", "dense_kernels.hip", "intrinsics_kernels.hip",
-               "window_kernels.hip", "window_kernels_512.hip", "window_kernels_256.hip", "pcg_kernels.hip"]
+               "window_kernels.hip", "window_kernels_512.hip", "window_kernels_256.hip", "pcg_kernels.hip",
+               "noalloc.hip"]
 
 # host-only translation units (no device code): g++, linked into libsfmba
 HOST_SOURCES = ["colmap_io.cpp", "scope.cpp"]
