@@ -206,6 +206,12 @@ typedef struct sfmba_stats {
   uint64_t tries_capacity;
   uint64_t tries_len;
   uint64_t num_fp32_tries;   /* mixed: tries whose reduced system was assembled in FP32 */
+  /* profile=1, mixed: the share of time_linearize_s / time_schur_s (and of
+   * their counts) spent in FP32 stages */
+  double time_linearize_fp32_s;
+  double time_schur_fp32_s;
+  uint64_t count_linearize_fp32;
+  uint64_t count_schur_fp32;
 } sfmba_stats;
 
 /* Outcome of one damping try (sfmba_stats.try_outcomes): the low two bits

@@ -98,6 +98,8 @@ class Stats(C.Structure):
         ("num_pcg_iterations", C.c_uint64),
         ("try_lambdas", C.c_void_p), ("try_outcomes", C.c_void_p), ("tries_capacity", C.c_uint64),
         ("tries_len", C.c_uint64), ("num_fp32_tries", C.c_uint64),
+        ("time_linearize_fp32_s", C.c_double), ("time_schur_fp32_s", C.c_double),
+        ("count_linearize_fp32", C.c_uint64), ("count_schur_fp32", C.c_uint64),
     ]
 
 

@@ -83,6 +83,7 @@ struct Layout {
   uint32_t D = 0, Dp = 0, ld = 0;
   uint64_t n_pairs = 0;
   uint32_t n_chunks = 0, n_dests = 0, n_multi = 0, n_parts = 0;
+  uint32_t n_chunks_plain = 0;  // leading chunks without same-observation pairs
   uint32_t max_grp_dim = 0;
   // observations (point-major)
   uint32_t* obs_img = nullptr;
@@ -109,7 +110,8 @@ struct Layout {
   uint32_t* grp_dim = nullptr;
   // Schur pair lists (destination-sorted)
   uint32_t* pairs = nullptr;      // 2 per pair: a's first F row, b's first row (flag bits)
-  uint4* chunks = nullptr;        // ((ga << 16) | gb, begin, end, part or NONE)
+  uint4* chunks = nullptr;        // ((ga << 16) | gb, begin, end, part or NONE): the
+                                  // n_chunks_plain chunks without same-observation pairs first
   uint32_t* multi_dest = nullptr; // destinations reduced from partials
   uint32_t* dest_part0 = nullptr; // first partial of each multi destination
   uint32_t* dest_nparts = nullptr;

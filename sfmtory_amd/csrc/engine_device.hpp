@@ -746,6 +746,166 @@ __device__ inline void schur_chunk_mfma8(const View<T>& v, const uint4 ch, doubl
   }
 }
 
+// ------------------------------------------------ LDS-staged Schur chunk
+typedef __attribute__((address_space(3))) void lds_void_t;
+
+// Pairs of one LDS batch of schur_chunk_lds: 8 KB of F and E row blocks.
+template <typename T>
+constexpr int lds_batch_pairs() {
+  return sizeof(T) == 4 ? 32 : 16;
+}
+
+// schur_chunk_mfma8 with its operands staged through LDS (blocks of at most
+// 8x8): the 8-row blocks of a batch of pairs move from global memory
+// straight into LDS (global_load_lds, 16 bytes per lane, one instruction per
+// 8 FP32 or 4 FP64 blocks, no registers held while they are in flight),
+// then the two-pairs-per-MFMA products read them from LDS.  The register
+// form is bound by its gathers' latency; here the bytes in flight per CU
+// are set by LDS instead of registers.  SAME: the chunk has pairs of the
+// same observation, so the J blocks of both sides are staged too, for
+// w J_a^T J_b (a second MFMA, K = the two residual components) and, on
+// diagonal blocks, the right-hand side F_a g_point - w J_a^T r over pairs
+// a == b, in pair order.  lds: this wavefront's (SAME ? 4 : 2) *
+// lds_batch_pairs<T>() * 32 elements.  Sums, FP32 flushing every two steps,
+// the quadrant merge and the outputs are schur_chunk_mfma8's.
+template <typename T, bool SAME>
+__device__ inline void schur_chunk_lds(const View<T>& v, const uint4 ch, double lambda, double* parts, T* lds) {
+  using M = Mfma<T>;
+  using acc_t = typename M::acc_t;
+  constexpr uint32_t HI = 0x80000000u, OFF = 0x7FFFFFFFu;
+  constexpr int BP = lds_batch_pairs<T>();
+  constexpr uint32_t ROWB = 4 * sizeof(T);  // bytes per row (4 components)
+  constexpr uint32_t LPB = 8 * ROWB / 16;   // lanes per 8-row block
+  constexpr uint32_t PPI = 64 / LPB;        // blocks per load instruction
+  static_assert(BP % PPI == 0 && 64 % BP == 0, "batches of whole load instructions within a 64-pair half");
+  static_assert(PAIR_CHUNK <= 128, "a chunk's pair words are held in two registers per lane");
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t i = lane & 7, h = (lane >> 3) & 1, k = lane >> 4;
+  const uint32_t ga = ch.x >> 16, gb = ch.x & 0xFFFFu;
+  const uint32_t da = v.grp_dim[ga], db = v.grp_dim[gb];
+  const bool diag = ga == gb;
+  const bool fa_on = i < da && k < 3, eb_on = i < db && k < 3;
+  const bool ja_on = i < da && k < 2, jb_on = i < db && k < 2;
+  const bool rlane = diag && lane < da;
+  const char* Fg = reinterpret_cast<const char*>(v.F4);
+  const char* Eg = reinterpret_cast<const char*>(v.E4);
+  const char* Jg = reinterpret_cast<const char*>(v.J4);
+  T* Fs = lds;
+  T* Es = lds + BP * 32;
+  T* JAs = lds + 2 * BP * 32;
+  T* JBs = lds + 3 * BP * 32;
+  const uint32_t n = ch.z - ch.y;
+  const uint2 w0 = lane < n ? v.pairs[ch.y + lane] : make_uint2(0u, 0u);
+  const uint2 w1 = lane + 64 < n ? v.pairs[ch.y + 64 + lane] : make_uint2(0u, 0u);
+  // per-pair flags of the chunk (bit q % 64 of half q / 64): same
+  // observation, and a == b
+  const uint64_t same_m[2] = {__ballot((w0.x & HI) != 0), __ballot((w1.x & HI) != 0)};
+  const uint64_t eq_m[2] = {__ballot((w0.y & HI) != 0), __ballot((w1.y & HI) != 0)};
+  // byte offset of this lane's 16 bytes in a block; rows past the block's
+  // dimension re-read row 0 of the same block (in bounds, masked below)
+  const uint32_t off = (lane % LPB) * 16;
+  const uint32_t offa = off / ROWB < da ? off : off % ROWB;
+  const uint32_t offb = off / ROWB < db ? off : off % ROWB;
+  acc_t s = {T(0), T(0), T(0), T(0)}, b = {T(0), T(0), T(0), T(0)};
+  double ss[4] = {0.0, 0.0, 0.0, 0.0}, bs[4] = {0.0, 0.0, 0.0, 0.0};
+  Acc<T> accR;
+  for (uint32_t q0 = 0; q0 < n; q0 += BP) {
+    const bool upper = q0 >= 64;
+    // pairs q0 .. q0 + BP - 1 into LDS; a lane past the chunk's end loads
+    // pair q0's block again (never read back)
+#pragma unroll
+    for (int j = 0; j < BP / (int)PPI; ++j) {
+      const uint32_t q = q0 + j * PPI;
+      uint32_t idx = q + lane / LPB;
+      if (idx >= n) idx = q0;
+      const uint32_t src = idx & 63;
+      const uint32_t ra = (uint32_t)__shfl((int)(upper ? w1.x : w0.x), (int)src) & OFF;
+      const uint32_t rb = (uint32_t)__shfl((int)(upper ? w1.y : w0.y), (int)src) & OFF;
+      __builtin_amdgcn_global_load_lds((const void*)(Fg + (size_t)ra * ROWB + offa),
+                                       (lds_void_t*)(Fs + j * PPI * 32), 16, 0, 0);
+      __builtin_amdgcn_global_load_lds((const void*)(Eg + (size_t)rb * ROWB + offb),
+                                       (lds_void_t*)(Es + j * PPI * 32), 16, 0, 0);
+      if (SAME) {
+        __builtin_amdgcn_global_load_lds((const void*)(Jg + (size_t)ra * ROWB + offa),
+                                         (lds_void_t*)(JAs + j * PPI * 32), 16, 0, 0);
+        __builtin_amdgcn_global_load_lds((const void*)(Jg + (size_t)rb * ROWB + offb),
+                                         (lds_void_t*)(JBs + j * PPI * 32), 16, 0, 0);
+      }
+    }
+    __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    const uint32_t nb = min((uint32_t)BP, n - q0);
+    const uint64_t sm = same_m[upper ? 1 : 0] >> (q0 & 63), em = eq_m[upper ? 1 : 0] >> (q0 & 63);
+    for (uint32_t u = 0; u < nb; u += 2) {
+      const uint32_t p = u + h;
+      const bool ok = p < nb;
+      const T fa = (ok && fa_on) ? Fs[p * 32 + i * 4 + k] : T(0);
+      const T eb = (ok && eb_on) ? Es[p * 32 + i * 4 + k] : T(0);
+      s = M::step(fa, eb, s);
+      if (SAME) {
+        if ((sm >> u) & 3u) {
+          const bool same = ok && ((sm >> p) & 1u);
+          const T ja = (same && ja_on) ? JAs[p * 32 + i * 4 + 2 + k] : T(0);
+          const T jb = (same && jb_on) ? JBs[p * 32 + i * 4 + k] : T(0);
+          b = M::step(ja, jb, b);
+        }
+        // right-hand side in pair order
+        if (rlane && ((em >> u) & 1u)) accR.add(Fs[u * 32 + lane * 4 + 3] - Es[u * 32 + lane * 4 + 3]);
+        if (rlane && u + 1 < nb && ((em >> (u + 1)) & 1u))
+          accR.add(Fs[(u + 1) * 32 + lane * 4 + 3] - Es[(u + 1) * 32 + lane * 4 + 3]);
+      }
+      if (sizeof(T) == 4 && (u & 2)) {
+        // FP32: at most two products per quadrant summed before FP64
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          ss[c] += (double)s[c];
+          bs[c] += (double)b[c];
+          s[c] = T(0);
+          b[c] = T(0);
+        }
+      }
+    }
+    // the next batch's loads overwrite this one: every lane's reads first
+    __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+  }
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    ss[c] += (double)s[c];
+    bs[c] += (double)b[c];
+  }
+  // Quadrant (8 + r, 8 + c) sits 40 lanes up in the same register (FP32
+  // layout) or 8 lanes up two registers on (FP64 layout).
+#pragma unroll
+  for (int q = 0; q < (sizeof(T) == 4 ? 4 : 2); ++q) {
+    const int src = sizeof(T) == 4 ? q : q + 2;
+    const int dl = sizeof(T) == 4 ? 40 : 8;
+    ss[q] += __shfl(ss[src], (int)((lane + dl) & 63));
+    bs[q] += __shfl(bs[src], (int)((lane + dl) & 63));
+  }
+  const uint32_t oa = v.grp_off[ga], ob = v.grp_off[gb];
+  const uint32_t c = lane & 15;
+  if (ch.w == NONE) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const uint32_t r = M::row(lane, q);
+      if (r >= da || c >= db || (diag && r < c)) continue;
+      double val = bs[q] - ss[q];
+      if (diag && r == c) val = (bs[q] + bs[q] * lambda) - ss[q];
+      v.S[(size_t)(ob + c) * v.ld + oa + r] = val;
+    }
+    if (rlane) v.S[(size_t)(oa + lane) * v.ld + v.Dp] = accR.value();
+  } else {
+    double* pp = parts + (size_t)ch.w * PART_STRIDE;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const uint32_t r = M::row(lane, q);
+      if (r >= da || c >= db) continue;
+      pp[r + c * da] = bs[q];
+      pp[196 + r + c * da] = ss[q];
+    }
+    if (rlane) pp[392 + lane] = accR.value();
+  }
+}
+
 // ------------------------------------------------- back-substitution + update
 template <typename T>
 __device__ inline void backsub_point(const View<T>& v, uint32_t p, int cur) {

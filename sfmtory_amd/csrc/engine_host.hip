@@ -396,7 +396,23 @@ static void upload_layout(const Pack& P, Problem& pr, hipStream_t s) {
   L.grp_off = A.upload(P.grp_off, s);
   L.grp_dim = A.upload(P.grp_dim, s);
   L.pairs = A.upload(P.pairs, s);
-  L.chunks = A.upload(P.chunks, s);
+  {
+    // Chunks without same-observation pairs first (the LDS-staged
+    // assembly), then those with them; tile order kept within each class
+    // (partials are indexed by chunk, so the order does not change sums).
+    std::vector<uint4> ordered;
+    ordered.reserve(P.chunks.size());
+    std::vector<uint8_t> same(P.chunks.size(), 0);
+    for (size_t c = 0; c < P.chunks.size(); ++c)
+      for (uint32_t q = P.chunks[c].y; q < P.chunks[c].z && !same[c]; ++q)
+        same[c] = (P.pairs[2 * (size_t)q] & 0x80000000u) != 0;
+    for (size_t c = 0; c < P.chunks.size(); ++c)
+      if (!same[c]) ordered.push_back(P.chunks[c]);
+    L.n_chunks_plain = (uint32_t)ordered.size();
+    for (size_t c = 0; c < P.chunks.size(); ++c)
+      if (same[c]) ordered.push_back(P.chunks[c]);
+    L.chunks = A.upload(ordered, s);
+  }
   L.multi_dest = A.upload(P.multi_dest, s);
   L.dest_part0 = A.upload(P.dest_part0, s);
   L.dest_nparts = A.upload(P.dest_nparts, s);
@@ -448,7 +464,15 @@ struct Profiler {
       float ms = 0.f;
       HIP_CHECK(hipEventElapsedTime(&ms, r.a, r.b));
       const double sec = ms * 1e-3;
-      switch (r.phase) {
+      // phases 5 and 6: the FP32 linearization and assembly of the mixed form
+      if (r.phase == 5) {
+        st->time_linearize_fp32_s += sec;
+        st->count_linearize_fp32++;
+      } else if (r.phase == 6) {
+        st->time_schur_fp32_s += sec;
+        st->count_schur_fp32++;
+      }
+      switch (r.phase % 5) {
         case 0: st->time_linearize_s += sec; st->count_linearize++; break;
         case 1: st->time_schur_s += sec; st->count_schur++; break;
         case 2: st->time_factor_s += sec; st->count_factor++; break;
@@ -678,7 +702,7 @@ struct Stages {
   }
   // Hot loops A and B (lib.rs:816-909) at state cur.
   void linearize(int cur) {
-    prof.begin(0);
+    prof.begin(sizeof(T) == 4 ? 5 : 0);
     launch_linearize<T>(H.pr.L, W, cur, loss, scale, H.s);
     launch_point_accumulate<T>(H.pr.L, W, H.s);
     prof.end();
@@ -700,12 +724,13 @@ struct Stages {
     if (reinterpret_cast<const int*>(h + 4)[2] != 0)
       throw Error(SFMBA_ERR_DEVICE, "back substitution hand-over timed out");
   }
-  // Reduced system at damping lam and its factorization.  Returns false when
-  // the Cholesky failed (flags[0], nalgebra's test) or PCG broke down.
+  // Reduced system at damping lam and its factorization, enqueued.  PCG
+  // (which synchronizes) returns false on a breakdown; the Cholesky's
+  // failure flag (flags[0], nalgebra's test) is read with the trial cost.
   bool assemble_factor(double lam) {
     const Layout& L = H.pr.L;
     hipStream_t s = H.s;
-    prof.begin(1);
+    prof.begin(sizeof(T) == 4 ? 6 : 1);
     launch_point_prepare<T>(L, W, lam, s);
     launch_schur<T>(L, W, lam, s);
     prof.end();
@@ -726,9 +751,7 @@ struct Stages {
     prof.begin(2);
     dense_cholesky(W.S, W.Linv, L.Dp, L.ld, L.D, W.flags, s, prof.on ? &prof.trailing : nullptr);
     prof.end();
-    HIP_CHECK(hipMemcpyAsync(H.pinned + 4, W.flags, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipStreamSynchronize(s));
-    return reinterpret_cast<const int*>(H.pinned + 4)[0] == 0;
+    return true;
   }
   // The step from the factor (L^T delta = L^-1 rhs), with optional FP64
   // iterative refinement against the assembled system.
@@ -753,9 +776,12 @@ struct Stages {
   bool try_step(int cur, double lam, double& trial, double& front, int& bits) {
     const Layout& L = H.pr.L;
     if (assemble_factor(lam)) {
+      // step, trial state and cost are enqueued behind the factorization
+      // and read back with its failure flag in one synchronization (a
+      // failed factorization's trial is discarded)
       if (!pcg) solve_factored();
       step_and_cost(cur, trial, front);
-      return true;
+      if (pcg || reinterpret_cast<const int*>(H.pinned + 4)[0] == 0) return true;
     }
     if (st) st->num_cholesky_failures++;
     if (pcg || sizeof(T) == 4) return false;

@@ -83,6 +83,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(schur_min_w
     schur_chunk_mfma<T, schur_pairs<T>()>(v, v.chunks[ci], lambda, parts);
 }
 
+// The LDS-staged Schur reduction (every block at most 8x8), one wavefront
+// per chunk: SAME = false for the chunks without same-observation pairs,
+// true for the others.
+template <typename T, bool SAME>
+__global__ __launch_bounds__(256) void k_schur_lds(View<T> v, double lambda, double* __restrict__ parts) {
+  constexpr int PER_WAVE = (SAME ? 4 : 2) * lds_batch_pairs<T>() * 32;
+  __shared__ T stage[4 * PER_WAVE];
+  const uint32_t w = threadIdx.x >> 6;
+  const uint32_t ci = blockIdx.x * 4 + w;
+  if (ci >= v.n_chunks) return;
+  schur_chunk_lds<T, SAME>(v, v.chunks[ci], lambda, parts, stage + w * PER_WAVE);
+}
+
 // Destinations split over several chunks: sum the partials in chunk order.
 __global__ __launch_bounds__(256) void k_schur_reduce(
     uint32_t n_multi, const uint32_t* __restrict__ multi_dest,
@@ -431,11 +444,25 @@ void launch_schur(const Layout& L, Work<T>& W, double lambda, hipStream_t s) {
   const View<T> v = make_view(L, W);
   if (L.n_chunks) {
     if (L.max_grp_dim > 16) throw Error(SFMBA_ERR_INVALID_ARGUMENT, "group dimension above 16");
-    const dim3 grid(blocks_for(L.n_chunks, 4));
-    if (L.max_grp_dim <= 8)
-      hipLaunchKernelGGL((k_schur_mfma<T, true>), grid, dim3(256), 0, s, v, lambda, W.parts);
-    else
-      hipLaunchKernelGGL((k_schur_mfma<T, false>), grid, dim3(256), 0, s, v, lambda, W.parts);
+    if (L.max_grp_dim <= 8) {
+      // LDS-staged: the chunks without same-observation pairs, then those
+      // with them (which stage the J blocks too)
+      View<T> vp = v, vr = v;
+      vp.n_chunks = L.n_chunks_plain;
+      vr.chunks = L.chunks + L.n_chunks_plain;
+      vr.n_chunks = L.n_chunks - L.n_chunks_plain;
+      if (vp.n_chunks) {
+        hipLaunchKernelGGL((k_schur_lds<T, false>), dim3(blocks_for(vp.n_chunks, 4)), dim3(256), 0, s, vp, lambda,
+                           W.parts);
+        HIP_CHECK(hipGetLastError());
+      }
+      if (vr.n_chunks)
+        hipLaunchKernelGGL((k_schur_lds<T, true>), dim3(blocks_for(vr.n_chunks, 4)), dim3(256), 0, s, vr, lambda,
+                           W.parts);
+    } else {
+      hipLaunchKernelGGL((k_schur_mfma<T, false>), dim3(blocks_for(L.n_chunks, 4)), dim3(256), 0, s, v, lambda,
+                         W.parts);
+    }
     HIP_CHECK(hipGetLastError());
   }
   if (L.n_multi) {
