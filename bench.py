@@ -494,7 +494,7 @@ def main() -> int:
         "higher_is_better": False,
         "scaling": "strong",
         "vs_baseline": None,
-        "dtype": "f32 assembly while lambda >= 1e-7, then f64; f64 point inverses, cost and Cholesky (mixed); "
+        "dtype": "f32 assembly while lambda >= 1e-5, then f64; f64 point inverses, cost and Cholesky (mixed); "
                  "f64 (plain)",
         "data": "synthetic (seeded scene generator)",
         "config": {"workload": "1000 PINHOLE cameras, 500k points, %d observations (mean track 6), 1%% outliers, "

@@ -142,12 +142,13 @@ int sfmba_bundle_adjust(const sfmba_input* input, const sfmba_params* params,
  * choices of this engine). */
 #define SFMBA_PRECISION_FP64 0  /* everything in FP64 (parity mode) */
 #define SFMBA_PRECISION_MIXED 1 /* FP32 Jacobian and Schur assembly while
-                                   the Marquardt damping is >= 1e-7 (the
-                                   resolution of an FP32-assembled system),
+                                   the Marquardt damping is >= 1e-5 (where
+                                   an FP32-assembled system still factors),
                                    FP64 assembly from then on; FP64 point
                                    inverses, residuals, cost and Cholesky
                                    throughout; optional FP64 iterative
                                    refinement of the step */
+#define SFMBA_MIXED_MIN_LAMBDA 1e-5 /* mixed: smallest damping assembled in FP32 */
 #define SFMBA_SOLVER_CHOLESKY 0 /* dense FP64 MFMA Cholesky (LU fallback) */
 #define SFMBA_SOLVER_PCG 1      /* block-Jacobi PCG on the reduced system */
 

@@ -20,6 +20,7 @@ SFMBA_ERR_UNSUPPORTED = -4
 NOALLOC_MAX_OBSERVATIONS = 1 << 26
 
 PRECISION_FP64 = 0
+MIXED_MIN_LAMBDA = 1e-5  # SFMBA_MIXED_MIN_LAMBDA
 PRECISION_MIXED = 1
 SOLVER_CHOLESKY = 0
 SOLVER_PCG = 1

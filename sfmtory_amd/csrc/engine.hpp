@@ -8,7 +8,8 @@
 //   (one merged block when the camera is used by this image alone);
 //   every incidence owns `gdim` consecutive "rows" (one per unknown of its
 //   block) in the row-indexed Jacobian stores:
-//     J4[row] = (J0, J1, w*J0, w*J1)   residual Jacobian column, plain and weighted
+//     J2[row] = sqrt(w) (J0, J1)       residual Jacobian column, scaled by the root
+//                                      of the observation's robust weight
 //     E4[row] = (E0, E1, E2, g)        E = w J^T Jx (camera-point coupling), g = w J^T r
 //     F4[row] = (F0, F1, F2, F.g)      F = E * Cinv(point), g the point gradient, per try;
 //   the reduced camera system S is dense, column-major, lower triangle,
@@ -39,14 +40,16 @@ inline size_t dense_workspace_doubles(uint32_t Dp) {
   return (size_t)(Dp / NB) * (LINV_BLOCK + TOP_BLOCK);
 }
 // Mixed form: the smallest Marquardt damping at which a reduced system is
-// assembled in FP32.  An FP32-assembled system is resolved only to about
-// 1e-7 of its diagonal, so below that damping its step no longer follows
-// the FP64 step, and rounding alone can make it indefinite (on the
-// 1,000-camera scene the first FP32 factorization fails at lambda = 5e-8).
-// The single-problem solver continues in FP64 from there; the window
-// solver, whose precision is fixed per launch, damps FP32 systems at least
-// this much (the LM schedule itself is unchanged).
-constexpr double MIXED_MIN_LAMBDA = 1e-7;
+// assembled in FP32.  Each entry of an FP32-assembled system carries
+// rounding of order 1e-7 of its diagonal, and over the D dimensions of the
+// system that perturbs its small eigenvalues (the free scale of a problem
+// gauged by one fixed pose has an eigenvalue of order lambda) by up to
+// about sqrt(D) times that: on the 1,000-camera scene (D = 7,994) the first
+// FP32 factorization fails at lambda = 1.4e-6.  Below this damping the
+// single-problem solver continues in FP64; the window solver, whose
+// precision is fixed per launch, damps FP32 systems at least this much (the
+// LM schedule itself is unchanged).
+constexpr double MIXED_MIN_LAMBDA = SFMBA_MIXED_MIN_LAMBDA;
 constexpr int PAIR_CHUNK = 128;   // max point pairs reduced by one wavefront
 constexpr uint64_t SCHUR_TILE = 64; // destination tile edge (parameter blocks) of the chunk order
 
@@ -126,7 +129,7 @@ struct Work {
   double* poses[2] = {nullptr, nullptr};   // 7 per image (w x y z tx ty tz)
   double* cams[2] = {nullptr, nullptr};    // 8 per camera
   double* pts[2] = {nullptr, nullptr};     // 3 per point
-  T* J4 = nullptr;
+  T* J2 = nullptr;             // 2 per row (+ one 8-row block of padding)
   T* E4 = nullptr;
   T* F4 = nullptr;
   T* obs_pc = nullptr;        // 9 per obs: w Jx^T Jx (6), w Jx^T r (3)

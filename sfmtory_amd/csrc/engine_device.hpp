@@ -37,10 +37,22 @@ struct Vec4<double> {
   using type = double4;
 };
 
+template <typename T>
+struct Vec2;
+template <>
+struct Vec2<float> {
+  using type = float2;
+};
+template <>
+struct Vec2<double> {
+  using type = double2;
+};
+
 // Pointers and sizes of one problem on the device (layout + work arrays).
 template <typename T>
 struct View {
   using V4 = typename Vec4<T>::type;
+  using V2 = typename Vec2<T>::type;
   uint32_t n_img, n_cam, n_pt, n_obs, n_inc, n_chunks;
   uint32_t D, Dp, ld;
   const uint32_t* obs_img;
@@ -63,7 +75,7 @@ struct View {
   const uint32_t* grp_dim;
   const uint2* pairs;
   const uint4* chunks;
-  V4* J4;
+  V2* J2;
   V4* E4;
   V4* F4;
   T* obs_pc;
@@ -100,7 +112,7 @@ struct Acc<double> {
 };
 
 // ---------------------------------------------------------------- linearize
-// Observation o (point-major order): for each of its incidences one J4 and
+// Observation o (point-major order): for each of its incidences one J2 and
 // one E4 row per unknown, and its contribution to its point's 3x3 block and
 // gradient (obs_pc).  With LPO > 1, LPO lanes share observation o: each
 // computes the same values and lane r writes rows r, r + LPO, ... and
@@ -143,7 +155,7 @@ __device__ inline void linearize_obs(const View<T>& v, uint32_t o, int cur, int 
       const uint32_t s = s0 + r;
       if (s >= ns) break;
       const uint32_t row = (two && s >= 6) ? roff1 + (s - 6) : roff0 + s;
-      v.J4[row] = z;
+      v.J2[row] = typename Vec2<T>::type{T(0), T(0)};
       v.E4[row] = z;
     }
     return;
@@ -152,8 +164,15 @@ __device__ inline void linearize_obs(const View<T>& v, uint32_t o, int cur, int 
   const double2 xy = v.obs_xy[o];
   double r0, r1;
   linearization_residual(model, P, c0, c1, c2, xy.x, xy.y, r0, r1);
+  // Every normal-equation term of the observation is built from one scaled
+  // Jacobian [sqrt(w) J | sqrt(w) Jx] and residual sqrt(w) r, rounded once:
+  // B = J2^T J2, E = J2^T X2, g = J2^T R2, C = X2^T X2, g_p = X2^T R2.  The
+  // terms are then exactly the Gram matrix of one rounded Jacobian, whose
+  // Schur complement stays positive semi-definite up to the rounding of
+  // the sums (with w and sqrt(w) rounded separately it does not, and an
+  // FP32 assembly loses definiteness at a larger damping).
   const double wd = robust_weight(sqrt(r0 * r0 + r1 * r1), loss, scale);
-  const T w = (T)wd;
+  const T sw = (T)sqrt(wd);
 
   T du[3], dv[3], jc0[8], jc1[8];
   model_jacobian<T>(model, P, (T)c0, (T)c1, (T)c2, du, dv, jc0, jc1);
@@ -162,21 +181,22 @@ __device__ inline void linearize_obs(const View<T>& v, uint32_t o, int cur, int 
   T jp[12], jx[6];
   pose_point_jacobians<T>(R, (T)(c0 - pose[4]), (T)(c1 - pose[5]), (T)(c2 - pose[6]), du, dv, jp,
                           jx);
-  const T tr0 = (T)r0, tr1 = (T)r1;
+  const T sr0 = (T)(sqrt(wd) * r0), sr1 = (T)(sqrt(wd) * r1);
+  T sx[6];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) sx[q] = sw * jx[q];
 
-  // Point terms: w Jx^T Jx (upper 6), w Jx^T r.
-  const T wx0 = w * jx[0], wx1 = w * jx[1], wx2 = w * jx[2];
-  const T wy0 = w * jx[3], wy1 = w * jx[4], wy2 = w * jx[5];
+  // Point terms: X2^T X2 (upper 6), X2^T R2.
   T pc[9];
-  pc[0] = wx0 * jx[0] + wy0 * jx[3];
-  pc[1] = wx0 * jx[1] + wy0 * jx[4];
-  pc[2] = wx0 * jx[2] + wy0 * jx[5];
-  pc[3] = wx1 * jx[1] + wy1 * jx[4];
-  pc[4] = wx1 * jx[2] + wy1 * jx[5];
-  pc[5] = wx2 * jx[2] + wy2 * jx[5];
-  pc[6] = wx0 * tr0 + wy0 * tr1;
-  pc[7] = wx1 * tr0 + wy1 * tr1;
-  pc[8] = wx2 * tr0 + wy2 * tr1;
+  pc[0] = sx[0] * sx[0] + sx[3] * sx[3];
+  pc[1] = sx[0] * sx[1] + sx[3] * sx[4];
+  pc[2] = sx[0] * sx[2] + sx[3] * sx[5];
+  pc[3] = sx[1] * sx[1] + sx[4] * sx[4];
+  pc[4] = sx[1] * sx[2] + sx[4] * sx[5];
+  pc[5] = sx[2] * sx[2] + sx[5] * sx[5];
+  pc[6] = sx[0] * sr0 + sx[3] * sr1;
+  pc[7] = sx[1] * sr0 + sx[4] * sr1;
+  pc[8] = sx[2] * sr0 + sx[5] * sr1;
 #pragma unroll
   for (uint32_t k0 = 0; k0 < 9; k0 += LPO) {
     T val = pc[k0];
@@ -241,10 +261,9 @@ __device__ inline void linearize_obs(const View<T>& v, uint32_t o, int cur, int 
         a1 = s1[q];
       }
     const uint32_t row = (two && s >= 6) ? roff1 + (s - 6) : roff0 + s;
-    const T w0 = w * a0, w1 = w * a1;
-    v.J4[row] = V4{a0, a1, w0, w1};
-    v.E4[row] = V4{w0 * jx[0] + w1 * jx[3], w0 * jx[1] + w1 * jx[4], w0 * jx[2] + w1 * jx[5],
-                   w0 * tr0 + w1 * tr1};
+    const T j0 = sw * a0, j1 = sw * a1;
+    v.J2[row] = typename Vec2<T>::type{j0, j1};
+    v.E4[row] = V4{j0 * sx[0] + j1 * sx[3], j0 * sx[1] + j1 * sx[4], j0 * sx[2] + j1 * sx[5], j0 * sr0 + j1 * sr1};
   }
 }
 
@@ -431,9 +450,9 @@ __device__ inline void schur_chunk(const View<T>& v, const uint4 ch, double lamb
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
           if (on[q]) {
-            const V4 ja = v.J4[ra + er[q]];
-            const V4 jb = v.J4[rb + ec[q]];
-            accB[q].add(ja.z * jb.x + ja.w * jb.y);
+            const auto ja = v.J2[ra + er[q]];
+            const auto jb = v.J2[rb + ec[q]];
+            accB[q].add(ja.x * jb.x + ja.y * jb.y);
           }
         }
       }
@@ -519,7 +538,7 @@ __device__ inline void schur_chunk_mfma(const View<T>& v, const uint4 ch, double
   const bool ja_on = i < da && k < 2, jb_on = i < db && k < 2;
   const T* F = reinterpret_cast<const T*>(v.F4);
   const T* E = reinterpret_cast<const T*>(v.E4);
-  const T* J = reinterpret_cast<const T*>(v.J4);
+  const T* J = reinterpret_cast<const T*>(v.J2);
   // All pair words of the chunk in one coalesced load: lane l holds pairs
   // l and 64 + l; pair q reaches every lane by v_readlane.
   const uint32_t n = ch.z - ch.y;
@@ -544,8 +563,8 @@ __device__ inline void schur_chunk_mfma(const View<T>& v, const uint4 ch, double
       st.fa[u] = (ok && fa_on) ? F[4 * (size_t)(ra + i) + k] : T(0);
       st.eb[u] = (ok && eb_on) ? E[4 * (size_t)(rb + i) + k] : T(0);
       const bool same = ok && (px & HI);
-      st.ja[u] = (same && ja_on) ? J[4 * (size_t)(ra + i) + 2 + k] : T(0);
-      st.jb[u] = (same && jb_on) ? J[4 * (size_t)(rb + i) + k] : T(0);
+      st.ja[u] = (same && ja_on) ? J[2 * (size_t)(ra + i) + k] : T(0);
+      st.jb[u] = (same && jb_on) ? J[2 * (size_t)(rb + i) + k] : T(0);
     }
   };
   acc_t s = {T(0), T(0), T(0), T(0)}, b = {T(0), T(0), T(0), T(0)};
@@ -634,7 +653,7 @@ __device__ inline void schur_chunk_mfma8(const View<T>& v, const uint4 ch, doubl
   const bool ja_on = i < da && k < 2, jb_on = i < db && k < 2;
   const T* F = reinterpret_cast<const T*>(v.F4);
   const T* E = reinterpret_cast<const T*>(v.E4);
-  const T* J = reinterpret_cast<const T*>(v.J4);
+  const T* J = reinterpret_cast<const T*>(v.J2);
   const uint32_t n_all = ch.z - ch.y;
   acc_t s = {T(0), T(0), T(0), T(0)}, b = {T(0), T(0), T(0), T(0)};
   double ss[4] = {0.0, 0.0, 0.0, 0.0}, bs[4] = {0.0, 0.0, 0.0, 0.0};
@@ -671,8 +690,8 @@ __device__ inline void schur_chunk_mfma8(const View<T>& v, const uint4 ch, doubl
         st.fa[u] = (ok && fa_on) ? F[4 * (size_t)(ra + i) + k] : T(0);
         st.eb[u] = (ok && eb_on) ? E[4 * (size_t)(rb + i) + k] : T(0);
         const bool same = ok && (px & HI);
-        st.ja[u] = (same && ja_on) ? J[4 * (size_t)(ra + i) + 2 + k] : T(0);
-        st.jb[u] = (same && jb_on) ? J[4 * (size_t)(rb + i) + k] : T(0);
+        st.ja[u] = (same && ja_on) ? J[2 * (size_t)(ra + i) + k] : T(0);
+        st.jb[u] = (same && jb_on) ? J[2 * (size_t)(rb + i) + k] : T(0);
       }
     };
     Stage cur, nxt;
@@ -749,10 +768,16 @@ __device__ inline void schur_chunk_mfma8(const View<T>& v, const uint4 ch, doubl
 // ------------------------------------------------ LDS-staged Schur chunk
 typedef __attribute__((address_space(3))) void lds_void_t;
 
-// Pairs of one LDS batch of schur_chunk_lds: 8 KB of F and E row blocks.
+// Pairs of one LDS batch of schur_chunk_lds: 8 KB of F and E row blocks
+// (and 4 KB of J2 blocks of both sides when the chunk has pairs of the same
+// observation).
 template <typename T>
 constexpr int lds_batch_pairs() {
   return sizeof(T) == 4 ? 32 : 16;
+}
+template <typename T, bool SAME>
+constexpr int lds_chunk_elems() {
+  return lds_batch_pairs<T>() * (2 * 32 + (SAME ? 2 * 16 : 0));
 }
 
 // schur_chunk_mfma8 with its operands staged through LDS (blocks of at most
@@ -762,12 +787,14 @@ constexpr int lds_batch_pairs() {
 // then the two-pairs-per-MFMA products read them from LDS.  The register
 // form is bound by its gathers' latency; here the bytes in flight per CU
 // are set by LDS instead of registers.  SAME: the chunk has pairs of the
-// same observation, so the J blocks of both sides are staged too, for
-// w J_a^T J_b (a second MFMA, K = the two residual components) and, on
-// diagonal blocks, the right-hand side F_a g_point - w J_a^T r over pairs
-// a == b, in pair order.  lds: this wavefront's (SAME ? 4 : 2) *
-// lds_batch_pairs<T>() * 32 elements.  Sums, FP32 flushing every two steps,
-// the quadrant merge and the outputs are schur_chunk_mfma8's.
+// same observation, so the sqrt(w) J blocks (two components per row) of
+// both sides are staged too, for w J_a^T J_b (a second MFMA, K = the two
+// residual components; b's block is a's when a == b, and is not loaded when
+// every pair of the chunk has a == b) and, on diagonal blocks, the
+// right-hand side F_a g_point - w J_a^T r over pairs a == b, in pair order.
+// lds: this wavefront's lds_chunk_elems<T, SAME>() elements.  Sums, FP32
+// flushing every two steps, the quadrant merge and the outputs are
+// schur_chunk_mfma8's.
 template <typename T, bool SAME>
 __device__ inline void schur_chunk_lds(const View<T>& v, const uint4 ch, double lambda, double* parts, T* lds) {
   using M = Mfma<T>;
@@ -789,11 +816,15 @@ __device__ inline void schur_chunk_lds(const View<T>& v, const uint4 ch, double 
   const bool rlane = diag && lane < da;
   const char* Fg = reinterpret_cast<const char*>(v.F4);
   const char* Eg = reinterpret_cast<const char*>(v.E4);
-  const char* Jg = reinterpret_cast<const char*>(v.J4);
+  const char* Jg = reinterpret_cast<const char*>(v.J2);
+  constexpr uint32_t JROWB = 2 * sizeof(T);  // bytes per J2 row
+  constexpr uint32_t JLPB = 8 * JROWB / 16;  // lanes per 8-row J2 block
+  constexpr uint32_t JPPI = 64 / JLPB;       // J2 blocks per load instruction
+  static_assert(BP % JPPI == 0, "whole J2 load instructions per batch");
   T* Fs = lds;
   T* Es = lds + BP * 32;
   T* JAs = lds + 2 * BP * 32;
-  T* JBs = lds + 3 * BP * 32;
+  T* JBs = lds + 2 * BP * 32 + BP * 16;
   const uint32_t n = ch.z - ch.y;
   const uint2 w0 = lane < n ? v.pairs[ch.y + lane] : make_uint2(0u, 0u);
   const uint2 w1 = lane + 64 < n ? v.pairs[ch.y + 64 + lane] : make_uint2(0u, 0u);
@@ -806,6 +837,12 @@ __device__ inline void schur_chunk_lds(const View<T>& v, const uint4 ch, double 
   const uint32_t off = (lane % LPB) * 16;
   const uint32_t offa = off / ROWB < da ? off : off % ROWB;
   const uint32_t offb = off / ROWB < db ? off : off % ROWB;
+  // J2 blocks: FP32 rows are 8 bytes, so a lane's 16 bytes may end one row
+  // past the block (the row array is padded by a block for that)
+  const uint32_t joff = (lane % JLPB) * 16;
+  const uint32_t joffa = joff / JROWB < da ? joff : 0u;
+  const uint32_t joffb = joff / JROWB < db ? joff : 0u;
+  const bool all_eq = SAME && (uint32_t)(__popcll(eq_m[0]) + __popcll(eq_m[1])) == n;
   acc_t s = {T(0), T(0), T(0), T(0)}, b = {T(0), T(0), T(0), T(0)};
   double ss[4] = {0.0, 0.0, 0.0, 0.0}, bs[4] = {0.0, 0.0, 0.0, 0.0};
   Acc<T> accR;
@@ -825,11 +862,23 @@ __device__ inline void schur_chunk_lds(const View<T>& v, const uint4 ch, double 
                                        (lds_void_t*)(Fs + j * PPI * 32), 16, 0, 0);
       __builtin_amdgcn_global_load_lds((const void*)(Eg + (size_t)rb * ROWB + offb),
                                        (lds_void_t*)(Es + j * PPI * 32), 16, 0, 0);
-      if (SAME) {
-        __builtin_amdgcn_global_load_lds((const void*)(Jg + (size_t)ra * ROWB + offa),
-                                         (lds_void_t*)(JAs + j * PPI * 32), 16, 0, 0);
-        __builtin_amdgcn_global_load_lds((const void*)(Jg + (size_t)rb * ROWB + offb),
-                                         (lds_void_t*)(JBs + j * PPI * 32), 16, 0, 0);
+    }
+    if (SAME) {
+#pragma unroll
+      for (int j = 0; j < BP / (int)JPPI; ++j) {
+        const uint32_t q = q0 + j * JPPI;
+        if (q >= n) break;
+        uint32_t idx = q + lane / JLPB;
+        if (idx >= n) idx = q0;
+        const uint32_t src = idx & 63;
+        const uint32_t ra = (uint32_t)__shfl((int)(upper ? w1.x : w0.x), (int)src) & OFF;
+        __builtin_amdgcn_global_load_lds((const void*)(Jg + (size_t)ra * JROWB + joffa),
+                                         (lds_void_t*)(JAs + j * JPPI * 16), 16, 0, 0);
+        if (!all_eq) {
+          const uint32_t rb = (uint32_t)__shfl((int)(upper ? w1.y : w0.y), (int)src) & OFF;
+          __builtin_amdgcn_global_load_lds((const void*)(Jg + (size_t)rb * JROWB + joffb),
+                                           (lds_void_t*)(JBs + j * JPPI * 16), 16, 0, 0);
+        }
       }
     }
     __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
@@ -844,8 +893,9 @@ __device__ inline void schur_chunk_lds(const View<T>& v, const uint4 ch, double 
       if (SAME) {
         if ((sm >> u) & 3u) {
           const bool same = ok && ((sm >> p) & 1u);
-          const T ja = (same && ja_on) ? JAs[p * 32 + i * 4 + 2 + k] : T(0);
-          const T jb = (same && jb_on) ? JBs[p * 32 + i * 4 + k] : T(0);
+          const T* jbs = ((em >> p) & 1u) ? JAs : JBs;  // a == b: b's block is a's
+          const T ja = (same && ja_on) ? JAs[p * 16 + i * 2 + k] : T(0);
+          const T jb = (same && jb_on) ? jbs[p * 16 + i * 2 + k] : T(0);
           b = M::step(ja, jb, b);
         }
         // right-hand side in pair order

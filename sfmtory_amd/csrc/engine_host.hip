@@ -627,7 +627,7 @@ static void alloc_work(sfmba_problem& H, Work<T>& W, double* shared_S = nullptr,
   if (L.D == 0) return;
   W.chain = A.alloc<uint32_t>(L.Dp / NB);
   HIP_CHECK(hipMemsetAsync(W.chain, 0, (L.Dp / NB) * sizeof(uint32_t), s));
-  W.J4 = A.alloc<T>(4 * (size_t)L.n_rows);
+  W.J2 = A.alloc<T>(2 * ((size_t)L.n_rows + 8));
   W.E4 = A.alloc<T>(4 * (size_t)L.n_rows);
   W.F4 = A.alloc<T>(4 * (size_t)L.n_rows);
   W.obs_pc = A.alloc<T>(9 * (size_t)L.n_obs);
@@ -993,7 +993,7 @@ struct WinPlan {
   size_t obs_img, obs_pt, obs_xy, obs_inc, pt_ptr, inc_grp, inc_pt, inc_roff, inc_froff, pt_inc_ptr;
   size_t cam_of_img, cam_model, img_slot, cam_slot, cam_free, cam_nfree, grp_off, grp_dim, pairs, chunks;
   size_t state;  // poses[0], cams[0], pts[0], poses[1], cams[1], pts[1] (doubles)
-  size_t J4, E4, F4, obs_pc, obs_valid, pt_C, pt_g, pt_active, pt_cinv, pt_ok, S, delta;
+  size_t J2, E4, F4, obs_pc, obs_valid, pt_C, pt_g, pt_active, pt_cinv, pt_ok, S, delta;
 };
 
 struct Planner {
@@ -1109,7 +1109,7 @@ static void batch_prepare(WindowBatch<T>& B, const sfmba_input* inputs, size_t c
   for (size_t k = 0; k < nw; ++k) {
     const Pack& P = B.packs[B.win[k]];
     WinPlan& q = B.plan[k];
-    q.J4 = wk.take(sizeof(V4) * (size_t)P.n_rows);
+    q.J2 = wk.take(2 * sizeof(T) * ((size_t)P.n_rows + 8));
     q.E4 = wk.take(sizeof(V4) * (size_t)P.n_rows);
     q.F4 = wk.take(sizeof(V4) * (size_t)P.n_rows);
     q.obs_pc = wk.take(9 * sizeof(T) * (size_t)P.n_obs);
@@ -1187,7 +1187,7 @@ static void batch_prepare(WindowBatch<T>& B, const sfmba_input* inputs, size_t c
     v.grp_dim = reinterpret_cast<const uint32_t*>(at(q.grp_dim));
     v.pairs = reinterpret_cast<const uint2*>(at(q.pairs));
     v.chunks = reinterpret_cast<const uint4*>(at(q.chunks));
-    v.J4 = reinterpret_cast<V4*>(at(q.J4));
+    v.J2 = reinterpret_cast<typename View<T>::V2*>(at(q.J2));
     v.E4 = reinterpret_cast<V4*>(at(q.E4));
     v.F4 = reinterpret_cast<V4*>(at(q.F4));
     v.obs_pc = reinterpret_cast<T*>(at(q.obs_pc));

@@ -88,7 +88,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(schur_min_w
 // true for the others.
 template <typename T, bool SAME>
 __global__ __launch_bounds__(256) void k_schur_lds(View<T> v, double lambda, double* __restrict__ parts) {
-  constexpr int PER_WAVE = (SAME ? 4 : 2) * lds_batch_pairs<T>() * 32;
+  constexpr int PER_WAVE = lds_chunk_elems<T, SAME>();
   __shared__ T stage[4 * PER_WAVE];
   const uint32_t w = threadIdx.x >> 6;
   const uint32_t ci = blockIdx.x * 4 + w;
@@ -384,7 +384,7 @@ View<T> make_view(const Layout& L, Work<T>& W) {
   v.grp_dim = L.grp_dim;
   v.pairs = reinterpret_cast<const uint2*>(L.pairs);
   v.chunks = L.chunks;
-  v.J4 = reinterpret_cast<V4*>(W.J4);
+  v.J2 = reinterpret_cast<typename View<T>::V2*>(W.J2);
   v.E4 = reinterpret_cast<V4*>(W.E4);
   v.F4 = reinterpret_cast<V4*>(W.F4);
   v.obs_pc = W.obs_pc;

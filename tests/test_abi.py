@@ -36,6 +36,14 @@ def test_struct_sizes():
     assert C.sizeof(N.Input) == 17 * 8
 
 
+def test_header_constants_mirrored():
+    text = HEADER.read_text()
+    lam = re.search(r"#define SFMBA_MIXED_MIN_LAMBDA (\S+)", text).group(1)
+    assert float(lam) == N.MIXED_MIN_LAMBDA
+    lim = re.search(r"#define SFMBA_NOALLOC_MAX_OBSERVATIONS \(1ull << (\d+)\)", text).group(1)
+    assert 1 << int(lim) == N.NOALLOC_MAX_OBSERVATIONS
+
+
 def test_defaults_match_reference():
     lib = N.load_engine()
     p = N.Params()

@@ -78,10 +78,10 @@ def test_mixed_trace_switches_to_fp64_once(c0_scene):
     lam = out.stats["try_lambdas"]
     fp32 = (oc & N.TRY_FP32) != 0
     assert len(oc) == out.stats["num_factorizations"] and fp32.sum() == out.stats["num_fp32_tries"]
-    # FP32 tries come first, at damping >= 1e-7; every later try is FP64
+    # FP32 tries come first, at damping >= SFMBA_MIXED_MIN_LAMBDA; every later try is FP64
     k = int(fp32.sum())
     assert fp32[:k].all() and not fp32[k:].any()
-    assert np.all(lam[:k] >= 1e-7)
+    assert np.all(lam[:k] >= N.MIXED_MIN_LAMBDA)
     f64 = ba.bundle_adjust(c0_scene.inp, ba.BaParams(max_iterations=30), precision="fp64")
     assert out.iterations_run == f64.iterations_run
     assert abs(plain_rms(c0_scene.inp, out) - plain_rms(c0_scene.inp, f64)) < 1e-4
