@@ -596,12 +596,13 @@ static void alloc_pcg(sfmba_problem& H) {
   B.row_grp = A.upload(row_grp, H.s);
   const size_t n = std::max<uint32_t>(L.Dp, 1);
   B.r = A.alloc<double>(n);
+  B.r2 = A.alloc<double>(n);
   B.z = A.alloc<double>(n);
   B.p = A.alloc<double>(n);
   B.q = A.alloc<double>(n);
   B.symv_part = A.alloc<double>(std::max<size_t>(pcg_symv_part_doubles(L.D), 1));
   B.dot_part = A.alloc<double>(pcg_dot_part_doubles(L.D));
-  B.scal = A.alloc<double>(4);
+  B.scal = A.alloc<double>(16);
   HIP_CHECK(hipHostMalloc((void**)&B.host, 8 * sizeof(double), hipHostMallocDefault));
   HIP_CHECK(hipStreamSynchronize(H.s));
   H.have_pcg = true;

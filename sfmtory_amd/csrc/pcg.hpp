@@ -12,11 +12,11 @@ struct PcgBuffers {
   double* Minv = nullptr;      // inverses of the diagonal parameter blocks (sum of d^2)
   uint32_t* moff = nullptr;    // per block: offset into Minv
   uint32_t* row_grp = nullptr; // per unknown: its parameter block
-  double *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
+  double *r = nullptr, *r2 = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
   double* symv_part = nullptr; // pcg_symv_part_doubles(D)
   double* dot_part = nullptr;  // pcg_dot_part_doubles(D)
-  double* scal = nullptr;      // 2 device scalars
-  double* host = nullptr;      // 4 pinned host scalars
+  double* scal = nullptr;      // 16 doubles: 2 dot products, then the iteration's device state
+  double* host = nullptr;      // 8 pinned host doubles
 };
 
 struct PcgResult {
@@ -28,8 +28,9 @@ struct PcgResult {
 size_t pcg_symv_part_doubles(uint32_t D);
 size_t pcg_dot_part_doubles(uint32_t D);
 
-// Solves S x = b (b = row Dp of S) to |r| <= tolerance |b| or max_iterations,
-// with one host synchronization per half-iteration.  x has D entries.
+// Solves S x = b (b = row Dp of S) to |r| <= tolerance |b| or max_iterations.
+// The iterations run on a device-side state; the host reads it once per
+// batch of iterations.  x has D entries.
 PcgResult pcg_solve(const PcgBuffers& B, const double* S, size_t ld, uint32_t D, uint32_t Dp,
                     uint32_t n_grp, const uint32_t* grp_off, const uint32_t* grp_dim, int max_iterations,
                     double tolerance, double* x, int* flags, double* host_scal, hipStream_t s);

@@ -15,14 +15,22 @@
 //                    one wavefront reduction per column, the four
 //                    wavefronts combined in a fixed order, per-row-block
 //                    partials)
-//   k_symv_finish    y = the row partials in chunk order + the column
-//                    partials in row-block order
+//   k_symv_finish_dot
+//                    y = the row partials in chunk order + the column
+//                    partials in row-block order, and block partials of p.y
 //   k_dot2 / k_dot2_final
 //                    two dot products, deterministic two-level reduction
+// The iteration runs without the host: alpha, beta, the convergence test
+// and the breakdown test live in a device-side state (PcgState) that every
+// iteration kernel reads; once it is no longer running the remaining
+// enqueued kernels return at once.  The host enqueues iterations in
+// batches and reads the state once per batch.
 // Every reduction has a fixed order, so a solve is reproducible.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
+#include <cstring>
 
 #include "engine.hpp"
 #include "pcg.hpp"
@@ -30,7 +38,14 @@
 namespace sfmba {
 
 constexpr int SYMV_ROWS = 256;
-constexpr int SYMV_COLS = 1024;
+// Column chunk of a matvec tile: 1,024 columns above SYMV_SMALL_D unknowns;
+// below, 256, so that a system of a few thousand unknowns still spreads
+// over enough workgroups (D = 7,994: 528 tiles of 0.5 MB instead of 144 of
+// 2 MB, which left most CUs idle).
+constexpr int SYMV_COLS_LARGE = 1024;
+constexpr int SYMV_COLS_SMALL = 256;
+constexpr uint32_t SYMV_SMALL_D = 16384;
+static inline int symv_cols(uint32_t D) { return D > SYMV_SMALL_D ? SYMV_COLS_LARGE : SYMV_COLS_SMALL; }
 constexpr int DOT_BLOCK = 256;
 constexpr int DOT_ITEMS = 8;
 
@@ -92,11 +107,14 @@ __global__ __launch_bounds__(256) void k_bj_apply(uint32_t D, const uint32_t* __
 // pcol[rb * D + j]: column j's products over the rows > j of row block rb.
 // Tiles right of the diagonal are skipped; every partial the finish reads
 // belongs to a tile that is not.
+template <int SYMV_COLS>
 __global__ __launch_bounds__(SYMV_ROWS) void k_symv_fused(uint32_t D, const double* __restrict__ S, size_t ld,
                                                           const double* __restrict__ x,
-                                                          double* __restrict__ prow, double* __restrict__ pcol) {
+                                                          double* __restrict__ prow, double* __restrict__ pcol,
+                                                          const int* __restrict__ stopped) {
   __shared__ double xs[SYMV_COLS];
   __shared__ double cs[SYMV_ROWS / 64][SYMV_COLS];
+  if (stopped && *stopped != 0) return;  // the device-driven iteration has finished
   const uint32_t rb = blockIdx.x, cc = blockIdx.y;
   const uint32_t r0 = rb * SYMV_ROWS, c0 = cc * SYMV_COLS;
   if (c0 > r0 + SYMV_ROWS - 1) return;  // chunk right of the diagonal: nothing
@@ -119,12 +137,25 @@ __global__ __launch_bounds__(SYMV_ROWS) void k_symv_fused(uint32_t D, const doub
   // of the eight by recursive halving (lanes l and l ^ 32 / 16 / 8 split
   // and add the eight values, then xor 4 / 2 / 1 finish the sum): lane l
   // ends with column 4 b5 + 2 b4 + b3 (bits of l) summed over the wavefront.
-  for (uint32_t j0 = c0; j0 < jend; j0 += 8) {
-    double v[8];
+  // The next eight columns' loads are issued before this group's sums.
+  auto load8 = [&](uint32_t j0, double (&a)[8]) {
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
       const uint32_t j = j0 + k;
-      const double a = (row_ok && j < jend && j <= i) ? S[(size_t)j * ld + i] : 0.0;
+      a[k] = (row_ok && j < jend && j <= i) ? S[(size_t)j * ld + i] : 0.0;
+    }
+  };
+  double an[8];
+  if (c0 < jend) load8(c0, an);
+  for (uint32_t j0 = c0; j0 < jend; j0 += 8) {
+    double v[8], a8[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) a8[k] = an[k];
+    if (j0 + 8 < jend) load8(j0 + 8, an);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const uint32_t j = j0 + k;
+      const double a = a8[k];
       acc += a * (j < jend ? xs[j - c0] : 0.0);
       v[k] = (j < i) ? a * xi : 0.0;
     }
@@ -159,18 +190,6 @@ __global__ __launch_bounds__(SYMV_ROWS) void k_symv_fused(uint32_t D, const doub
     for (int q = 0; q < SYMV_ROWS / 64; ++q) v += cs[q][j];
     pcol[(size_t)rb * D + c0 + j] = v;
   }
-}
-
-__global__ __launch_bounds__(256) void k_symv_finish(uint32_t D, const double* __restrict__ prow,
-                                                     const double* __restrict__ pcol, double* __restrict__ y) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= D) return;
-  const uint32_t ncc = i / SYMV_COLS + 1;  // chunks left of / at the diagonal
-  double s = 0.0;
-  for (uint32_t cc = 0; cc < ncc; ++cc) s += prow[(size_t)cc * D + i];
-  const uint32_t nrb = (D + SYMV_ROWS - 1) / SYMV_ROWS;
-  for (uint32_t rb = i / SYMV_ROWS; rb < nrb; ++rb) s += pcol[(size_t)rb * D + i];
-  y[i] = s;
 }
 
 // part[2 b], part[2 b + 1] = block b's sums of a.b and c.d.
@@ -228,23 +247,6 @@ __global__ __launch_bounds__(1024) void k_dot2_final(uint32_t n_part, const doub
   }
 }
 
-// x += alpha p, r -= alpha q
-__global__ __launch_bounds__(256) void k_pcg_xr(uint32_t n, double alpha, const double* __restrict__ p,
-                                                const double* __restrict__ q, double* __restrict__ x,
-                                                double* __restrict__ r) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  x[i] += alpha * p[i];
-  r[i] -= alpha * q[i];
-}
-
-// p = z + beta p
-__global__ __launch_bounds__(256) void k_pcg_p(uint32_t n, double beta, const double* __restrict__ z,
-                                               double* __restrict__ p) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = z[i] + beta * p[i];
-}
-
 // r = b (row Dp of S), x = 0
 __global__ __launch_bounds__(256) void k_pcg_init(uint32_t n, const double* __restrict__ S, size_t ld,
                                                   uint32_t Dp, double* __restrict__ r, double* __restrict__ x) {
@@ -254,18 +256,160 @@ __global__ __launch_bounds__(256) void k_pcg_init(uint32_t n, const double* __re
   x[i] = 0.0;
 }
 
+// ------------------------------------------------- device-driven iteration
+// status: 0 running, 1 converged, 2 breakdown (a block or p^T S p not
+// positive, or r.z not finite), 3 residual not finite, 4 zero right-hand
+// side.  rz[2]: r.z of the current iteration and of the next (by iteration
+// parity), so no kernel reads a value another block of it writes.
+struct PcgState {
+  double bn, rn;
+  double rz[2];
+  int status, iterations;
+};
+
+__device__ __forceinline__ bool pcg_stopped(const PcgState* st) { return st->status != 0; }
+
+// Block sum in a fixed order (256 threads).
+__device__ __forceinline__ double block_sum256(double v, double* red) {
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  const double t = red[0];
+  __syncthreads();
+  return t;
+}
+
+// Fixed-order sum of n block partials (stride apart), in every block that
+// needs it (each gets the same value): strided per thread, then a tree.
+__device__ __forceinline__ double sum_partials256(uint32_t n, const double* __restrict__ part, uint32_t stride,
+                                                  double* red) {
+  double a = 0.0;
+  for (uint32_t i = threadIdx.x; i < n; i += 256) a += part[(size_t)i * stride];
+  return block_sum256(a, red);
+}
+
+// After the initial residual: |b| and r.z from the two dot products, the
+// block-inverse breakdown flag, and p = z.
+__global__ __launch_bounds__(256) void k_pcg_start(uint32_t D, const double* __restrict__ dots,
+                                                   const int* __restrict__ flags, const double* __restrict__ z,
+                                                   double* __restrict__ p, PcgState* __restrict__ st) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < D) p[i] = z[i];
+  if (i == 0) {
+    st->bn = sqrt(dots[0]);
+    st->rn = st->bn;
+    st->rz[0] = dots[1];
+    st->rz[1] = 0.0;
+    st->iterations = 0;
+    st->status = flags[0] != 0 ? 2 : (st->bn == 0.0 ? 4 : 0);
+  }
+}
+
+// q = S p from the row and column partials, and block partials of p.q.
+__global__ __launch_bounds__(256) void k_symv_finish_dot(uint32_t D, uint32_t cols, const double* __restrict__ prow,
+                                                         const double* __restrict__ pcol,
+                                                         const double* __restrict__ p, double* __restrict__ q,
+                                                         double* __restrict__ part, const PcgState* __restrict__ st) {
+  __shared__ double red[256];
+  if (pcg_stopped(st)) return;
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  double pq = 0.0;
+  if (i < D) {
+    const uint32_t ncc = i / cols + 1;
+    double s = 0.0;
+    for (uint32_t cc = 0; cc < ncc; ++cc) s += prow[(size_t)cc * D + i];
+    const uint32_t nrb = (D + SYMV_ROWS - 1) / SYMV_ROWS;
+    for (uint32_t rb = i / SYMV_ROWS; rb < nrb; ++rb) s += pcol[(size_t)rb * D + i];
+    q[i] = s;
+    pq = p[i] * s;
+  }
+  const double t = block_sum256(pq, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// Iteration it, first half: alpha = r.z / p.q (every block sums the p.q
+// partials itself, in the same order), or a breakdown; x += alpha p,
+// r_out = r_in - alpha q, z = M^{-1} r_out (the block's rows of r_out
+// recomputed from r_in and q, so no other thread's write is read), and
+// block partials of r.r and r.z into part2.
+__global__ __launch_bounds__(256) void k_pcg_update(uint32_t D, int it, uint32_t n_part,
+                                                    const double* __restrict__ part_pq,
+                                                    const uint32_t* __restrict__ row_grp,
+                                                    const uint32_t* __restrict__ grp_off,
+                                                    const uint32_t* __restrict__ grp_dim,
+                                                    const uint32_t* __restrict__ moff, const double* __restrict__ Minv,
+                                                    const double* __restrict__ p, const double* __restrict__ q,
+                                                    const double* __restrict__ r_in, double* __restrict__ r_out,
+                                                    double* __restrict__ x, double* __restrict__ z,
+                                                    double* __restrict__ part2, PcgState* __restrict__ st) {
+  __shared__ double red[256];
+  if (pcg_stopped(st)) return;
+  const double pq = sum_partials256(n_part, part_pq, 1, red);
+  const double rz = st->rz[it & 1];
+  if (!(pq > 0.0) || !isfinite(rz)) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->status = 2;
+    return;
+  }
+  const double alpha = rz / pq;
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  double rr = 0.0, rzi = 0.0;
+  if (i < D) {
+    x[i] += alpha * p[i];
+    const double ri = r_in[i] - alpha * q[i];
+    r_out[i] = ri;
+    const uint32_t g = row_grp[i], off = grp_off[g], d = grp_dim[g];
+    const double* m = Minv + moff[g];
+    double zi = 0.0;
+    for (uint32_t c = 0; c < d; ++c) zi += m[c * d + (i - off)] * (r_in[off + c] - alpha * q[off + c]);
+    z[i] = zi;
+    rr = ri * ri;
+    rzi = ri * zi;
+  }
+  const double t0 = block_sum256(rr, red);
+  const double t1 = block_sum256(rzi, red);
+  if (threadIdx.x == 0) {
+    part2[2 * blockIdx.x] = t0;
+    part2[2 * blockIdx.x + 1] = t1;
+  }
+}
+
+// Iteration it, second half: the convergence test on |r|, then
+// beta = r_new.z_new / r.z and p = z + beta p (every block sums the
+// partials itself); block 0 records the state.
+__global__ __launch_bounds__(256) void k_pcg_next(uint32_t D, int it, uint32_t n_part,
+                                                  const double* __restrict__ part2, double tolerance,
+                                                  const double* __restrict__ z, double* __restrict__ p,
+                                                  PcgState* __restrict__ st) {
+  __shared__ double red[256];
+  if (pcg_stopped(st)) return;
+  const double rr = sum_partials256(n_part, part2, 2, red);
+  const double rz_new = sum_partials256(n_part, part2 + 1, 2, red);
+  const double rn = sqrt(rr);
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  if (lead) {
+    st->iterations = it + 1;
+    st->rn = rn;
+  }
+  if (!isfinite(rn) || rn <= tolerance * st->bn) {
+    if (lead) st->status = isfinite(rn) ? 1 : 3;
+    return;
+  }
+  const double beta = rz_new / st->rz[it & 1];
+  if (lead) st->rz[(it + 1) & 1] = rz_new;
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < D) p[i] = z[i] + beta * p[i];
+}
+
 static inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
-size_t pcg_symv_part_doubles(uint32_t D) { return ((size_t)cdiv(D, SYMV_COLS) + cdiv(D, SYMV_ROWS)) * D; }
-size_t pcg_dot_part_doubles(uint32_t D) { return 2 * (size_t)cdiv(D, DOT_BLOCK * DOT_ITEMS) + 2; }
-
-static void symv(const PcgBuffers& B, const double* S, size_t ld, uint32_t D, const double* x, double* y,
-                 hipStream_t s) {
-  double* prow = B.symv_part;
-  double* pcol = B.symv_part + (size_t)cdiv(D, SYMV_COLS) * D;
-  hipLaunchKernelGGL(k_symv_fused, dim3(cdiv(D, SYMV_ROWS), cdiv(D, SYMV_COLS)), dim3(SYMV_ROWS), 0, s, D, S,
-                     ld, x, prow, pcol);
-  hipLaunchKernelGGL(k_symv_finish, dim3(cdiv(D, 256)), dim3(256), 0, s, D, prow, pcol, y);
+size_t pcg_symv_part_doubles(uint32_t D) { return ((size_t)cdiv(D, symv_cols(D)) + cdiv(D, SYMV_ROWS)) * D; }
+size_t pcg_dot_part_doubles(uint32_t D) {
+  // the dot products' block partials, and three per 256 unknowns for the
+  // iteration's own block partials (p.q, then r.r and r.z)
+  return std::max<size_t>(2 * (size_t)cdiv(D, DOT_BLOCK * DOT_ITEMS) + 2, 3 * (size_t)cdiv(D, 256));
 }
 
 static void dot2(const PcgBuffers& B, uint32_t n, const double* a, const double* b, const double* c,
@@ -275,6 +419,9 @@ static void dot2(const PcgBuffers& B, uint32_t n, const double* a, const double*
   hipLaunchKernelGGL(k_dot2_final, dim3(1), dim3(1024), 0, s, nb, B.dot_part, B.scal);
 }
 
+// Iterations enqueued between two reads of the device state.
+constexpr int PCG_BATCH = 16;
+
 PcgResult pcg_solve(const PcgBuffers& B, const double* S, size_t ld, uint32_t D, uint32_t Dp,
                     uint32_t n_grp, const uint32_t* grp_off, const uint32_t* grp_dim, int max_iterations,
                     double tolerance, double* x, int* flags, double* host_scal, hipStream_t s) {
@@ -283,7 +430,13 @@ PcgResult pcg_solve(const PcgBuffers& B, const double* S, size_t ld, uint32_t D,
     res.ok = true;
     return res;
   }
+  static_assert(sizeof(PcgState) <= 8 * sizeof(double), "PCG state in B.scal + 8");
+  PcgState* st = reinterpret_cast<PcgState*>(B.scal + 8);
+  const int* stopped = &st->status;
   const uint32_t g256 = cdiv(D, 256);
+  double* prow = B.symv_part;
+  const int cols = symv_cols(D);
+  double* pcol = B.symv_part + (size_t)cdiv(D, cols) * D;
   HIP_CHECK(hipMemsetAsync(flags, 0, sizeof(int), s));
   hipLaunchKernelGGL(k_bj_invert, dim3(cdiv(n_grp, 4)), dim3(256), 0, s, n_grp, grp_off, grp_dim, B.moff, S,
                      ld, B.Minv, flags);
@@ -291,47 +444,42 @@ PcgResult pcg_solve(const PcgBuffers& B, const double* S, size_t ld, uint32_t D,
   hipLaunchKernelGGL(k_bj_apply, dim3(g256), dim3(256), 0, s, D, B.row_grp, grp_off, grp_dim, B.moff, B.Minv,
                      B.r, B.z);
   dot2(B, D, B.r, B.r, B.r, B.z, s);  // |b|^2, r.z
-  HIP_CHECK(hipMemcpyAsync(host_scal, B.scal, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
-  HIP_CHECK(hipMemcpyAsync(host_scal + 2, flags, sizeof(int), hipMemcpyDeviceToHost, s));
-  HIP_CHECK(hipStreamSynchronize(s));
-  if (reinterpret_cast<const int*>(host_scal + 2)[0] != 0) return res;  // a block is not positive definite
-  const double bn = sqrt(host_scal[0]);
-  double rz = host_scal[1];
-  if (bn == 0.0) {
-    res.ok = true;
-    return res;
-  }
-  HIP_CHECK(hipMemcpyAsync(B.p, B.z, (size_t)D * sizeof(double), hipMemcpyDeviceToDevice, s));
-  for (int it = 0; it < max_iterations; ++it) {
-    symv(B, S, ld, D, B.p, B.q, s);
-    dot2(B, D, B.p, B.q, nullptr, nullptr, s);
-    HIP_CHECK(hipMemcpyAsync(host_scal, B.scal, sizeof(double), hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipStreamSynchronize(s));
-    const double pq = host_scal[0];
-    if (!(pq > 0.0) || !std::isfinite(rz)) return res;  // breakdown: not positive definite
-    const double alpha = rz / pq;
-    hipLaunchKernelGGL(k_pcg_xr, dim3(g256), dim3(256), 0, s, D, alpha, B.p, B.q, x, B.r);
-    hipLaunchKernelGGL(k_bj_apply, dim3(g256), dim3(256), 0, s, D, B.row_grp, grp_off, grp_dim, B.moff,
-                       B.Minv, B.r, B.z);
-    dot2(B, D, B.r, B.r, B.r, B.z, s);
-    HIP_CHECK(hipMemcpyAsync(host_scal, B.scal, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipStreamSynchronize(s));
-    res.iterations = it + 1;
-    const double rn = sqrt(host_scal[0]);
-    res.rel_residual = rn / bn;
-    if (!std::isfinite(rn)) return res;
-    if (rn <= tolerance * bn) {
-      res.ok = true;
-      return res;
+  hipLaunchKernelGGL(k_pcg_start, dim3(g256), dim3(256), 0, s, D, B.scal, flags, B.z, B.p, st);
+  HIP_CHECK(hipGetLastError());
+  PcgState h{};
+  int it = 0;
+  while (it < max_iterations) {
+    const int n = std::min(PCG_BATCH, max_iterations - it);
+    for (int k = 0; k < n; ++k, ++it) {
+      // r alternates between two buffers (the update reads one, writes the other)
+      const double* r_in = (it & 1) ? B.r2 : B.r;
+      double* r_out = (it & 1) ? B.r : B.r2;
+      const dim3 grid(cdiv(D, SYMV_ROWS), cdiv(D, cols));
+      if (cols == SYMV_COLS_LARGE)
+        hipLaunchKernelGGL(k_symv_fused<SYMV_COLS_LARGE>, grid, dim3(SYMV_ROWS), 0, s, D, S, ld, B.p, prow, pcol,
+                           stopped);
+      else
+        hipLaunchKernelGGL(k_symv_fused<SYMV_COLS_SMALL>, grid, dim3(SYMV_ROWS), 0, s, D, S, ld, B.p, prow, pcol,
+                           stopped);
+      hipLaunchKernelGGL(k_symv_finish_dot, dim3(g256), dim3(256), 0, s, D, (uint32_t)cols, prow, pcol, B.p, B.q,
+                         B.dot_part, st);
+      hipLaunchKernelGGL(k_pcg_update, dim3(g256), dim3(256), 0, s, D, it, g256, B.dot_part, B.row_grp, grp_off,
+                         grp_dim, B.moff, B.Minv, B.p, B.q, r_in, r_out, x, B.z, B.dot_part + g256, st);
+      hipLaunchKernelGGL(k_pcg_next, dim3(g256), dim3(256), 0, s, D, it, g256, B.dot_part + g256, tolerance, B.z,
+                         B.p, st);
     }
-    const double rz_new = host_scal[1];
-    const double beta = rz_new / rz;
-    rz = rz_new;
-    hipLaunchKernelGGL(k_pcg_p, dim3(g256), dim3(256), 0, s, D, beta, B.z, B.p);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpyAsync(host_scal, st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    std::memcpy(&h, host_scal, sizeof(h));
+    if (h.status != 0) break;
   }
-  // iteration cap: the step so far is still a descent step of the damped
-  // model; it is used as is
-  res.ok = true;
+  res.iterations = h.iterations;
+  res.rel_residual = h.bn > 0.0 ? h.rn / h.bn : 0.0;
+  // converged, zero right-hand side, or the iteration cap (the step so far
+  // is still a descent step of the damped model, used as is); a breakdown
+  // or a non-finite residual gives no step
+  res.ok = h.status == 0 || h.status == 1 || h.status == 4;
   return res;
 }
 
