@@ -151,6 +151,10 @@ int sfmba_bundle_adjust(const sfmba_input* input, const sfmba_params* params,
 #define SFMBA_MIXED_MIN_LAMBDA 1e-5 /* mixed: smallest damping assembled in FP32 */
 #define SFMBA_SOLVER_CHOLESKY 0 /* dense FP64 MFMA Cholesky (LU fallback) */
 #define SFMBA_SOLVER_PCG 1      /* block-Jacobi PCG on the reduced system */
+#define SFMBA_SOLVER_PCG_MIXED 2 /* block-Jacobi PCG whose iterations read an FP32
+                                    copy of the reduced system, with the true
+                                    residual recomputed in FP64 every 16
+                                    iterations and at convergence */
 
 typedef struct sfmba_options {
   int32_t precision;  /* SFMBA_PRECISION_* */

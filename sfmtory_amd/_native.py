@@ -24,6 +24,7 @@ MIXED_MIN_LAMBDA = 1e-5  # SFMBA_MIXED_MIN_LAMBDA
 PRECISION_MIXED = 1
 SOLVER_CHOLESKY = 0
 SOLVER_PCG = 1
+SOLVER_PCG_MIXED = 2
 INTRINSICS_FIXED = 0
 INTRINSICS_FREE = 1
 INTRINSICS_FREE_GUARDED = 2

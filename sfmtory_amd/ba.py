@@ -238,7 +238,7 @@ def _c_params(p: BaParams) -> N.Params:
 
 
 PRECISIONS = {"fp64": N.PRECISION_FP64, "mixed": N.PRECISION_MIXED}
-SOLVERS = {"cholesky": N.SOLVER_CHOLESKY, "pcg": N.SOLVER_PCG}
+SOLVERS = {"cholesky": N.SOLVER_CHOLESKY, "pcg": N.SOLVER_PCG, "pcg_mixed": N.SOLVER_PCG_MIXED}
 
 
 def _options(precision, solver, device, profile, refine_steps=None, pcg_max_iterations=None,

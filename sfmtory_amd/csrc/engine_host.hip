@@ -578,9 +578,17 @@ static sfmba_problem* create_problem(const sfmba_input* in, int device) {
   return H.release();
 }
 
-// PCG work buffers, allocated on the first PCG solve of a problem.
-static void alloc_pcg(sfmba_problem& H) {
-  if (H.have_pcg) return;
+// PCG work buffers, allocated on the first PCG solve of a problem (the FP32
+// copy of the reduced system on the first mixed-storage one).
+static void alloc_pcg(sfmba_problem& H, bool mixed_storage) {
+  if (H.have_pcg) {
+    if (mixed_storage && !H.pcg.S32) {
+      const Layout& L = H.pr.L;
+      H.pcg.S32 = reinterpret_cast<float*>(H.pr.arena.alloc<double>(((size_t)L.ld * L.Dp + 1) / 2));
+      H.pcg.x_best = H.pr.arena.alloc<double>(std::max<uint32_t>(L.Dp, 1));
+    }
+    return;
+  }
   const Layout& L = H.pr.L;
   DevArena& A = H.pr.arena;
   PcgBuffers& B = H.pcg;
@@ -603,6 +611,10 @@ static void alloc_pcg(sfmba_problem& H) {
   B.symv_part = A.alloc<double>(std::max<size_t>(pcg_symv_part_doubles(L.D), 1));
   B.dot_part = A.alloc<double>(pcg_dot_part_doubles(L.D));
   B.scal = A.alloc<double>(16);
+  if (mixed_storage) {
+    B.S32 = reinterpret_cast<float*>(A.alloc<double>(((size_t)L.ld * L.Dp + 1) / 2));
+    B.x_best = A.alloc<double>(n);
+  }
   HIP_CHECK(hipHostMalloc((void**)&B.host, 8 * sizeof(double), hipHostMallocDefault));
   HIP_CHECK(hipStreamSynchronize(H.s));
   H.have_pcg = true;
@@ -683,7 +695,7 @@ struct Stages {
     const Layout& L = H.pr.L;
     refine = sizeof(T) == 4 && opt->refine_steps > 0 && L.D;
     if (refine && !W.S_copy) W.S_copy = H.pr.arena.alloc<double>((size_t)L.ld * L.Dp);
-    pcg = opt->solver == SFMBA_SOLVER_PCG;
+    pcg = opt->solver == SFMBA_SOLVER_PCG || opt->solver == SFMBA_SOLVER_PCG_MIXED;
   }
   // State buffer 0 <- the given device arrays.
   void load(const double* poses, const double* cams, const double* pts) {
@@ -740,7 +752,8 @@ struct Stages {
       // Iterative solve of the damped reduced system (no factorization).
       prof.begin(3);
       const PcgResult pr = pcg_solve(H.pcg, W.S, L.ld, L.D, L.Dp, L.n_grp, L.grp_off, L.grp_dim,
-                                     opt->pcg_max_iterations, opt->pcg_tolerance, W.delta, W.flags, H.pcg.host, s);
+                                     opt->pcg_max_iterations, opt->pcg_tolerance, W.delta, W.flags, H.pcg.host, s,
+                                     opt->solver == SFMBA_SOLVER_PCG_MIXED);
       prof.end();
       if (st) st->num_pcg_iterations += pr.iterations;
       return pr.ok;
@@ -815,7 +828,7 @@ static void solve(sfmba_problem& H, const sfmba_params* prm, const sfmba_options
   const Layout& L = H.pr.L;
   hipStream_t s = H.s;
   const bool mixed = opt->precision == SFMBA_PRECISION_MIXED;
-  if (opt->solver == SFMBA_SOLVER_PCG && L.D) alloc_pcg(H);
+  if (opt->solver != SFMBA_SOLVER_CHOLESKY && L.D) alloc_pcg(H, opt->solver == SFMBA_SOLVER_PCG_MIXED);
   // FP64 work always (the mixed form finishes in FP64); FP32 work for the
   // mixed form, sharing S and the factor workspace.
   if (!H.have64) {
@@ -879,7 +892,7 @@ static void solve(sfmba_problem& H, const sfmba_params* prm, const sfmba_options
           bits = SFMBA_TRY_FP32;
           ok = s32.try_step(cur, lambda, trial, front, bits);
           if (st) st->num_factorizations++;
-          if (!ok && opt->solver != SFMBA_SOLVER_PCG) {
+          if (!ok && opt->solver == SFMBA_SOLVER_CHOLESKY) {
             // the FP32-assembled system did not factor: this damping again
             // with FP64 assembly
             push_try(st, lambda, SFMBA_TRY_NO_STEP | bits);
@@ -960,9 +973,9 @@ static void solve_problem(sfmba_problem* H, const sfmba_params* params, const sf
   sfmba_options_default(&o);
   if (options) o = *options;
   if (stats) reset_stats(stats);  // keeps the caller's array pointers and capacities
-  if (o.solver != SFMBA_SOLVER_CHOLESKY && o.solver != SFMBA_SOLVER_PCG)
+  if (o.solver != SFMBA_SOLVER_CHOLESKY && o.solver != SFMBA_SOLVER_PCG && o.solver != SFMBA_SOLVER_PCG_MIXED)
     throw Error(SFMBA_ERR_INVALID_ARGUMENT, "unknown solver");
-  if (o.solver == SFMBA_SOLVER_PCG && !(o.pcg_max_iterations > 0 && o.pcg_tolerance >= 0.0))
+  if (o.solver != SFMBA_SOLVER_CHOLESKY && !(o.pcg_max_iterations > 0 && o.pcg_tolerance >= 0.0))
     throw Error(SFMBA_ERR_INVALID_ARGUMENT, "PCG needs pcg_max_iterations > 0 and pcg_tolerance >= 0");
   if (o.precision != SFMBA_PRECISION_MIXED && o.precision != SFMBA_PRECISION_FP64)
     throw Error(SFMBA_ERR_INVALID_ARGUMENT, "unknown precision");
@@ -2146,7 +2159,7 @@ int sfmba_bundle_adjust_batch(const sfmba_input* inputs, size_t count, const sfm
     if (options) o = *options;
     if (o.precision != SFMBA_PRECISION_FP64 && o.precision != SFMBA_PRECISION_MIXED)
       throw Error(SFMBA_ERR_INVALID_ARGUMENT, "unknown precision");
-    if (o.solver == SFMBA_SOLVER_PCG)
+    if (o.solver != SFMBA_SOLVER_CHOLESKY)
       throw Error(SFMBA_ERR_UNSUPPORTED, "batched windows are solved by Cholesky only");
     for (size_t i = 0; i < count; ++i) {
       const sfmba_output& out = outputs[i];
@@ -2176,7 +2189,7 @@ int sfmba_batch_create(const sfmba_input* inputs, size_t count, const sfmba_para
     if (options) o = *options;
     if (o.precision != SFMBA_PRECISION_FP64 && o.precision != SFMBA_PRECISION_MIXED)
       throw Error(SFMBA_ERR_INVALID_ARGUMENT, "unknown precision");
-    if (o.solver == SFMBA_SOLVER_PCG)
+    if (o.solver != SFMBA_SOLVER_CHOLESKY)
       throw Error(SFMBA_ERR_UNSUPPORTED, "batched windows are solved by Cholesky only");
     std::unique_ptr<sfmba_batch> Q(new sfmba_batch());
     {

@@ -192,6 +192,99 @@ __global__ __launch_bounds__(SYMV_ROWS) void k_symv_fused(uint32_t D, const doub
   }
 }
 
+// The same product with an FP32 matrix (FP64 products and sums): two rows
+// per lane, so each load moves 8 bytes a lane as in the FP64 kernel, and
+// half the lanes per row block (two wavefronts).  Same partial layout as
+// k_symv_fused.  ld is even (a float2 at an even row is aligned); row
+// D (ld > D) may be read and is discarded.
+template <int SYMV_COLS>
+__global__ __launch_bounds__(SYMV_ROWS / 2) void k_symv_fused32(uint32_t D, const float* __restrict__ S, size_t ld,
+                                                                const double* __restrict__ x,
+                                                                double* __restrict__ prow,
+                                                                double* __restrict__ pcol,
+                                                                const int* __restrict__ stopped) {
+  constexpr int NT = SYMV_ROWS / 2, NW = NT / 64;
+  __shared__ double xs[SYMV_COLS];
+  __shared__ double cs[NW][SYMV_COLS];
+  if (stopped && *stopped != 0) return;  // the device-driven iteration has finished
+  const uint32_t rb = blockIdx.x, cc = blockIdx.y;
+  const uint32_t r0 = rb * SYMV_ROWS, c0 = cc * SYMV_COLS;
+  if (c0 > r0 + SYMV_ROWS - 1) return;  // chunk right of the diagonal: nothing
+  const uint32_t c1 = min(D, c0 + SYMV_COLS);
+  const uint32_t nc = c1 - c0;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  for (uint32_t j = t; j < nc; j += NT) xs[j] = x[c0 + j];
+  for (uint32_t j = t; j < SYMV_COLS; j += NT)
+#pragma unroll
+    for (int q = 0; q < NW; ++q) cs[q][j] = 0.0;
+  __syncthreads();
+  const uint32_t i0 = r0 + 2 * t, i1 = i0 + 1;
+  const bool ok0 = i0 < D, ok1 = i1 < D;
+  const double x0 = ok0 ? x[i0] : 0.0, x1 = ok1 ? x[i1] : 0.0;
+  const uint32_t wlast = r0 + 128 * w + 127;
+  const uint32_t jend = min(c1, wlast + 1);
+  double acc0 = 0.0, acc1 = 0.0;
+  auto load8 = [&](uint32_t j0, float2 (&a)[8]) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const uint32_t j = j0 + k;
+      a[k] = (ok0 && j < jend && j <= i1) ? *reinterpret_cast<const float2*>(S + (size_t)j * ld + i0)
+                                          : make_float2(0.f, 0.f);
+    }
+  };
+  float2 an[8];
+  if (c0 < jend) load8(c0, an);
+  for (uint32_t j0 = c0; j0 < jend; j0 += 8) {
+    double v[8];
+    float2 a8[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) a8[k] = an[k];
+    if (j0 + 8 < jend) load8(j0 + 8, an);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const uint32_t j = j0 + k;
+      const double a0 = j <= i0 ? (double)a8[k].x : 0.0;
+      const double a1 = ok1 ? (double)a8[k].y : 0.0;
+      const double xj = j < jend ? xs[j - c0] : 0.0;
+      acc0 += a0 * xj;
+      acc1 += a1 * xj;
+      v[k] = (j < i0 ? a0 * x0 : 0.0) + (j < i1 ? a1 * x1 : 0.0);
+    }
+    // column sums over the wavefront, as in k_symv_fused
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const bool up = lane & 32;
+      const double send = up ? v[k] : v[k + 4], keep = up ? v[k + 4] : v[k];
+      v[k] = keep + __shfl_xor(send, 32);
+    }
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const bool up = lane & 16;
+      const double send = up ? v[k] : v[k + 2], keep = up ? v[k + 2] : v[k];
+      v[k] = keep + __shfl_xor(send, 16);
+    }
+    {
+      const bool up = lane & 8;
+      const double send = up ? v[0] : v[1], keep = up ? v[1] : v[0];
+      v[0] = keep + __shfl_xor(send, 8);
+    }
+    v[0] += __shfl_xor(v[0], 4);
+    v[0] += __shfl_xor(v[0], 2);
+    v[0] += __shfl_xor(v[0], 1);
+    const uint32_t jc = j0 + ((lane >> 5) & 1) * 4 + ((lane >> 4) & 1) * 2 + ((lane >> 3) & 1);
+    if ((lane & 7) == 0 && jc < jend) cs[w][jc - c0] = v[0];
+  }
+  if (ok0) prow[(size_t)cc * D + i0] = acc0;
+  if (ok1) prow[(size_t)cc * D + i1] = acc1;
+  __syncthreads();
+  for (uint32_t j = t; j < nc; j += NT) {
+    double v = 0.0;
+#pragma unroll
+    for (int q = 0; q < NW; ++q) v += cs[q][j];
+    pcol[(size_t)rb * D + c0 + j] = v;
+  }
+}
+
 // part[2 b], part[2 b + 1] = block b's sums of a.b and c.d.
 __global__ __launch_bounds__(DOT_BLOCK) void k_dot2(uint32_t n, const double* __restrict__ a,
                                                     const double* __restrict__ b, const double* __restrict__ c,
@@ -257,13 +350,15 @@ __global__ __launch_bounds__(256) void k_pcg_init(uint32_t n, const double* __re
 }
 
 // ------------------------------------------------- device-driven iteration
-// status: 0 running, 1 converged, 2 breakdown (a block or p^T S p not
-// positive, or r.z not finite), 3 residual not finite, 4 zero right-hand
-// side.  rz[2]: r.z of the current iteration and of the next (by iteration
+// status: 0 running, 1 converged, 2 breakdown (p^T S p not positive, or
+// r.z not finite), 3 residual not finite, 4 zero right-hand side, 5 a
+// diagonal block not positive definite.  rz[2]: r.z of the current iteration and of the next (by iteration
 // parity), so no kernel reads a value another block of it writes.
 struct PcgState {
   double bn, rn;
   double rz[2];
+  double rn_rec;  // mixed storage, last replacement: the recurrence's |r| before it
+  double gap;     // and |r_true - r_recurrence|
   int status, iterations;
 };
 
@@ -304,7 +399,7 @@ __global__ __launch_bounds__(256) void k_pcg_start(uint32_t D, const double* __r
     st->rz[0] = dots[1];
     st->rz[1] = 0.0;
     st->iterations = 0;
-    st->status = flags[0] != 0 ? 2 : (st->bn == 0.0 ? 4 : 0);
+    st->status = flags[0] != 0 ? 5 : (st->bn == 0.0 ? 4 : 0);
   }
 }
 
@@ -403,13 +498,103 @@ __global__ __launch_bounds__(256) void k_pcg_next(uint32_t D, int it, uint32_t n
   if (i < D) p[i] = z[i] + beta * p[i];
 }
 
+// The FP32 copy of the lower triangle (rows >= column, below D) for the
+// mixed-storage matvec: columns strided over y, rows over x.
+__global__ __launch_bounds__(256) void k_lower_to_fp32(uint32_t D, const double* __restrict__ S, size_t ld,
+                                                       float* __restrict__ S32) {
+  for (uint32_t c = blockIdx.y; c < D; c += gridDim.y)
+    for (uint32_t r = c + blockIdx.x * 256 + threadIdx.x; r < D; r += gridDim.x * 256)
+      S32[(size_t)c * ld + r] = (float)S[(size_t)c * ld + r];
+}
+
+// Residual replacement of the mixed-storage iteration, before iteration
+// st->iterations: r = b - q with q = S x in FP64 (into the r buffer that
+// iteration reads, over the recurrence's), z = M^{-1} r, and block partials
+// of r.r, r.z and |r - r_recurrence|^2 (stride 3).  Runs while the
+// iteration is running or has just converged on its recurrence residual
+// (status 0 or 1).
+__global__ __launch_bounds__(256) void k_pcg_replace(uint32_t D, const double* __restrict__ S, size_t ld,
+                                                     uint32_t Dp, const double* __restrict__ q,
+                                                     const uint32_t* __restrict__ row_grp,
+                                                     const uint32_t* __restrict__ grp_off,
+                                                     const uint32_t* __restrict__ grp_dim,
+                                                     const uint32_t* __restrict__ moff,
+                                                     const double* __restrict__ Minv, double* __restrict__ r_even,
+                                                     double* __restrict__ r_odd, double* __restrict__ z,
+                                                     double* __restrict__ part3, const PcgState* __restrict__ st) {
+  __shared__ double red[256];
+  if (st->status > 1) return;
+  double* r = (st->iterations & 1) ? r_odd : r_even;
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  double rr = 0.0, rz = 0.0, gg = 0.0;
+  if (i < D) {
+    const double ri = S[(size_t)i * ld + Dp] - q[i];
+    gg = (ri - r[i]) * (ri - r[i]);
+    r[i] = ri;
+    const uint32_t g = row_grp[i], off = grp_off[g], d = grp_dim[g];
+    const double* m = Minv + moff[g];
+    double zi = 0.0;
+    for (uint32_t c = 0; c < d; ++c) zi += m[c * d + (i - off)] * (S[(size_t)(off + c) * ld + Dp] - q[off + c]);
+    z[i] = zi;
+    rr = ri * ri;
+    rz = ri * zi;
+  }
+  const double t0 = block_sum256(rr, red);
+  const double t1 = block_sum256(rz, red);
+  const double t2 = block_sum256(gg, red);
+  if (threadIdx.x == 0) {
+    part3[3 * blockIdx.x] = t0;
+    part3[3 * blockIdx.x + 1] = t1;
+    part3[3 * blockIdx.x + 2] = t2;
+  }
+}
+
+// After a replacement: the true residual decides convergence (a
+// recurrence-residual convergence is confirmed or undone); the recurrence's
+// |r| and its distance from the true residual are kept, and r.z of the next
+// iteration is the replaced residual's.
+__global__ __launch_bounds__(256) void k_pcg_replace_fin(uint32_t n_part, const double* __restrict__ part3,
+                                                         double tolerance, PcgState* __restrict__ st) {
+  __shared__ double red[256];
+  if (st->status > 1) return;
+  const double rr = sum_partials256(n_part, part3, 3, red);
+  const double rz = sum_partials256(n_part, part3 + 1, 3, red);
+  const double gg = sum_partials256(n_part, part3 + 2, 3, red);
+  if (threadIdx.x == 0) {
+    const double rn = sqrt(rr);
+    st->rn_rec = st->rn;
+    st->gap = sqrt(gg);
+    st->rn = rn;
+    st->rz[st->iterations & 1] = rz;
+    if (!isfinite(rn)) st->status = 3;
+    else st->status = rn <= tolerance * st->bn ? 1 : 0;
+  }
+}
+
+// The FP64 matvec S x for a replacement; runs unless the iteration broke
+// down (status > 1).
+__global__ __launch_bounds__(256) void k_symv_finish_plain(uint32_t D, uint32_t cols, const double* __restrict__ prow,
+                                                           const double* __restrict__ pcol, double* __restrict__ y,
+                                                           const PcgState* __restrict__ st) {
+  if (st->status > 1) return;
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= D) return;
+  const uint32_t ncc = i / cols + 1;
+  double s = 0.0;
+  for (uint32_t cc = 0; cc < ncc; ++cc) s += prow[(size_t)cc * D + i];
+  const uint32_t nrb = (D + SYMV_ROWS - 1) / SYMV_ROWS;
+  for (uint32_t rb = i / SYMV_ROWS; rb < nrb; ++rb) s += pcol[(size_t)rb * D + i];
+  y[i] = s;
+}
+
 static inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
 size_t pcg_symv_part_doubles(uint32_t D) { return ((size_t)cdiv(D, symv_cols(D)) + cdiv(D, SYMV_ROWS)) * D; }
 size_t pcg_dot_part_doubles(uint32_t D) {
-  // the dot products' block partials, and three per 256 unknowns for the
-  // iteration's own block partials (p.q, then r.r and r.z)
-  return std::max<size_t>(2 * (size_t)cdiv(D, DOT_BLOCK * DOT_ITEMS) + 2, 3 * (size_t)cdiv(D, 256));
+  // the dot products' block partials, and four per 256 unknowns for the
+  // iteration's own block partials (p.q, then r.r and r.z, or a
+  // replacement's r.r, r.z and gap)
+  return std::max<size_t>(2 * (size_t)cdiv(D, DOT_BLOCK * DOT_ITEMS) + 2, 4 * (size_t)cdiv(D, 256));
 }
 
 static void dot2(const PcgBuffers& B, uint32_t n, const double* a, const double* b, const double* c,
@@ -422,9 +607,29 @@ static void dot2(const PcgBuffers& B, uint32_t n, const double* a, const double*
 // Iterations enqueued between two reads of the device state.
 constexpr int PCG_BATCH = 16;
 
+static void symv_launch(uint32_t D, int cols, const double* S, size_t ld, const double* x, double* prow,
+                        double* pcol, const int* stopped, hipStream_t s) {
+  const dim3 grid(cdiv(D, SYMV_ROWS), cdiv(D, cols));
+  if (cols == SYMV_COLS_LARGE)
+    hipLaunchKernelGGL(k_symv_fused<SYMV_COLS_LARGE>, grid, dim3(SYMV_ROWS), 0, s, D, S, ld, x, prow, pcol, stopped);
+  else
+    hipLaunchKernelGGL(k_symv_fused<SYMV_COLS_SMALL>, grid, dim3(SYMV_ROWS), 0, s, D, S, ld, x, prow, pcol, stopped);
+}
+
+static void symv_launch(uint32_t D, int cols, const float* S, size_t ld, const double* x, double* prow,
+                        double* pcol, const int* stopped, hipStream_t s) {
+  const dim3 grid(cdiv(D, SYMV_ROWS), cdiv(D, cols));
+  if (cols == SYMV_COLS_LARGE)
+    hipLaunchKernelGGL(k_symv_fused32<SYMV_COLS_LARGE>, grid, dim3(SYMV_ROWS / 2), 0, s, D, S, ld, x, prow, pcol,
+                       stopped);
+  else
+    hipLaunchKernelGGL(k_symv_fused32<SYMV_COLS_SMALL>, grid, dim3(SYMV_ROWS / 2), 0, s, D, S, ld, x, prow, pcol,
+                       stopped);
+}
+
 PcgResult pcg_solve(const PcgBuffers& B, const double* S, size_t ld, uint32_t D, uint32_t Dp,
                     uint32_t n_grp, const uint32_t* grp_off, const uint32_t* grp_dim, int max_iterations,
-                    double tolerance, double* x, int* flags, double* host_scal, hipStream_t s) {
+                    double tolerance, double* x, int* flags, double* host_scal, hipStream_t s, bool fp32_matrix) {
   PcgResult res;
   if (D == 0) {
     res.ok = true;
@@ -445,8 +650,37 @@ PcgResult pcg_solve(const PcgBuffers& B, const double* S, size_t ld, uint32_t D,
                      B.r, B.z);
   dot2(B, D, B.r, B.r, B.r, B.z, s);  // |b|^2, r.z
   hipLaunchKernelGGL(k_pcg_start, dim3(g256), dim3(256), 0, s, D, B.scal, flags, B.z, B.p, st);
+  if (fp32_matrix)
+    hipLaunchKernelGGL(k_lower_to_fp32, dim3(std::min<uint32_t>(cdiv(D, 256), 64), std::min<uint32_t>(D, 16384)),
+                       dim3(256), 0, s, D, S, ld, B.S32);
   HIP_CHECK(hipGetLastError());
   PcgState h{};
+  auto read_state = [&]() {
+    HIP_CHECK(hipMemcpyAsync(host_scal, st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    std::memcpy(&h, host_scal, sizeof(h));
+  };
+  // mixed storage: the true residual b - S x, in FP64, replaces the
+  // recurrence's before the next iteration and decides convergence
+  auto replace = [&]() {
+    symv_launch(D, cols, S, ld, x, prow, pcol, nullptr, s);
+    hipLaunchKernelGGL(k_symv_finish_plain, dim3(g256), dim3(256), 0, s, D, (uint32_t)cols, prow, pcol, B.q, st);
+    hipLaunchKernelGGL(k_pcg_replace, dim3(g256), dim3(256), 0, s, D, S, ld, Dp, B.q, B.row_grp, grp_off, grp_dim,
+                       B.moff, B.Minv, B.r, B.r2, B.z, B.dot_part + g256, st);
+    hipLaunchKernelGGL(k_pcg_replace_fin, dim3(1), dim3(256), 0, s, g256, B.dot_part + g256, tolerance, st);
+    HIP_CHECK(hipGetLastError());
+  };
+  // the iterations read the FP32 copy, with a replacement after every
+  // batch, until its rounding dominates the residual (the replaced residual
+  // is more than half its distance from the recurrence's) or the true
+  // residual grows past 4x the smallest so far (the rounded matrix is not
+  // positive definite on what is left); then the plain FP64 iteration from
+  // the best iterate, converging on its recurrence residual like the FP64
+  // solver (a true residual can sit above a tight tolerance at FP64's own
+  // rounding floor)
+  bool use32 = fp32_matrix;
+  double best_rn = INFINITY;
+  const size_t xbytes = (size_t)D * sizeof(double);
   int it = 0;
   while (it < max_iterations) {
     const int n = std::min(PCG_BATCH, max_iterations - it);
@@ -454,13 +688,8 @@ PcgResult pcg_solve(const PcgBuffers& B, const double* S, size_t ld, uint32_t D,
       // r alternates between two buffers (the update reads one, writes the other)
       const double* r_in = (it & 1) ? B.r2 : B.r;
       double* r_out = (it & 1) ? B.r : B.r2;
-      const dim3 grid(cdiv(D, SYMV_ROWS), cdiv(D, cols));
-      if (cols == SYMV_COLS_LARGE)
-        hipLaunchKernelGGL(k_symv_fused<SYMV_COLS_LARGE>, grid, dim3(SYMV_ROWS), 0, s, D, S, ld, B.p, prow, pcol,
-                           stopped);
-      else
-        hipLaunchKernelGGL(k_symv_fused<SYMV_COLS_SMALL>, grid, dim3(SYMV_ROWS), 0, s, D, S, ld, B.p, prow, pcol,
-                           stopped);
+      if (use32) symv_launch(D, cols, B.S32, ld, B.p, prow, pcol, stopped, s);
+      else symv_launch(D, cols, S, ld, B.p, prow, pcol, stopped, s);
       hipLaunchKernelGGL(k_symv_finish_dot, dim3(g256), dim3(256), 0, s, D, (uint32_t)cols, prow, pcol, B.p, B.q,
                          B.dot_part, st);
       hipLaunchKernelGGL(k_pcg_update, dim3(g256), dim3(256), 0, s, D, it, g256, B.dot_part, B.row_grp, grp_off,
@@ -469,10 +698,44 @@ PcgResult pcg_solve(const PcgBuffers& B, const double* S, size_t ld, uint32_t D,
                          B.p, st);
     }
     HIP_CHECK(hipGetLastError());
-    HIP_CHECK(hipMemcpyAsync(host_scal, st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipStreamSynchronize(s));
-    std::memcpy(&h, host_scal, sizeof(h));
+    const bool replaced = use32;
+    if (replaced) replace();
+    read_state();
+    if (!replaced) {
+      if (h.status != 0) break;
+      continue;
+    }
+    // the iterations after a recurrence convergence or a breakdown did nothing
+    it = h.iterations;
+    bool restart = false;
+    if (use32 && (h.status == 2 || (h.status == 0 && h.rn > 4.0 * best_rn))) {
+      // p.Sp <= 0 with the rounded matrix, or a growing residual: back to
+      // the best iterate, on with the FP64 matrix
+      use32 = false;
+      restart = true;
+      if (h.status == 2) HIP_CHECK(hipMemsetAsync(&st->status, 0, sizeof(int), s));
+      if (best_rn < INFINITY) HIP_CHECK(hipMemcpyAsync(x, B.x_best, xbytes, hipMemcpyDeviceToDevice, s));
+      replace();
+      read_state();
+    } else if (h.status == 0) {
+      if (use32 && h.gap > 0.5 * h.rn) use32 = false;
+      // the direction is kept while the replaced residual stays close to
+      // the recurrence's; also restarted when the recurrence converged and
+      // the true residual did not (its last direction was not updated)
+      restart = h.gap > 0.25 * h.rn || h.rn_rec <= tolerance * h.bn;
+    }
     if (h.status != 0) break;
+    if (use32 && h.rn < best_rn) {
+      best_rn = h.rn;
+      HIP_CHECK(hipMemcpyAsync(B.x_best, x, xbytes, hipMemcpyDeviceToDevice, s));
+    }
+    if (restart) HIP_CHECK(hipMemcpyAsync(B.p, B.z, xbytes, hipMemcpyDeviceToDevice, s));
+  }
+  if (use32 && h.status == 0 && h.rn > best_rn) {
+    // the cap, still on the rounded matrix: the best iterate is the result
+    HIP_CHECK(hipMemcpyAsync(x, B.x_best, xbytes, hipMemcpyDeviceToDevice, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    h.rn = best_rn;
   }
   res.iterations = h.iterations;
   res.rel_residual = h.bn > 0.0 ? h.rn / h.bn : 0.0;

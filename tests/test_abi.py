@@ -42,6 +42,9 @@ def test_header_constants_mirrored():
     assert float(lam) == N.MIXED_MIN_LAMBDA
     lim = re.search(r"#define SFMBA_NOALLOC_MAX_OBSERVATIONS \(1ull << (\d+)\)", text).group(1)
     assert 1 << int(lim) == N.NOALLOC_MAX_OBSERVATIONS
+    for name in ("CHOLESKY", "PCG", "PCG_MIXED"):
+        v = re.search(rf"#define SFMBA_SOLVER_{name} (\d+)", text).group(1)
+        assert int(v) == getattr(N, f"SOLVER_{name}")
 
 
 def test_defaults_match_reference():

@@ -65,11 +65,14 @@ def test_c4_full_pcg_matches_cholesky():
     p = ba.BaParams(max_iterations=3)
     chol = prob.solve(p, precision="fp64", solver="cholesky")
     pcg = prob.solve(p, precision="fp64", solver="pcg")
+    m32 = prob.solve(p, precision="fp64", solver="pcg_mixed")
     prob.close()
     assert chol.stats["dim"] > 40000
     assert pcg.iterations_run == chol.iterations_run
     assert pcg.final_cost == pytest.approx(chol.final_cost, rel=1e-6)
     assert pcg.final_cost < 0.5 * pcg.initial_cost
+    assert m32.iterations_run == chol.iterations_run
+    assert m32.final_cost == pytest.approx(chol.final_cost, rel=1e-6)
 
 
 def test_mixed_trace_switches_to_fp64_once(c0_scene):
