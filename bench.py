@@ -387,6 +387,21 @@ def main() -> int:
     prof_mixed = prob.solve(params, precision="mixed", profile=True).stats
     prof_fp64 = prob.solve(params, precision="fp64", profile=True).stats
     log("profiled solves done")
+    # The same scene with the reduced system solved by PCG instead of the
+    # factorization (FP64 assembly; FP64 matrix, then the FP32 copy with FP64
+    # residual replacement): one untimed solve each, then one timed.
+    solvers = {}
+    for solver in ("pcg", "pcg_mixed"):
+        prob.solve(params, precision="fp64", solver=solver)
+        t = time.perf_counter()
+        out = prob.solve(params, precision="fp64", solver=solver)
+        wall = time.perf_counter() - t
+        st = out.stats
+        solvers[LARGE_KEYS[solver] + "_fp64"] = {
+            "time_to_converge_s": wall, "iterations": out.iterations_run, "tries": st["num_factorizations"],
+            "pcg_iterations": st["num_pcg_iterations"], "final_cost": out.final_cost}
+        log(f"{solver} fp64: {wall:.3f}s, {out.iterations_run} iterations, {st['num_pcg_iterations']} PCG "
+            f"iterations, cost {out.final_cost:.6f}")
 
     mixed_t = float(np.mean(mixed_walls))
     fp64_t = float(np.mean(fp64_walls))
@@ -526,6 +541,7 @@ def main() -> int:
         "final_cost": {"mixed": mixed_outs[-1].final_cost, "fp64": fp64_outs[-1].final_cost},
         "mixed": r_mixed,
         "fp64": r_fp64,
+        "solvers": solvers,
         "roofline": roof,
         "roofline_factorization": roof_fact,
         "roofline_schur": roof_schur,
