@@ -43,14 +43,14 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_syrk_update_hbm_traffic.txt")
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_syrk_persist_hbm_traffic.txt")
 ASSEMBLY_TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "assembly_hbm_traffic.txt")
 
 
 def measured_traffic(path=TRAFFIC_PROFILE, key="traffic_bytes_per_launch"):
     """HBM bytes per launch of this workload from a committed counter-run
     summary (FETCH_SIZE + WRITE_SIZE per dispatch, each scaled by the same
-    counter's reading for a 1 GiB copy): k_syrk_update by default, the
+    counter's reading for a 1 GiB copy): k_syrk_persist by default, the
     assembly kernels by key; None when the summary is absent."""
     try:
         with open(path) as f:
@@ -431,14 +431,14 @@ def main() -> int:
     r_mixed["fp32_schur_assembly_ms"] = per_unit(prof_mixed, "time_schur_fp32_s", "count_schur_fp32") * 1e3
     r_mixed["fp32_jac_obs_per_s"] = n_obs / (r_mixed["fp32_linearize_ms"] * 1e-3)
     r_mixed["fp32_tries"] = prof_mixed["num_fp32_tries"]
-    # Dominant kernel: the factorization's trailing update (k_syrk_update),
+    # Dominant kernel: the factorization's trailing update (k_syrk_persist),
     # timed per launch with HIP events on the solve stream; its algorithmic
     # FLOP per launch (lower-triangle update + rhs row + next diagonal block)
     # come from the library beside the times.
     n_tu = prof_fp64["count_trailing_update"]
     t_tu = prof_fp64["time_trailing_update_s"] / max(1, n_tu)
     f_tu = prof_fp64["flop_trailing_update"] / max(1, n_tu)
-    roof = {"bound": "mfma", "kernel": "k_syrk_update (FP64 trailing update + lookahead diagonal factorization)",
+    roof = {"bound": "mfma", "kernel": "k_syrk_persist (FP64 trailing update + lookahead diagonal factorization)",
             "achieved": f_tu / t_tu / 1e12, "peak": PEAK_FP64_MFMA / 1e12, "unit": "TFLOP/s",
             "frac": f_tu / t_tu / PEAK_FP64_MFMA, "traffic": measured_traffic(),
             "traffic_source": traffic_source(),

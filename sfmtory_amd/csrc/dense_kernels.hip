@@ -18,11 +18,12 @@
 //   k_trsm_panel    L_ik = A_ik L_kk^{-T} for every row block i > k (and
 //                   the rhs row) by blocked substitution with the X11, one
 //                   wavefront per 16-row strip, all in MFMA registers;
-//   k_syrk_update   A_ij -= L_ik L_jk^T on the trailing lower triangle,
-//                   128x128 tiles, v_mfma_f64_16x16x4_f64, panels staged
-//                   through LDS by global_load_lds (double buffered); its
-//                   first workgroup updates and factors the next diagonal
-//                   block.
+//   k_syrk_persist  A_ij -= L_ik L_jk^T on the trailing lower triangle,
+//                   128x128 tiles on v_mfma_f64_16x16x4_f64, one
+//                   eight-wavefront workgroup per CU walking its tiles,
+//                   panels staged through LDS by global_load_lds (double
+//                   buffered), C loaded while the products run; its first
+//                   workgroup updates and factors the next diagonal block.
 // While the trailing matrix is large, block columns go in pairs: the update
 // of column block k+1 alone, its factorization and panel solve, then one
 // update of the rest with both panels (K = 2 NB), which halves the trailing
@@ -47,8 +48,9 @@ constexpr int PW = 16;   // panel width inside a diagonal block
 // remain: the trailing lower triangle is then larger than the 256 MB MALL
 // (64 blocks of 128 doubles: 8192^2 / 2 x 8 bytes), so its HBM traffic, not
 // the chain of diagonal factorizations, bounds each step.  Measured on
-// MI355X: D = 44,994 factors in 560 ms in pairs vs 673 ms in single steps;
-// D = 7,994 (always below the threshold) in 7.0 ms vs 7.4 ms in pairs.
+// MI355X: D = 44,994 factors in 571 ms in pairs; D = 7,994 (always below the
+// threshold) in 6.7 ms in single steps against 6.9 - 7.4 ms with pairs from
+// 40 - 16 trailing block columns on.
 constexpr int PAIR_MIN_TRAIL = 64;
 // The diagonal block in LDS is stored as its 36 lower 16x16 tiles, tile
 // (ti, tj), ti >= tj, at (ti (ti + 1) / 2 + tj) * 256, each column-major
@@ -70,157 +72,100 @@ __device__ __attribute__((always_inline)) inline int doff(int r, int c) {
   return tile_index(r >> 4, c >> 4) * 256 + (c & 15) * 16 + (r & 15);
 }
 
-// acc(128x128 tile, 4 waves of 64x64) += sign * A(128 x K) * B(128 x K)^T,
-// A and B column-major (rows contiguous) with leading dims lda, ldb.
 // The v_mfma_f64_16x16x4_f64 operand layout: lane l supplies A[l%16][l/16]
 // and B^T[l/16][l%16]; result register v of lane l is D[l/16 + 4v][l%16]
 // (checked on gfx950 with a one-wavefront probe of the builtin).
-// Software-pipelined: the next K chunk moves from global memory to LDS
-// while the current chunk's MFMAs run (two LDS buffers, one barrier per
-// chunk).
 typedef __attribute__((address_space(3))) void lds_void;
 
-// One chunk (KC columns of 128 rows) of A and of B straight from global
-// memory into LDS (global_load_lds_dwordx4: lane l moves 16 bytes to
-// dst + 16 l), column k of the chunk to offset k * KS; wave w moves columns
-// w, w + 4, ...
-__device__ __attribute__((always_inline)) inline void chunk_to_lds(const double* __restrict__ A, size_t lda,
-                                                                    const double* __restrict__ B, size_t ldb,
-                                                                    int k0, double* As, double* Bs) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+// One chunk (KC columns of 128 rows) of the panel A straight from global
+// memory into LDS (global_load_lds_dwordx4: lane l moves 16 bytes to dst +
+// 16 l), column k of the chunk to offset k KS; wavefront w of eight moves
+// columns w and w + 8.
+__device__ __attribute__((always_inline)) inline void diag_stage(const double* __restrict__ A, size_t lda, int k0,
+                                                                  double* As) {
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 #pragma unroll
-  for (int k = wave; k < KC; k += 4) {
-    __builtin_amdgcn_global_load_lds((const void*)(A + (size_t)(k0 + k) * lda + 2 * lane),
-                                     (lds_void*)(As + k * KS), 16, 0, 0);
-    __builtin_amdgcn_global_load_lds((const void*)(B + (size_t)(k0 + k) * ldb + 2 * lane),
-                                     (lds_void*)(Bs + k * KS), 16, 0, 0);
-  }
+  for (int k = wave; k < KC; k += 8)
+    __builtin_amdgcn_global_load_lds((const void*)(A + (size_t)(k0 + k) * lda + 2 * lane), (lds_void*)(As + k * KS),
+                                     16, 0, 0);
 }
 
-__device__ __attribute__((always_inline)) inline void tile_mma_begin(const double* __restrict__ A, size_t lda,
-                                                                      const double* __restrict__ B, size_t ldb,
-                                                                      double* As) {
-  __syncthreads();
-  chunk_to_lds(A, lda, B, ldb, 0, As, As + KC * KS);
-}
-
-__device__ inline void tile_mma(double4_t acc[4][4], const double* __restrict__ A, size_t lda,
-                                const double* __restrict__ B, size_t ldb, int K, double sign,
-                                double* As, double* Bs) {
-  (void)Bs;
-  const int t = threadIdx.x;
-  const int lane = t & 63, wave = t >> 6;
-  const int wr = wave >> 1, wc = wave & 1;
-  // chunk 0 was issued by the caller (tile_mma_begin)
-  __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
-  __syncthreads();
-  int buf = 0;
-  for (int k0 = 0; k0 < K; k0 += KC) {
-    if (k0 + KC < K) {
-      double* An = As + (buf ^ 1) * 2 * KC * KS;
-      chunk_to_lds(A, lda, B, ldb, k0 + KC, An, An + KC * KS);
-    }
-    const double* Ac = As + buf * 2 * KC * KS;
-    const double* Bc = Ac + KC * KS;
-#pragma unroll
-    for (int kk = 0; kk < KC; kk += 4) {
-      double a[4], b[4];
-      const int kr = kk + (lane >> 4);
-#pragma unroll
-      for (int m = 0; m < 4; ++m) a[m] = sign * Ac[kr * KS + wr * 64 + m * 16 + (lane & 15)];
-#pragma unroll
-      for (int n = 0; n < 4; ++n) b[n] = Bc[kr * KS + wc * 64 + n * 16 + (lane & 15)];
-#pragma unroll
-      for (int m = 0; m < 4; ++m)
-#pragma unroll
-        for (int n = 0; n < 4; ++n)
-          acc[m][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[m], b[n], acc[m][n], 0, 0, 0);
-    }
-    __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
-    __syncthreads();
-    buf ^= 1;
-  }
-}
-
-// The diagonal tile's update, lower triangle only: its 36 16x16 tiles
-// (row tile ti >= column tile tj) split nine per wavefront.  Wavefront W
-// takes row tiles 7 - W (column tiles 0..7-W, accumulators 0..7-W) and W
-// (column tiles 0..W, accumulators 8-W..8), so one K step needs at most
-// eight column fragments and two row fragments of the panel.  tile_mma's
-// 64x64 split gives one wavefront 16 tiles, and the diagonal tile's MFMA
-// chain is on the factorization's critical path.  Element v of lane l of
-// tile (ti, tj) is C[ti*16 + l%16][tj*16 + l/16 + 4v]: the same products
-// in the same K order as tile_mma.  W is a template argument so that every
-// accumulator index is a constant (nine accumulators, no dynamic indexing);
-// callers switch on the wavefront index.
+// The diagonal tile's update, lower triangle only, on eight wavefronts (two
+// per SIMD: gfx950 reaches its FP64 MFMA rate only so): its 36 16x16 tiles
+// (row tile ti >= column tile tj) in four groups of nine, group p = W / 2
+// holding row tiles 7 - p (column tiles 0..7-p) and p (column tiles 0..p),
+// the first five of a group on wavefront 2p, the last four on 2p + 1.
+// Element v of lane l of tile (ti, tj) is C[ti*16 + l%16][tj*16 + l/16 + 4v]
+// (the tile is computed transposed: the column tile's panel rows are the A
+// operand).  W is a template argument so that every accumulator index is a
+// constant; callers switch on the wavefront index.
 template <int W>
 struct DiagSplit {
-  static constexpr int NH = 8 - W, NL = W + 1;
-  static __device__ __attribute__((always_inline)) inline int row_tile(int e) { return e < NH ? 7 - W : W; }
-  static __device__ __attribute__((always_inline)) inline int col_tile(int e) { return e < NH ? e : e - NH; }
-  // one K step of four: row points at the step's panel row for this lane
-  static __device__ __attribute__((always_inline)) inline void step(double4_t acc[9], const double* row) {
-    double a[NH];
-#pragma unroll
-    for (int q = 0; q < NH; ++q) a[q] = -row[q * 16];
-    const double bh = row[(7 - W) * 16], bl = row[W * 16];
-#pragma unroll
-    for (int q = 0; q < NH; ++q) acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[q], bh, acc[q], 0, 0, 0);
-#pragma unroll
-    for (int q = 0; q < NL; ++q)
-      acc[NH + q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[q], bl, acc[NH + q], 0, 0, 0);
+  static constexpr int P = W >> 1, NH = 8 - P;
+  static constexpr int FIRST = (W & 1) ? 5 : 0, COUNT = (W & 1) ? 4 : 5;
+  static __device__ __attribute__((always_inline)) inline int row_tile(int u) { return FIRST + u < NH ? 7 - P : P; }
+  static __device__ __attribute__((always_inline)) inline int col_tile(int u) {
+    return FIRST + u < NH ? FIRST + u : FIRST + u - NH;
   }
-  static __device__ __attribute__((always_inline)) inline void load(double4_t acc[9], const double* C, size_t ldc) {
+  // one K step of four: row points at the step's panel row for this lane
+  static __device__ __attribute__((always_inline)) inline void step(double4_t (&acc)[5], const double* row) {
+#pragma unroll
+    for (int u = 0; u < COUNT; ++u)
+      acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(-row[col_tile(u) * 16], row[row_tile(u) * 16], acc[u], 0, 0, 0);
+  }
+  static __device__ __attribute__((always_inline)) inline void load(double4_t (&acc)[5], const double* C, size_t ldc) {
     const int lane = threadIdx.x & 63;
 #pragma unroll
-    for (int e = 0; e < 9; ++e)
+    for (int u = 0; u < COUNT; ++u)
 #pragma unroll
       for (int v = 0; v < 4; ++v)
-        acc[e][v] = C[(size_t)(col_tile(e) * 16 + (lane >> 4) + 4 * v) * ldc + row_tile(e) * 16 + (lane & 15)];
+        acc[u][v] = C[(size_t)(col_tile(u) * 16 + (lane >> 4) + 4 * v) * ldc + row_tile(u) * 16 + (lane & 15)];
   }
   // every entry of the lower 16x16 tiles into the tiled LDS block, the
   // identity on padding
-  static __device__ __attribute__((always_inline)) inline void to_lds(const double4_t acc[9], double* L, uint32_t base,
-                                                                      uint32_t D) {
+  static __device__ __attribute__((always_inline)) inline void to_lds(const double4_t (&acc)[5], double* L,
+                                                                      uint32_t base, uint32_t D) {
     const int lane = threadIdx.x & 63;
 #pragma unroll
-    for (int e = 0; e < 9; ++e)
+    for (int u = 0; u < COUNT; ++u)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int r = row_tile(e) * 16 + (lane & 15), c = col_tile(e) * 16 + (lane >> 4) + 4 * v;
+        const int r = row_tile(u) * 16 + (lane & 15), c = col_tile(u) * 16 + (lane >> 4) + 4 * v;
         const bool pad = base + r >= D || base + c >= D;
-        L[tile_index(row_tile(e), col_tile(e)) * 256 + 64 * v + lane] = pad ? (r == c ? 1.0 : 0.0) : acc[e][v];
+        L[tile_index(row_tile(u), col_tile(u)) * 256 + 64 * v + lane] = pad ? (r == c ? 1.0 : 0.0) : acc[u][v];
       }
   }
 };
 
-#define SFMBA_DIAG_SWITCH(w, CALL) \
-  switch (w) {                      \
-    case 0: DiagSplit<0>::CALL; break; \
-    case 1: DiagSplit<1>::CALL; break; \
-    case 2: DiagSplit<2>::CALL; break; \
-    default: DiagSplit<3>::CALL; break; \
+#define SFMBA_DIAG_SWITCH(w, CALL)       \
+  switch (w) {                           \
+    case 0: DiagSplit<0>::CALL; break;   \
+    case 1: DiagSplit<1>::CALL; break;   \
+    case 2: DiagSplit<2>::CALL; break;   \
+    case 3: DiagSplit<3>::CALL; break;   \
+    case 4: DiagSplit<4>::CALL; break;   \
+    case 5: DiagSplit<5>::CALL; break;   \
+    case 6: DiagSplit<6>::CALL; break;   \
+    default: DiagSplit<7>::CALL; break;  \
   }
 
 // The diagonal workgroup's update of its tile (C -= A A^T over K columns of
-// the panel A) and the result into the tiled LDS block.  The panel's first
-// chunk was issued by the caller (tile_mma_begin with A = B); staging and
-// barriers are tile_mma's.
+// the panel A, eight wavefronts) and the result into the tiled LDS block at
+// As, which also holds the two staging buffers until then.
 __device__ inline void diag_tile_update(const double* __restrict__ C, size_t ldc, const double* __restrict__ A,
                                         size_t lda, int K, double* As, uint32_t base, uint32_t D) {
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  double4_t acc[9];
+  double4_t acc[5];
+  diag_stage(A, lda, 0, As);
   SFMBA_DIAG_SWITCH(w, load(acc, C, ldc))
   __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
   __syncthreads();
   int buf = 0;
   for (int k0 = 0; k0 < K; k0 += KC) {
-    if (k0 + KC < K) {
-      double* An = As + (buf ^ 1) * 2 * KC * KS;
-      chunk_to_lds(A, lda, A, lda, k0 + KC, An, An + KC * KS);
-    }
-    const double* Ac = As + buf * 2 * KC * KS;
+    if (k0 + KC < K) diag_stage(A, lda, k0 + KC, As + (buf ^ 1) * KC * KS);
+    const double* Ac = As + buf * KC * KS;
 #pragma unroll
     for (int kk = 0; kk < KC; kk += 4) {
       const double* row = Ac + (kk + (lane >> 4)) * KS + (lane & 15);
@@ -231,49 +176,6 @@ __device__ inline void diag_tile_update(const double* __restrict__ C, size_t ldc
     buf ^= 1;
   }
   SFMBA_DIAG_SWITCH(w, to_lds(acc, As, base, D))
-}
-
-// The trailing-update tiles are computed transposed, acc = C^T: tile_mma
-// is given the column panel as A and the row panel as B, so element
-// (m, n, v) of lane l is C[r][c] with r = wc*64 + n*16 + l%16 and
-// c = wr*64 + m*16 + l/16 + 4v.  Every global access of C then reads or
-// writes 16 consecutive rows (128 bytes) of 4 columns.
-__device__ __attribute__((always_inline)) inline int acc_row(int n, int lane) {
-  return (((threadIdx.x >> 6) & 1) * 64) + n * 16 + (lane & 15);
-}
-__device__ __attribute__((always_inline)) inline int acc_col(int m, int v, int lane) {
-  return ((threadIdx.x >> 7) * 64) + m * 16 + (lane >> 4) + 4 * v;
-}
-
-__device__ inline void tile_load_t(double4_t acc[4][4], const double* C, size_t ldc) {
-  const int lane = threadIdx.x & 63;
-#pragma unroll
-  for (int m = 0; m < 4; ++m)
-#pragma unroll
-    for (int n = 0; n < 4; ++n)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) acc[m][n][v] = C[(size_t)acc_col(m, v, lane) * ldc + acc_row(n, lane)];
-}
-
-// Stores only entries that carry data: global column < D, and global row
-// < D (the system) or == Dp (the right-hand side).  Padding and the spare
-// rows of the rhs block keep their initial values, so a failed (NaN)
-// factorization cannot leak into storage the next assembly does not rewrite
-// (NaN * 0 is NaN).
-__device__ inline void tile_store_t(const double4_t acc[4][4], double* C, size_t ldc, bool lower_only,
-                                    uint32_t row0, uint32_t col0, uint32_t D, uint32_t Dp) {
-  const int lane = threadIdx.x & 63;
-#pragma unroll
-  for (int m = 0; m < 4; ++m)
-#pragma unroll
-    for (int n = 0; n < 4; ++n)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int r = acc_row(n, lane), c = acc_col(m, v, lane);
-        const uint32_t g = row0 + r;
-        if ((!lower_only || r >= c) && (g < D || g == Dp) && col0 + c < D)
-          C[(size_t)c * ldc + r] = acc[m][n][v];
-      }
 }
 
 // ------------------------------------------------------- diagonal block
@@ -598,66 +500,278 @@ __global__ __launch_bounds__(64) void k_trsm_panel(double* __restrict__ S, size_
   }
 }
 
-// A_ij -= sum over the panels kp .. kp + K/NB - 1 of L_ip L_jp^T, for
-// j0 <= j <= i, j < nblk, i <= nblk (i == nblk: the rhs block); a grid of
-// one column (gridDim.x == 1) updates block column j0 only.  The workgroup
-// of the diagonal tile (j0, j0) factors it right away (lookahead), so the
-// next panel solve can start.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_syrk_update(
-    double* __restrict__ S, size_t ld, int kp, int K, int j0, int nblk, uint32_t D, uint32_t Dp,
-    double* __restrict__ Top, int* __restrict__ flags) {
-  __shared__ double smem[SMEM_DIAG];
-  // Workgroup (0, 0) is the diagonal tile (j0, j0): dispatched first, so its
-  // factorization overlaps the other tiles' updates.
-  const int j = j0 + blockIdx.x;
-  const int i = j0 + blockIdx.y;
-  if (j > i || j >= nblk) return;
-  double* C = S + (size_t)j * NB * ld + (size_t)i * NB;
-  const double* Ai = S + (size_t)kp * NB * ld + (size_t)i * NB;
-  const double* Aj = S + (size_t)kp * NB * ld + (size_t)j * NB;
-  double4_t acc[4][4];
-  // The workgroups of the second dispatch round (two per CU) take C after
-  // the products rather than before, so the two workgroups sharing a CU do
-  // not load C at the same time (a scheduling heuristic; either order gives
-  // C - sum L L^T).
-  const uint32_t lin = blockIdx.y * gridDim.x + blockIdx.x;
-  const bool late = (lin >> 8) & 1u;
-  // The diagonal workgroup runs tile_mma with K = 0 (it only waits for the
-  // first staged chunk) and updates its tile's lower triangle with
-  // diag_tile_update instead: 7.43 -> 7.30 ms per factorization at
-  // D = 7,994, same results bit for bit.
-  const bool dwg = i == j0 && j == j0;
-  tile_mma_begin(Aj, ld, Ai, ld, smem);
-  if (!late && !dwg) {
-    tile_load_t(acc, C, ld);
-  } else {
+// Tile t of a trailing update in patch order: the lower triangle of
+// block columns c = 0 .. ncol - 1 (block column j0 + c) and block rows
+// r = c .. nrow - 1 (block row j0 + r, the last one the rhs block) cut into
+// patches of 8 block rows by 4 block columns, patches column group by column
+// group, top to bottom, tiles column-major inside a patch.  t = 0 is the
+// diagonal tile (0, 0).  The 32 tiles of a patch share 12 panel blocks, so
+// the workgroups of one XCD working on a patch find most of them in their L2.
+// Uniform per workgroup; a few scalar loops.
+__device__ inline void trail_tile(int t, int nrow, int ncol, int& r, int& c) {
+  int c0 = 0;
+  for (;;) {
+    const int c1 = min(c0 + 4, ncol);
+    int cnt = 0;
+    for (int cc = c0; cc < c1; ++cc) cnt += nrow - cc;
+    if (t < cnt) break;
+    t -= cnt;
+    c0 = c1;
+  }
+  const int c1 = min(c0 + 4, ncol);
+  int r0 = c0 & ~7;
+  for (;;) {
+    const int r1 = min(r0 + 8, nrow);
+    int cnt = 0;
+    for (int cc = c0; cc < c1; ++cc) cnt += max(0, r1 - max(r0, cc));
+    if (t < cnt) break;
+    t -= cnt;
+    r0 = r1;
+  }
+  const int r1 = min(r0 + 8, nrow);
+  c = c0;
+  for (;;) {
+    const int n = max(0, r1 - max(r0, c));
+    if (t < n) break;
+    t -= n;
+    ++c;
+  }
+  r = max(r0, c) + t;
+}
+
+// The trailing update with one workgroup of eight wavefronts per CU
+// (persistent form).  gfx950 issues FP64 MFMAs at its full rate only from
+// two wavefronts per SIMD (one wavefront alone reaches about half), so the
+// 128x128 tile is split over eight wavefronts of 64x32 (wr = w / 4 picks
+// 64 columns, wc = w % 4 picks 32 rows).  The tile is computed transposed,
+// D = A B^T with A the column panel and B the row panel, so that every
+// global access to C covers 16 consecutive rows (128 bytes): lane l,
+// accumulator (m, n), register v holds C[r][c] with r = 32 wc + 16 n +
+// l % 16, c = 64 wr + 16 m + l / 16 + 4 v.  Chunks of PK panel columns are
+// staged double buffered; the products are summed from zero while the
+// tile's C arrives in registers during the first four steps, and the sum
+// is subtracted from C at the end.
+struct Persist {
+  static constexpr int PK = 32;            // panel columns per step
+  static constexpr int BUF = 2 * PK * KS;  // one staged chunk (A then B)
+  static constexpr int SMEM = 2 * BUF > SMEM_DIAG ? 2 * BUF : SMEM_DIAG;
+
+  static __device__ __attribute__((always_inline)) inline int row(int n, int lane) {
+    return ((threadIdx.x >> 6) & 3) * 32 + n * 16 + (lane & 15);
+  }
+  static __device__ __attribute__((always_inline)) inline int col(int m, int v, int lane) {
+    return (threadIdx.x >> 8) * 64 + m * 16 + (lane >> 4) + 4 * v;
+  }
+
+  // Chunk k0 of panels A (column) and B (row) into LDS: row k of the chunk
+  // at offset k KS; the wavefront index made uniform so that the row
+  // addresses are scalar.
+  static __device__ __attribute__((always_inline)) inline void stage(const double* __restrict__ A,
+                                                                     const double* __restrict__ B, size_t ld, int k0,
+                                                                     double* As) {
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+#pragma unroll
+    for (int k = wave; k < PK; k += 8) {
+      __builtin_amdgcn_global_load_lds((const void*)(A + (size_t)(k0 + k) * ld + 2 * lane),
+                                       (lds_void*)(As + k * KS), 16, 0, 0);
+      __builtin_amdgcn_global_load_lds((const void*)(B + (size_t)(k0 + k) * ld + 2 * lane),
+                                       (lds_void*)(As + PK * KS + k * KS), 16, 0, 0);
+    }
+  }
+
+  // acc += A B^T over the staged chunk; the operands of the next four
+  // columns are read from LDS before the products of the current four.
+  static __device__ __attribute__((always_inline)) inline void mma(double4_t (&acc)[4][2], const double* Ac) {
+    const int lane = threadIdx.x & 63;
+    const int wr = threadIdx.x >> 8, wc = (threadIdx.x >> 6) & 3;
+    const double* pa = Ac + (lane >> 4) * KS + wr * 64 + (lane & 15);
+    const double* pb = Ac + PK * KS + (lane >> 4) * KS + wc * 32 + (lane & 15);
+    double a[2][4], b[2][2];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) a[0][m] = pa[m * 16];
+#pragma unroll
+    for (int n = 0; n < 2; ++n) b[0][n] = pb[n * 16];
+#pragma unroll
+    for (int s = 0; s < PK / 4; ++s) {
+      const int cur = s & 1;
+      if (s + 1 < PK / 4) {
+#pragma unroll
+        for (int m = 0; m < 4; ++m) a[cur ^ 1][m] = pa[(s + 1) * 4 * KS + m * 16];
+#pragma unroll
+        for (int n = 0; n < 2; ++n) b[cur ^ 1][n] = pb[(s + 1) * 4 * KS + n * 16];
+      }
+#pragma unroll
+      for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int n = 0; n < 2; ++n)
+          acc[m][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[cur][m], b[cur][n], acc[m][n], 0, 0, 0);
+    }
+  }
+
+  // One step: stage the next chunk (A = An, B = Bn; none if An is null),
+  // on the tile's last step also load its C, run the products of the
+  // current buffer, wait for every load, meet the other wavefronts.  Each
+  // wavefront's LDS reads of the current buffer are complete at the
+  // barrier, so the next step may stage into it.  (s_barrier alone: the
+  // fence of __syncthreads() would add nothing here.)
+  // C group G (8 of the 32 values per lane: m = G) into the prefetch
+  // registers.
+  template <int G>
+  static __device__ __attribute__((always_inline)) inline void c_group(double (&cr)[4][2][4], const double* C,
+                                                                      size_t ld) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+#pragma unroll
+      for (int n = 0; n < 2; ++n) cr[G][n][v] = C[(size_t)col(G, v, lane) * ld + row(n, lane)];
+  }
+
+  // One step: stage the next chunk (A = An, B = Bn; none if An is null),
+  // load C group G (G >= 0), run the products of the
+  // current buffer, wait for the staged chunk (a C group issued after it
+  // may stay in flight: loads of a wavefront complete in issue order), meet
+  // the other wavefronts.  Each wavefront's LDS reads of the current buffer
+  // are complete at the barrier, so the next step may stage into it.
+  // (s_barrier alone: the fence of __syncthreads() would wait for the C
+  // group too.)
+  template <int G>
+  static __device__ __attribute__((always_inline)) inline void step(double4_t (&acc)[4][2], double (&cr)[4][2][4],
+                                                                    const double* C, size_t ld, const double* An,
+                                                                    const double* Bn, int k_next, double* smem,
+                                                                    int& buf) {
+    if (An) stage(An, Bn, ld, k_next, smem + (buf ^ 1) * BUF);
+    if constexpr (G >= 0) {
+      __asm__ volatile("" ::: "memory");
+      c_group<G>(cr, C, ld);
+      __asm__ volatile("" ::: "memory");
+    }
+    mma(acc, smem + buf * BUF);
+    if constexpr (G >= 0) {
+      static_assert(G < 4, "four C groups of eight loads");
+      __asm__ volatile("s_waitcnt vmcnt(8) lgkmcnt(0)" ::: "memory");
+    } else {
+      __asm__ volatile("s_waitcnt vmcnt(0) lgkmcnt(0)" ::: "memory");
+    }
+    __builtin_amdgcn_s_barrier();
+    __asm__ volatile("" ::: "memory");
+    buf ^= 1;
+  }
+
+  // Entries that carry data only (as tile_store_t): column < D, row < D or
+  // == Dp, the lower triangle on a diagonal tile.
+  static __device__ __attribute__((always_inline)) inline void store(const double4_t (&acc)[4][2], double* C,
+                                                                     size_t ldc, bool lower_only, uint32_t row0,
+                                                                     uint32_t col0, uint32_t D, uint32_t Dp) {
+    const int lane = threadIdx.x & 63;
+    const bool full = !lower_only && row0 + NB <= D && col0 + NB <= D;
 #pragma unroll
     for (int m = 0; m < 4; ++m)
 #pragma unroll
-      for (int n = 0; n < 4; ++n) acc[m][n] = double4_t{0.0, 0.0, 0.0, 0.0};
+      for (int n = 0; n < 2; ++n)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int r = row(n, lane), c = col(m, v, lane);
+          const uint32_t g = row0 + r;
+          if (full || ((!lower_only || r >= c) && (g < D || g == Dp) && col0 + c < D))
+            C[(size_t)c * ldc + r] = acc[m][n][v];
+        }
   }
-  tile_mma(acc, Aj, ld, Ai, ld, dwg ? 0 : K, -1.0, smem, smem + KC * KS);
-  if (late) {
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int m = 0; m < 4; ++m) {
-      double cm[4][4];
-#pragma unroll
-      for (int n = 0; n < 4; ++n)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) cm[n][v] = C[(size_t)acc_col(m, v, lane) * ld + acc_row(n, lane)];
-#pragma unroll
-      for (int n = 0; n < 4; ++n)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) acc[m][n][v] += cm[n][v];
-    }
-  }
-  if (dwg) {
-    diag_tile_update(C, ld, Aj, ld, K, smem, (uint32_t)j * NB, D);
+};
+
+// A_ij -= sum over the panels kp .. kp + K/NB - 1 of L_ip L_jp^T for
+// j0 <= j <= i, j < nblk, i <= nblk (i == nblk: the rhs block); ncol == 1
+// updates block column j0 only.  One workgroup per CU walks its tiles in
+// turn; the next tile's first chunk is staged during the last step of the
+// current one, and the stores of one tile drain during the next one's
+// first step.  Workgroup 0 updates the diagonal tile (j0, j0) with all
+// eight wavefronts and factors it with the first four (lookahead: the next
+// panel solve can start as soon as this launch ends); the other four end
+// there (s_barrier waits for the wavefronts still running).  Measured at
+// D = 7,994 on MI355X: 6.7 ms per factorization against 7.6 ms with two
+// four-wavefront workgroups per CU, one tile each.
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_syrk_persist(
+    double* __restrict__ S, size_t ld, int kp, int K, int j0, int nblk, int ncol, uint32_t D, uint32_t Dp,
+    double* __restrict__ Top, int* __restrict__ flags) {
+  __shared__ double smem[Persist::SMEM];
+  const double* P = S + (size_t)kp * NB * ld;
+  if (blockIdx.x == 0) {
+    const double* Aj = P + (size_t)j0 * NB;
+    diag_tile_update(S + (size_t)j0 * NB * ld + (size_t)j0 * NB, ld, Aj, ld, K, smem, (uint32_t)j0 * NB, D);
+    __syncthreads();
+    if (threadIdx.x >= 256) return;
     diag_factor(smem, S, ld, j0, D, Top, flags);
     return;
   }
-  tile_store_t(acc, C, ld, i == j, (uint32_t)i * NB, (uint32_t)j * NB, D, Dp);
+  // Workgroup b >= 1 takes tiles u(b), u(b) + G - 1, ... (G = gridDim.x).
+  // With one workgroup per CU on all CUs, workgroups are dispatched to the
+  // eight XCDs in turn (b -> XCD b % 8), and u(b) = (G / 8) (b % 8) + b / 8
+  // gives the workgroups of one XCD consecutive tiles, i.e. whole patches.
+  const int nrow = nblk - j0 + 1;
+  const int ntiles = ncol * nrow - ncol * (ncol - 1) / 2;
+  const int G = (int)gridDim.x, stride = G - 1;
+  const int b = (int)blockIdx.x;
+  const bool spread = ntiles >= G && (G & 7) == 0;
+  int t = spread ? (G >> 3) * (b & 7) + (b >> 3) : b;
+  if (t >= ntiles) return;
+  int r, c;
+  trail_tile(t, nrow, ncol, r, c);
+  int i = j0 + r, j = j0 + c;
+  const double* Ai = P + (size_t)i * NB;
+  const double* Aj = P + (size_t)j * NB;
+  Persist::stage(Aj, Ai, ld, 0, smem);
+  __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  __syncthreads();
+  int buf = 0;
+  const int nch = K / Persist::PK;
+  for (;;) {
+    const int tn = t + stride;
+    const bool more = tn < ntiles;
+    int in = 0, jn = 0;
+    if (more) {
+      trail_tile(tn, nrow, ncol, r, c);
+      in = j0 + r;
+      jn = j0 + c;
+    }
+    const double* Ain = P + (size_t)in * NB;
+    const double* Ajn = more ? P + (size_t)jn * NB : nullptr;
+    double* C = S + (size_t)j * NB * ld + (size_t)i * NB;
+    double4_t acc[4][2];
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+      for (int n = 0; n < 2; ++n) acc[m][n] = double4_t{0.0, 0.0, 0.0, 0.0};
+    double cr[4][2][4];
+    // C arrives in four groups during the first four steps (nch >= 4)
+    Persist::step<0>(acc, cr, C, ld, Aj, Ai, 1 * Persist::PK, smem, buf);
+    Persist::step<1>(acc, cr, C, ld, Aj, Ai, 2 * Persist::PK, smem, buf);
+    Persist::step<2>(acc, cr, C, ld, Aj, Ai, 3 * Persist::PK, smem, buf);
+    Persist::step<3>(acc, cr, C, ld, 4 < nch ? Aj : Ajn, 4 < nch ? Ai : Ain, 4 < nch ? 4 * Persist::PK : 0, smem, buf);
+#pragma unroll 1
+    for (int ch = 4; ch < nch; ++ch) {
+      const bool inner = ch + 1 < nch;
+      Persist::step<-1>(acc, cr, C, ld, inner ? Aj : Ajn, inner ? Ai : Ain, inner ? (ch + 1) * Persist::PK : 0, smem,
+                        buf);
+    }
+    // C - sum, all before the first store: the stores then issue back to
+    // back (a difference left next to its predicated store would wait for
+    // every earlier store too).
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+      for (int n = 0; n < 2; ++n) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[m][n][v] = cr[m][n][v] - acc[m][n][v];
+        __asm__ volatile("" : "+v"(acc[m][n]));
+      }
+    Persist::store(acc, C, ld, i == j, (uint32_t)i * NB, (uint32_t)j * NB, D, Dp);
+    if (!more) break;
+    t = tn;
+    i = in;
+    j = jn;
+    Ai = Ain;
+    Aj = Ajn;
+  }
 }
 
 // L^{-1} of every diagonal block, all blocks at once after the
@@ -968,10 +1082,19 @@ __global__ __launch_bounds__(1024) void k_lu_solve(const double* __restrict__ A,
 }
 
 // ------------------------------------------------------------- launchers
+// Compute units of the current device (one persistent workgroup each).
+static int compute_units() {
+  int dev = 0, n = 0;
+  HIP_CHECK(hipGetDevice(&dev));
+  HIP_CHECK(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev));
+  return std::max(2, n);
+}
+
 void dense_cholesky(double* S, double* Linv, uint32_t Dp, uint32_t ld, uint32_t D, int* flags,
                     hipStream_t s, KernelTimer* kt) {
   const int nblk = (int)(Dp / NB);
   double* Top = Linv + (size_t)nblk * LINV_BLOCK;
+  const int n_cu = compute_units();
   auto trsm = [&](int k) {
     // 8 strips per row block k+1 .. nblk-1, one for the rhs row
     const int strips = 8 * (nblk - 1 - k) + 1;
@@ -982,8 +1105,12 @@ void dense_cholesky(double* S, double* Linv, uint32_t Dp, uint32_t ld, uint32_t 
   auto syrk = [&](int kp, int np, int j0, bool column_only) {
     const int ncol = column_only ? 1 : nblk - j0;
     hipEvent_t e0 = kt ? kt->before(s) : nullptr;
-    hipLaunchKernelGGL(k_syrk_update, dim3(ncol, nblk - j0 + 1), dim3(256), 0, s, S, (size_t)ld, kp, np * NB, j0,
-                       nblk, D, Dp, Top, flags);
+    {
+      const int nrow = nblk - j0 + 1;
+      const int ntiles = ncol * nrow - ncol * (ncol - 1) / 2;
+      hipLaunchKernelGGL(k_syrk_persist, dim3(std::min(ntiles, n_cu)), dim3(512), 0, s, S, (size_t)ld, kp, np * NB,
+                         j0, nblk, ncol, D, Dp, Top, flags);
+    }
     if (kt) {
       // algorithmic FLOP: the panel columns that carry data (nk) times the
       // lower-triangle entries updated (rows >= j0 NB, below D) and the rhs
