@@ -257,13 +257,14 @@ __device__ __attribute__((always_inline)) inline void panel_factor(double* A, in
 // tile per wave at a time on v_mfma_f64_16x16x4f64, computed transposed
 // (D'[n][i] = C[r0+i][c0+n]) so that operands and accumulators are read and
 // written as 64 consecutive doubles of a tile.
+template <int NW>
 __device__ __attribute__((always_inline)) inline void lds_trailing_update(double* A, int p) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int T = NB / PW - 1 - p;
   const int ntiles = T * (T + 1) / 2;
   // two tiles per wave at a time, their MFMA chains interleaved
-  for (int e0 = wave; e0 < ntiles; e0 += 8) {
-    const int e1 = e0 + 4;
+  for (int e0 = wave; e0 < ntiles; e0 += 2 * NW) {
+    const int e1 = e0 + NW;
     const bool two = e1 < ntiles;
     double* C[2];
     const double* Li[2];
@@ -305,6 +306,7 @@ __device__ __attribute__((always_inline)) inline void lds_trailing_update(double
 //                                    operand s, lane l -> -L[16p + l%16][16q + 4s + l/16])
 //   block 28 + p:                    X11_p = L_pp^{-1}, column-major 16x16
 //                                    (operand s, lane l -> X11_p[l%16][4s + l/16])
+template <int NW>
 __device__ __attribute__((always_inline)) inline void diag_factor(double* A, double* __restrict__ S,
                                                                    size_t ld, int k, uint32_t D,
                                                                    double* __restrict__ Top,
@@ -316,26 +318,28 @@ __device__ __attribute__((always_inline)) inline void diag_factor(double* A, dou
   double* G = S + (size_t)k * NB * ld + (size_t)k * NB;
 #pragma unroll 1
   for (int p = 0; p < NB / PW; ++p) {
-    panel_factor(A, PW * p, base, D, flags);
+    if (t < 256) panel_factor(A, PW * p, base, D, flags);
     __syncthreads();
     if (p + 1 < NB / PW) {
-      lds_trailing_update(A, p);
+      lds_trailing_update<NW>(A, p);
       __syncthreads();
     }
   }
   // Threads 0..127: X11_p by forward substitution, one thread per (p,
-  // column c), straight to Top.  Threads 128..255 meanwhile store L (lower
+  // column c), straight to Top.  The other threads meanwhile store L (lower
   // triangle, real rows) into S and the -L tiles into Top, 16 bytes at a
   // time for the 28 off-diagonal tiles.
   if (t >= 128) {
+    constexpr int NT = NW * 64 - 128;
     const int u = t - 128;
 #pragma unroll 4
-    for (int it = 0; it < 28; ++it) {
-      // Top tile it = ti (ti - 1) / 2 + tj, ti > tj (uniform per iteration)
+    for (int idx = u; idx < 28 * 128; idx += NT) {
+      // Top tile it = ti (ti - 1) / 2 + tj, ti > tj
+      const int it = idx >> 7;
       int ti = 1;
       while ((ti + 1) * ti / 2 <= it) ++ti;
       const int tj = it - ti * (ti - 1) / 2;
-      const int e = 2 * u, rr = e & 15, cc = e >> 4;
+      const int e = 2 * (idx & 127), rr = e & 15, cc = e >> 4;
       const double2 v = *reinterpret_cast<const double2*>(A + tile_index(ti, tj) * 256 + e);
       *reinterpret_cast<double2*>(T + it * 256 + e) = make_double2(-v.x, -v.y);
       const uint32_t r = base + PW * ti + rr;
@@ -347,7 +351,7 @@ __device__ __attribute__((always_inline)) inline void diag_factor(double* A, dou
       }
     }
 #pragma unroll 2
-    for (int idx = u; idx < 8 * 256; idx += 128) {
+    for (int idx = u; idx < 8 * 256; idx += NT) {
       const int ti = idx >> 8, e = idx & 255, rr = e & 15, cc = e >> 4;
       if (rr < cc || base + PW * ti + rr >= D) continue;
       G[(size_t)(PW * ti + cc) * ld + PW * ti + rr] = A[tile_index(ti, ti) * 256 + e];
@@ -452,7 +456,7 @@ __global__ __launch_bounds__(256) void k_potrf_block(double* __restrict__ S, siz
     const bool pad = base + r >= D || base + c >= D;
     smem[doff(r, c)] = pad ? (r == c ? 1.0 : 0.0) : (r >= c ? G[(size_t)c * ld + r] : 0.0);
   }
-  diag_factor(smem, S, ld, k, D, Top, flags);
+  diag_factor<4>(smem, S, ld, k, D, Top, flags);
 }
 
 // L_ik = A_ik L_kk^{-T} for row blocks i = k+1 .. nblk-1 and the rhs row
@@ -500,43 +504,17 @@ __global__ __launch_bounds__(64) void k_trsm_panel(double* __restrict__ S, size_
   }
 }
 
-// Tile t of a trailing update in patch order: the lower triangle of
-// block columns c = 0 .. ncol - 1 (block column j0 + c) and block rows
-// r = c .. nrow - 1 (block row j0 + r, the last one the rhs block) cut into
-// patches of 8 block rows by 4 block columns, patches column group by column
-// group, top to bottom, tiles column-major inside a patch.  t = 0 is the
-// diagonal tile (0, 0).  The 32 tiles of a patch share 12 panel blocks, so
-// the workgroups of one XCD working on a patch find most of them in their L2.
-// Uniform per workgroup; a few scalar loops.
-__device__ inline void trail_tile(int t, int nrow, int ncol, int& r, int& c) {
-  int c0 = 0;
-  for (;;) {
-    const int c1 = min(c0 + 4, ncol);
-    int cnt = 0;
-    for (int cc = c0; cc < c1; ++cc) cnt += nrow - cc;
-    if (t < cnt) break;
-    t -= cnt;
-    c0 = c1;
-  }
-  const int c1 = min(c0 + 4, ncol);
-  int r0 = c0 & ~7;
-  for (;;) {
-    const int r1 = min(r0 + 8, nrow);
-    int cnt = 0;
-    for (int cc = c0; cc < c1; ++cc) cnt += max(0, r1 - max(r0, cc));
-    if (t < cnt) break;
-    t -= cnt;
-    r0 = r1;
-  }
-  const int r1 = min(r0 + 8, nrow);
-  c = c0;
-  for (;;) {
-    const int n = max(0, r1 - max(r0, c));
-    if (t < n) break;
-    t -= n;
+// Tile t of a trailing update, in column-major order of its lower
+// triangle: column c = 0 .. (block column j0 + c), rows r = c .. nrow - 1
+// (block row j0 + r; the last row is the rhs block).  Uniform per
+// workgroup, at most one iteration per block column.
+__device__ __attribute__((always_inline)) inline void trail_tile(int t, int nrow, int& r, int& c) {
+  c = 0;
+  while (t >= nrow - c) {
+    t -= nrow - c;
     ++c;
   }
-  r = max(r0, c) + t;
+  r = c + t;
 }
 
 // The trailing update with one workgroup of eight wavefronts per CU
@@ -698,24 +676,17 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
   if (blockIdx.x == 0) {
     const double* Aj = P + (size_t)j0 * NB;
     diag_tile_update(S + (size_t)j0 * NB * ld + (size_t)j0 * NB, ld, Aj, ld, K, smem, (uint32_t)j0 * NB, D);
-    __syncthreads();
-    if (threadIdx.x >= 256) return;
-    diag_factor(smem, S, ld, j0, D, Top, flags);
+    diag_factor<8>(smem, S, ld, j0, D, Top, flags);
     return;
   }
-  // Workgroup b >= 1 takes tiles u(b), u(b) + G - 1, ... (G = gridDim.x).
-  // With one workgroup per CU on all CUs, workgroups are dispatched to the
-  // eight XCDs in turn (b -> XCD b % 8), and u(b) = (G / 8) (b % 8) + b / 8
-  // gives the workgroups of one XCD consecutive tiles, i.e. whole patches.
+  // Workgroup b >= 1 takes tiles b, b + G - 1, ... (G = gridDim.x).
   const int nrow = nblk - j0 + 1;
   const int ntiles = ncol * nrow - ncol * (ncol - 1) / 2;
-  const int G = (int)gridDim.x, stride = G - 1;
-  const int b = (int)blockIdx.x;
-  const bool spread = ntiles >= G && (G & 7) == 0;
-  int t = spread ? (G >> 3) * (b & 7) + (b >> 3) : b;
+  const int stride = (int)gridDim.x - 1;
+  int t = (int)blockIdx.x;
   if (t >= ntiles) return;
   int r, c;
-  trail_tile(t, nrow, ncol, r, c);
+  trail_tile(t, nrow, r, c);
   int i = j0 + r, j = j0 + c;
   const double* Ai = P + (size_t)i * NB;
   const double* Aj = P + (size_t)j * NB;
@@ -729,7 +700,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     const bool more = tn < ntiles;
     int in = 0, jn = 0;
     if (more) {
-      trail_tile(tn, nrow, ncol, r, c);
+      trail_tile(tn, nrow, r, c);
       in = j0 + r;
       jn = j0 + c;
     }
