@@ -587,12 +587,6 @@ struct Persist {
     }
   }
 
-  // One step: stage the next chunk (A = An, B = Bn; none if An is null),
-  // on the tile's last step also load its C, run the products of the
-  // current buffer, wait for every load, meet the other wavefronts.  Each
-  // wavefront's LDS reads of the current buffer are complete at the
-  // barrier, so the next step may stage into it.  (s_barrier alone: the
-  // fence of __syncthreads() would add nothing here.)
   // C group G (8 of the 32 values per lane: m = G) into the prefetch
   // registers.
   template <int G>
@@ -606,10 +600,10 @@ struct Persist {
   }
 
   // One step: stage the next chunk (A = An, B = Bn; none if An is null),
-  // load C group G (G >= 0), run the products of the
-  // current buffer, wait for the staged chunk (a C group issued after it
-  // may stay in flight: loads of a wavefront complete in issue order), meet
-  // the other wavefronts.  Each wavefront's LDS reads of the current buffer
+  // load C group G (G >= 0), run the products of the current buffer, wait
+  // for the staged chunk (a C group issued after it may stay in flight:
+  // loads of a wavefront complete in issue order), meet the other
+  // wavefronts.  Each wavefront's LDS reads of the current buffer
   // are complete at the barrier, so the next step may stage into it.
   // (s_barrier alone: the fence of __syncthreads() would wait for the C
   // group too.)
@@ -636,8 +630,11 @@ struct Persist {
     buf ^= 1;
   }
 
-  // Entries that carry data only (as tile_store_t): column < D, row < D or
-  // == Dp, the lower triangle on a diagonal tile.
+  // Stores only entries that carry data: column < D, row < D or == Dp (the
+  // rhs), the lower triangle on a diagonal tile.  Padding and the spare rows
+  // of the rhs block keep their initial values, so a failed (NaN)
+  // factorization cannot leak into storage the next assembly does not
+  // rewrite (NaN * 0 is NaN).
   static __device__ __attribute__((always_inline)) inline void store(const double4_t (&acc)[4][2], double* C,
                                                                      size_t ldc, bool lower_only, uint32_t row0,
                                                                      uint32_t col0, uint32_t D, uint32_t Dp) {
@@ -665,9 +662,10 @@ struct Persist {
 // first step.  Workgroup 0 updates the diagonal tile (j0, j0) with all
 // eight wavefronts and factors it with the first four (lookahead: the next
 // panel solve can start as soon as this launch ends); the other four end
-// there (s_barrier waits for the wavefronts still running).  Measured at
-// D = 7,994 on MI355X: 6.7 ms per factorization against 7.6 ms with two
-// four-wavefront workgroups per CU, one tile each.
+// there (s_barrier waits for the wavefronts still running).  K is a
+// multiple of 128, so a tile takes at least four steps of PK = 32 columns.
+// Measured at D = 7,994 on MI355X: 6.4 ms per factorization against 7.6 ms
+// with two four-wavefront workgroups per CU, one tile each.
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_syrk_persist(
     double* __restrict__ S, size_t ld, int kp, int K, int j0, int nblk, int ncol, uint32_t D, uint32_t Dp,
     double* __restrict__ Top, int* __restrict__ flags) {

@@ -78,6 +78,22 @@ def test_dense_cholesky_and_lu_match_lapack(n):
     assert ok and np.abs(x - want).max() <= 1e-11 * np.abs(want).max()
 
 
+@pytest.mark.parametrize("n", [3100, 8500])
+def test_dense_cholesky_large_matches_lapack(n):
+    # 3100: more trailing tiles than CUs in the first steps, so each
+    # persistent workgroup walks several tiles; 8500: 67 blocks, so the first
+    # steps go in pairs (the two-panel K = 256 update) before single steps.
+    rng = np.random.default_rng(n)
+    R = rng.uniform(-1.0, 1.0, (n, n))
+    A = 0.5 * (R + R.T) + 2.0 * n * np.eye(n)  # diagonally dominant: SPD
+    b = rng.standard_normal(n)
+    want = np.linalg.solve(A, b)
+    x, ok = ba.dense_solve(A, b, "cholesky")
+    assert ok
+    assert np.abs(x - want).max() <= 1e-12 * np.abs(want).max()
+    assert np.abs(A @ x - b).max() <= 1e-12 * np.abs(b).max() * n
+
+
 def test_dense_cholesky_reports_indefinite_and_lu_solves_it():
     A = np.diag([1.0, -2.0, 3.0] * 50)
     b = np.ones(150)
