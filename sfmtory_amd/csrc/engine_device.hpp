@@ -8,8 +8,9 @@
 //                      and the per-observation terms of hot loop B, 880-909
 //   point_accumulate   c_diag / c_rhs sums, 885-886; the points_to_obs test, 960
 //   point_prepare      damped point-block inverse, 958-965
-//   compute_F / schur_chunk (schur_chunk_mfma, schur_chunk_mfma8: the same
-//   reduction on the matrix cores)
+//   compute_F / schur_chunk (schur_chunk_mfma8: the same reduction on the
+//   matrix cores), for the window solver's row streams; compute_Y /
+//   schur_chunk_rec, from incidence records, for the single-problem solve
 //                      damped diagonal and cross blocks, 923-956, and the
 //                      point elimination S -= E C^-1 E^T, rhs += E C^-1 g of
 //                      accumulate_point_schur, 112-211 / 967-1043
@@ -78,6 +79,7 @@ struct View {
   V2* J2;
   V4* E4;
   V4* F4;
+  T* rec;
   T* obs_pc;
   uint8_t* obs_valid;
   double* pt_C;
@@ -113,16 +115,18 @@ struct Acc<double> {
 
 // ---------------------------------------------------------------- linearize
 // Observation o (point-major order): for each of its incidences one J2 and
-// one E4 row per unknown, and its contribution to its point's 3x3 block and
-// gradient (obs_pc).  With LPO > 1, LPO lanes share observation o: each
-// computes the same values and lane r writes rows r, r + LPO, ... and
-// obs_pc entries r, r + LPO, ..., so that one store instruction writes
-// whole rows of consecutive observations (a full cache line per
-// observation) instead of one 16- or 32-byte piece per lane.
-template <typename T, int LPO = 1>
+// one E4 row per unknown (DB = 0), or its incidence records (DB > 0: X2 and
+// R2, and one J2 row per unknown; rec_values), and its contribution to its
+// point's 3x3 block and gradient (obs_pc).  With LPO > 1, LPO lanes share
+// observation o: each computes the same values and lane r writes rows r,
+// r + LPO, ... and obs_pc entries r, r + LPO, ..., so that one store
+// instruction writes whole rows of consecutive observations (a full cache
+// line per observation) instead of one 16- or 32-byte piece per lane.
+template <typename T, int LPO = 1, int DB = 0>
 __device__ inline void linearize_obs(const View<T>& v, uint32_t o, int cur, int loss, double scale,
                                      uint32_t r = 0) {
   using V4 = typename Vec4<T>::type;
+  using V2 = typename Vec2<T>::type;
   const uint32_t img = v.obs_img[o];
   const uint32_t pt = v.obs_pt[o];
   const uint32_t cam = v.cam_of_img[img];
@@ -142,6 +146,15 @@ __device__ inline void linearize_obs(const View<T>& v, uint32_t o, int cur, int 
   const bool two = inc1 != NONE;
   const uint32_t roff1 = two ? v.inc_roff[inc1] : 0u;
   T* pc_out = v.obs_pc + 9 * (size_t)o;
+  // DB > 0: the incidences' records, in V2 units (an observation without a
+  // free unknown has none and writes no record)
+  V2* rec0 = nullptr;
+  V2* rec1 = nullptr;
+  if constexpr (DB > 0) {
+    constexpr size_t RS2 = rec_values(DB) / 2;
+    if (inc0 != NONE) rec0 = reinterpret_cast<V2*>(v.rec) + (size_t)inc0 * RS2;
+    if (two) rec1 = reinterpret_cast<V2*>(v.rec) + (size_t)inc1 * RS2;
+  }
 
   if (c2 <= 1e-9) {
     // Behind the camera: dropped for this iteration (lib.rs:339-341).
@@ -150,13 +163,27 @@ __device__ inline void linearize_obs(const View<T>& v, uint32_t o, int cur, int 
     for (uint32_t k0 = 0; k0 < 9; k0 += LPO)
       if (k0 + r < 9) pc_out[k0 + r] = T(0);
     const V4 z = {T(0), T(0), T(0), T(0)};
+    const V2 z2 = {T(0), T(0)};
+    if constexpr (DB > 0) {
+#pragma unroll
+      for (uint32_t k0 = 0; k0 < 4; k0 += LPO)
+        if (k0 + r < 4) {
+          if (rec0) rec0[k0 + r] = z2;
+          if (rec1) rec1[k0 + r] = z2;
+        }
+    }
 #pragma unroll
     for (uint32_t s0 = 0; s0 < 14; s0 += LPO) {
       const uint32_t s = s0 + r;
       if (s >= ns) break;
-      const uint32_t row = (two && s >= 6) ? roff1 + (s - 6) : roff0 + s;
-      v.J2[row] = typename Vec2<T>::type{T(0), T(0)};
-      v.E4[row] = z;
+      if constexpr (DB > 0) {
+        V2* dst = (two && s >= 6) ? rec1 + 4 + (s - 6) : rec0 + 4 + s;
+        *dst = z2;
+      } else {
+        const uint32_t row = (two && s >= 6) ? roff1 + (s - 6) : roff0 + s;
+        v.J2[row] = z2;
+        v.E4[row] = z;
+      }
     }
     return;
   }
@@ -204,6 +231,19 @@ __device__ inline void linearize_obs(const View<T>& v, uint32_t o, int cur, int 
     for (uint32_t k = k0 + 1; k < k0 + LPO && k < 9; ++k)
       if (k == k0 + r) val = pc[k];
     if (k0 + r < 9) pc_out[k0 + r] = val;
+  }
+  if constexpr (DB > 0) {
+    // X2 (three V2) and R2 into every record of the observation
+#pragma unroll
+    for (uint32_t k0 = 0; k0 < 4; k0 += LPO) {
+      const uint32_t k = k0 + r;
+      if (k < 4) {
+        const T x0 = k == 0 ? sx[0] : k == 1 ? sx[2] : k == 2 ? sx[4] : sr0;
+        const T x1 = k == 0 ? sx[1] : k == 1 ? sx[3] : k == 2 ? sx[5] : sr1;
+        if (rec0) rec0[k] = V2{x0, x1};
+        if (rec1) rec1[k] = V2{x0, x1};
+      }
+    }
   }
 
   // Side vector: free pose columns, then the camera's free parameters.
@@ -260,10 +300,15 @@ __device__ inline void linearize_obs(const View<T>& v, uint32_t o, int cur, int 
         a0 = s0[q];
         a1 = s1[q];
       }
-    const uint32_t row = (two && s >= 6) ? roff1 + (s - 6) : roff0 + s;
     const T j0 = sw * a0, j1 = sw * a1;
-    v.J2[row] = typename Vec2<T>::type{j0, j1};
-    v.E4[row] = V4{j0 * sx[0] + j1 * sx[3], j0 * sx[1] + j1 * sx[4], j0 * sx[2] + j1 * sx[5], j0 * sr0 + j1 * sr1};
+    if constexpr (DB > 0) {
+      V2* dst = (two && s >= 6) ? rec1 + 4 + (s - 6) : rec0 + 4 + s;
+      *dst = V2{j0, j1};
+    } else {
+      const uint32_t row = (two && s >= 6) ? roff1 + (s - 6) : roff0 + s;
+      v.J2[row] = V2{j0, j1};
+      v.E4[row] = V4{j0 * sx[0] + j1 * sx[3], j0 * sx[1] + j1 * sx[4], j0 * sx[2] + j1 * sx[5], j0 * sr0 + j1 * sr1};
+    }
   }
 }
 
@@ -358,28 +403,6 @@ __device__ inline void compute_F(const View<T>& v, uint32_t a) {
     // w: this row's share of rhs, F . g_point = E Cinv g_point
     v.F4[f0ff + r] = V4{f0, f1, f2, f0 * g0 + f1 * g1 + f2 * g2};
   }
-}
-
-// Row r of incidence a, for the grid kernel: the lanes of one incidence
-// read and write its consecutive rows in one instruction (whole cache
-// lines), and share the point's Cinv and g by broadcast.
-template <typename T>
-__device__ inline void compute_F_row(const View<T>& v, uint32_t a, uint32_t r) {
-  using V4 = typename Vec4<T>::type;
-  const uint32_t n = v.grp_dim[v.inc_grp[a]];
-  if (r >= n) return;
-  const uint32_t pt = v.inc_pt[a];
-  const double* ci = v.pt_cinv + 9 * (size_t)pt;
-  T c[9];
-#pragma unroll
-  for (int k = 0; k < 9; ++k) c[k] = (T)ci[k];
-  const T g0 = (T)v.pt_g[3 * (size_t)pt], g1 = (T)v.pt_g[3 * (size_t)pt + 1],
-          g2 = (T)v.pt_g[3 * (size_t)pt + 2];
-  const V4 e = v.E4[v.inc_roff[a] + r];
-  const T f0 = e.x * c[0] + e.y * c[3] + e.z * c[6];
-  const T f1 = e.x * c[1] + e.y * c[4] + e.z * c[7];
-  const T f2 = e.x * c[2] + e.y * c[5] + e.z * c[8];
-  v.F4[v.inc_froff[a] + r] = V4{f0, f1, f2, f0 * g0 + f1 * g1 + f2 * g2};
 }
 
 constexpr int PART_STRIDE = 2 * 196 + 16;
@@ -514,121 +537,13 @@ struct Mfma<double> {
   }
 };
 
-// The same destination-block reduction as schur_chunk, on the matrix cores:
-// the block is one 16x16 MFMA tile (da, db <= 16).  Per pair, lane l loads
-// one component, F_a[l % 16][l / 16] and E_b[l % 16][l / 16] for the three
-// point coordinates (K slot 3 stays zero), and one MFMA adds F_a E_b^T; a
-// pair of the same observation adds w J_a^T J_b with a second MFMA over the
-// two residual components.  The FP32 instantiation flushes its MFMA sums
-// into FP64 after every U pairs, so no more than U terms are summed in
-// FP32.  Output entries, partials and the right-hand side follow
-// schur_chunk.
-template <typename T, int U>
-__device__ inline void schur_chunk_mfma(const View<T>& v, const uint4 ch, double lambda, double* parts) {
-  using M = Mfma<T>;
-  using acc_t = typename M::acc_t;
-  constexpr uint32_t HI = 0x80000000u, OFF = 0x7FFFFFFFu;
-  static_assert(PAIR_CHUNK <= 128, "a chunk's pair words are held in two registers per lane");
-  const uint32_t lane = threadIdx.x & 63;
-  const uint32_t i = lane & 15, k = lane >> 4;
-  const uint32_t ga = ch.x >> 16, gb = ch.x & 0xFFFFu;
-  const uint32_t da = v.grp_dim[ga], db = v.grp_dim[gb];
-  const bool diag = ga == gb;
-  const bool fa_on = i < da && k < 3, eb_on = i < db && k < 3;
-  const bool ja_on = i < da && k < 2, jb_on = i < db && k < 2;
-  const T* F = reinterpret_cast<const T*>(v.F4);
-  const T* E = reinterpret_cast<const T*>(v.E4);
-  const T* J = reinterpret_cast<const T*>(v.J2);
-  // All pair words of the chunk in one coalesced load: lane l holds pairs
-  // l and 64 + l; pair q reaches every lane by v_readlane.
-  const uint32_t n = ch.z - ch.y;
-  const uint2 w0 = lane < n ? v.pairs[ch.y + lane] : make_uint2(0u, 0u);
-  const uint2 w1 = lane + 64 < n ? v.pairs[ch.y + 64 + lane] : make_uint2(0u, 0u);
-  auto word = [&](uint32_t q, uint32_t& x, uint32_t& y) {
-    const uint32_t sl = q & 63;
-    x = __builtin_amdgcn_readlane(q < 64 ? w0.x : w1.x, sl);
-    y = __builtin_amdgcn_readlane(q < 64 ? w0.y : w1.y, sl);
-  };
-  // Operands of U pairs starting at q (zero past the chunk's end).
-  struct Stage {
-    T fa[U], eb[U], ja[U], jb[U];
-  };
-  auto load = [&](uint32_t q, Stage& st) {
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const bool ok = q + u < n;
-      uint32_t px = 0u, py = 0u;
-      if (ok) word(q + u, px, py);
-      const uint32_t ra = px & OFF, rb = py & OFF;
-      st.fa[u] = (ok && fa_on) ? F[4 * (size_t)(ra + i) + k] : T(0);
-      st.eb[u] = (ok && eb_on) ? E[4 * (size_t)(rb + i) + k] : T(0);
-      const bool same = ok && (px & HI);
-      st.ja[u] = (same && ja_on) ? J[2 * (size_t)(ra + i) + k] : T(0);
-      st.jb[u] = (same && jb_on) ? J[2 * (size_t)(rb + i) + k] : T(0);
-    }
-  };
-  acc_t s = {T(0), T(0), T(0), T(0)}, b = {T(0), T(0), T(0), T(0)};
-  double ss[4] = {0.0, 0.0, 0.0, 0.0}, bs[4] = {0.0, 0.0, 0.0, 0.0};
-  Acc<T> accR;
-  const bool rlane = diag && lane < da;
-  Stage cur, nxt;
-  load(0, cur);
-  for (uint32_t q = 0; q < n; q += U) {
-    if (q + U < n) load(q + U, nxt);  // next group's loads in flight during this group's MFMAs
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      if (q + u >= n) break;
-      uint32_t px, py;
-      word(q + u, px, py);
-      s = M::step(cur.fa[u], cur.eb[u], s);
-      if (px & HI) b = M::step(cur.ja[u], cur.jb[u], b);
-      if (rlane && (py & HI)) accR.add(v.F4[(px & OFF) + lane].w - v.E4[(py & OFF) + lane].w);
-    }
-    if (sizeof(T) == 4) {
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        ss[c] += (double)s[c];
-        bs[c] += (double)b[c];
-        s[c] = T(0);
-        b[c] = T(0);
-      }
-    }
-    cur = nxt;
-  }
-  if (sizeof(T) == 8) {
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      ss[c] = (double)s[c];
-      bs[c] = (double)b[c];
-    }
-  }
-  const uint32_t oa = v.grp_off[ga], ob = v.grp_off[gb];
-  const uint32_t c = i;
-  if (ch.w == NONE) {
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const uint32_t r = M::row(lane, q);
-      if (r >= da || c >= db || (diag && r < c)) continue;
-      double val = bs[q] - ss[q];
-      if (diag && r == c) val = (bs[q] + bs[q] * lambda) - ss[q];
-      v.S[(size_t)(ob + c) * v.ld + oa + r] = val;
-    }
-    if (rlane) v.S[(size_t)(oa + lane) * v.ld + v.Dp] = accR.value();
-  } else {
-    double* pp = parts + (size_t)ch.w * PART_STRIDE;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const uint32_t r = M::row(lane, q);
-      if (r >= da || c >= db) continue;
-      pp[r + c * da] = bs[q];
-      pp[196 + r + c * da] = ss[q];
-    }
-    if (rlane) pp[392 + lane] = accR.value();
-  }
-}
-
-// schur_chunk_mfma for blocks of at most 8x8 (da, db <= 8), two pairs per
-// MFMA: rows 0-7 of both operands carry the even pair of a step, rows 8-15
+// The destination-block reduction of schur_chunk on the matrix cores, for
+// blocks of at most 8x8 (da, db <= 8; the window solver's row-stream
+// layout).  Per pair, lane l supplies one component, F_a[row][k] and
+// E_b[row][k] for the three point coordinates (K slot 3 zero), so one
+// 16x16x4 MFMA adds F_a E_b^T; a pair of the same observation adds
+// w J_a^T J_b with a second MFMA over the two residual components.  Two
+// pairs per MFMA: rows 0-7 of both operands carry the even pair of a step, rows 8-15
 // the odd pair, so the 16x16 tile's two diagonal quadrants sum the even and
 // the odd pairs of the chunk and its off-diagonal quadrants (cross terms)
 // are discarded.  Lane l loads row l % 8 of pair 2u + (l / 8) % 2,
@@ -765,66 +680,108 @@ __device__ inline void schur_chunk_mfma8(const View<T>& v, const uint4 ch, doubl
   }
 }
 
-// ------------------------------------------------ LDS-staged Schur chunk
+// ------------------------------------------------ incidence records
 typedef __attribute__((address_space(3))) void lds_void_t;
 
-// Pairs of one LDS batch of schur_chunk_lds: 8 KB of F and E row blocks
-// (and 4 KB of J2 blocks of both sides when the chunk has pairs of the same
-// observation).
-template <typename T>
-constexpr int lds_batch_pairs() {
-  return sizeof(T) == 4 ? 32 : 16;
-}
-template <typename T, bool SAME>
-constexpr int lds_chunk_elems() {
-  return lds_batch_pairs<T>() * (2 * 32 + (SAME ? 2 * 16 : 0));
+// Y = X2 Cinv and z = Y g_point - R2 into incidence a's record (rec_values),
+// after the damped point inverse.  A point left out of the Schur complement
+// has Cinv = 0, so Y = 0 and z = -R2: its pairs add only the camera block
+// J2_a^T J2_b of their observation and the gradient.  Rows of F = E Cinv
+// and the rhs share F g_point are J2 Y and J2 (Y g_point), so the assembly
+// needs no F (or E) rows: E = J2 X2.
+template <typename T, int DB>
+__device__ inline void compute_Y(const View<T>& v, uint32_t a) {
+  using V2 = typename Vec2<T>::type;
+  V2* rc = reinterpret_cast<V2*>(v.rec + (size_t)a * rec_values(DB));
+  const uint32_t pt = v.inc_pt[a];
+  const double* ci = v.pt_cinv + 9 * (size_t)pt;
+  T c[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) c[k] = (T)ci[k];
+  const T g0 = (T)v.pt_g[3 * (size_t)pt], g1 = (T)v.pt_g[3 * (size_t)pt + 1],
+          g2 = (T)v.pt_g[3 * (size_t)pt + 2];
+  const V2 q0 = rc[0], q1 = rc[1], q2 = rc[2], q3 = rc[3];
+  const T x[6] = {q0.x, q0.y, q1.x, q1.y, q2.x, q2.y};
+  T y[6];
+#pragma unroll
+  for (int q = 0; q < 2; ++q)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) y[3 * q + j] = x[3 * q] * c[j] + x[3 * q + 1] * c[3 + j] + x[3 * q + 2] * c[6 + j];
+  const T z0 = (y[0] * g0 + y[1] * g1 + y[2] * g2) - q3.x;
+  const T z1 = (y[3] * g0 + y[4] * g1 + y[5] * g2) - q3.y;
+  V2* o = rc + 4 + DB;
+  o[0] = V2{y[0], y[1]};
+  o[1] = V2{y[2], y[3]};
+  o[2] = V2{y[4], y[5]};
+  o[3] = V2{z0, z1};
 }
 
-// schur_chunk_mfma8 with its operands staged through LDS (blocks of at most
-// 8x8): the 8-row blocks of a batch of pairs move from global memory
-// straight into LDS (global_load_lds, 16 bytes per lane, one instruction per
-// 8 FP32 or 4 FP64 blocks, no registers held while they are in flight),
-// then the two-pairs-per-MFMA products read them from LDS.  The register
-// form is bound by its gathers' latency; here the bytes in flight per CU
-// are set by LDS instead of registers.  SAME: the chunk has pairs of the
-// same observation, so the sqrt(w) J blocks (two components per row) of
-// both sides are staged too, for w J_a^T J_b (a second MFMA, K = the two
-// residual components; b's block is a's when a == b, and is not loaded when
-// every pair of the chunk has a == b) and, on diagonal blocks, the
-// right-hand side F_a g_point - w J_a^T r over pairs a == b, in pair order.
-// lds: this wavefront's lds_chunk_elems<T, SAME>() elements.  Sums, FP32
-// flushing every two steps, the quadrant merge and the outputs are
-// schur_chunk_mfma8's.
-template <typename T, bool SAME>
-__device__ inline void schur_chunk_lds(const View<T>& v, const uint4 ch, double lambda, double* parts, T* lds) {
+// Shapes of schur_chunk_rec for records of DB rows.  A pair's a side (J2
+// rows, Y, z) and b side (X2, R2, J2 rows) are SIDE values each, moved by
+// LPS lanes of 16 bytes; one global_load_lds instruction (64 lanes, 1 KB of
+// LDS) carries NPI sides, and a batch is INS instructions per side.  Blocks
+// of at most 8 unknowns take four pairs per MFMA (two per 8x8 diagonal
+// quadrant of the 16x16 tile), larger ones two (one per K-slot pair).
+template <typename T, int DB>
+struct RecShape {
+  static constexpr uint32_t RS = rec_values(DB);
+  static constexpr uint32_t SIDE = 8 + 2 * DB;
+  static constexpr uint32_t LPS = SIDE * sizeof(T) / 16;
+  static constexpr uint32_t NPI = 64 / LPS;
+  static constexpr uint32_t INS = 4;
+  static constexpr uint32_t BP = INS * NPI;            // pairs per batch
+  static constexpr uint32_t IE = 1024 / sizeof(T);     // LDS elements per load instruction
+  static constexpr bool QUAD = DB <= 8;
+  static constexpr uint32_t PPM = QUAD ? 4 : 2;        // pairs per MFMA
+  static constexpr uint32_t LDS = 2 * INS * IE + 4 * BP;  // elements per wavefront
+  static_assert(SIDE * sizeof(T) % 16 == 0, "sides of whole 16-byte pieces");
+  static_assert(BP % PPM == 0 && BP <= 64, "batches of whole MFMA steps, one pair per lane");
+  static_assert(LDS * sizeof(T) % 16 == 0, "16-byte aligned wavefront stages");
+  // LDS offset of batch pair p's side (within its side's area)
+  static __device__ inline uint32_t at(uint32_t p) { return (p / NPI) * IE + (p % NPI) * SIDE; }
+};
+
+// One wavefront reduces one chunk of a destination block (ga >= gb) from
+// incidence records.  For a pair (a, b) of one point, with Y_a, X2_b of
+// their observations,
+//   F_a E_b^T = J2_a (Y_a X2_b^T) J2_b^T,
+// and a pair of the same observation also adds J2_a J2_b^T, so the pair
+// adds J2_a M J2_b^T with the 2 x 2 M = [same] I - Y_a X2_b^T.  Per batch:
+// the pairs' a and b sides move from global memory straight into LDS
+// (global_load_lds), every lane with a pair computes its M into LDS, then
+// each MFMA step takes PPM pairs: lane l supplies A = (J2_a M)[r][c] and
+// B = J2_b[r][c] for its pair, row r and residual component c.  On
+// diagonal blocks, the lanes below the block size also sum, in pair order
+// over pairs a == b, the right-hand side J2_a z_a (= F_a g_point - J2_a R2)
+// and the damping diagonal |J2_a row|^2.  FP32 MFMA sums are flushed into
+// FP64 after every step (two products per entry).  A chunk covering its
+// whole destination writes S (with the Marquardt damping of the diagonal,
+// lib.rs:923-956); others leave partials (sums, damping diagonal, rhs) for
+// k_schur_reduce_rec.  lds: RecShape<T, DB>::LDS elements of this wavefront.
+template <typename T, int DB>
+__device__ inline void schur_chunk_rec(const View<T>& v, const uint4 ch, double lambda, double* parts, T* lds) {
   using M = Mfma<T>;
   using acc_t = typename M::acc_t;
+  using Sh = RecShape<T, DB>;
   constexpr uint32_t HI = 0x80000000u, OFF = 0x7FFFFFFFu;
-  constexpr int BP = lds_batch_pairs<T>();
-  constexpr uint32_t ROWB = 4 * sizeof(T);  // bytes per row (4 components)
-  constexpr uint32_t LPB = 8 * ROWB / 16;   // lanes per 8-row block
-  constexpr uint32_t PPI = 64 / LPB;        // blocks per load instruction
-  static_assert(BP % PPI == 0 && 64 % BP == 0, "batches of whole load instructions within a 64-pair half");
   static_assert(PAIR_CHUNK <= 128, "a chunk's pair words are held in two registers per lane");
   const uint32_t lane = threadIdx.x & 63;
-  const uint32_t i = lane & 7, h = (lane >> 3) & 1, k = lane >> 4;
   const uint32_t ga = ch.x >> 16, gb = ch.x & 0xFFFFu;
   const uint32_t da = v.grp_dim[ga], db = v.grp_dim[gb];
   const bool diag = ga == gb;
-  const bool fa_on = i < da && k < 3, eb_on = i < db && k < 3;
-  const bool ja_on = i < da && k < 2, jb_on = i < db && k < 2;
   const bool rlane = diag && lane < da;
-  const char* Fg = reinterpret_cast<const char*>(v.F4);
-  const char* Eg = reinterpret_cast<const char*>(v.E4);
-  const char* Jg = reinterpret_cast<const char*>(v.J2);
-  constexpr uint32_t JROWB = 2 * sizeof(T);  // bytes per J2 row
-  constexpr uint32_t JLPB = 8 * JROWB / 16;  // lanes per 8-row J2 block
-  constexpr uint32_t JPPI = 64 / JLPB;       // J2 blocks per load instruction
-  static_assert(BP % JPPI == 0, "whole J2 load instructions per batch");
-  T* Fs = lds;
-  T* Es = lds + BP * 32;
-  T* JAs = lds + 2 * BP * 32;
-  T* JBs = lds + 2 * BP * 32 + BP * 16;
+  // MFMA roles of this lane: row r (a block) / column r (b block), residual
+  // component c (its K slot within the pair), pair pi of the step
+  const uint32_t c = (lane >> 4) & 1;
+  const uint32_t r = Sh::QUAD ? (lane & 7) : (lane & 15);
+  const uint32_t pi = Sh::QUAD ? 2 * ((lane >> 3) & 1) + (lane >> 5) : (lane >> 5);
+  const uint32_t rr = r < DB ? r : 0u;  // in-record row of the address (masked below)
+  const bool a_on = r < da, b_on = r < db;
+  T* As = lds;
+  T* Bs = lds + Sh::INS * Sh::IE;
+  T* Ms = lds + 2 * Sh::INS * Sh::IE;
+  const char* recb = reinterpret_cast<const char*>(v.rec);
+  constexpr size_t RSB = Sh::RS * sizeof(T);
   const uint32_t n = ch.z - ch.y;
   const uint2 w0 = lane < n ? v.pairs[ch.y + lane] : make_uint2(0u, 0u);
   const uint2 w1 = lane + 64 < n ? v.pairs[ch.y + 64 + lane] : make_uint2(0u, 0u);
@@ -832,127 +789,119 @@ __device__ inline void schur_chunk_lds(const View<T>& v, const uint4 ch, double 
   // observation, and a == b
   const uint64_t same_m[2] = {__ballot((w0.x & HI) != 0), __ballot((w1.x & HI) != 0)};
   const uint64_t eq_m[2] = {__ballot((w0.y & HI) != 0), __ballot((w1.y & HI) != 0)};
-  // byte offset of this lane's 16 bytes in a block; rows past the block's
-  // dimension re-read row 0 of the same block (in bounds, masked below)
-  const uint32_t off = (lane % LPB) * 16;
-  const uint32_t offa = off / ROWB < da ? off : off % ROWB;
-  const uint32_t offb = off / ROWB < db ? off : off % ROWB;
-  // J2 blocks: FP32 rows are 8 bytes, so a lane's 16 bytes may end one row
-  // past the block (the row array is padded by a block for that)
-  const uint32_t joff = (lane % JLPB) * 16;
-  const uint32_t joffa = joff / JROWB < da ? joff : 0u;
-  const uint32_t joffb = joff / JROWB < db ? joff : 0u;
-  const bool all_eq = SAME && (uint32_t)(__popcll(eq_m[0]) + __popcll(eq_m[1])) == n;
-  acc_t s = {T(0), T(0), T(0), T(0)}, b = {T(0), T(0), T(0), T(0)};
-  double ss[4] = {0.0, 0.0, 0.0, 0.0}, bs[4] = {0.0, 0.0, 0.0, 0.0};
+  // this lane's side slot and 16-byte piece in each load instruction
+  const uint32_t slot = lane / Sh::LPS, piece = (lane % Sh::LPS) * 16;
+  acc_t s = {T(0), T(0), T(0), T(0)};
+  double ss[4] = {0.0, 0.0, 0.0, 0.0};
   Acc<T> accR;
-  for (uint32_t q0 = 0; q0 < n; q0 += BP) {
-    const bool upper = q0 >= 64;
-    // pairs q0 .. q0 + BP - 1 into LDS; a lane past the chunk's end loads
-    // pair q0's block again (never read back)
+  double bdg = 0.0;
+  for (uint32_t q0 = 0; q0 < n; q0 += Sh::BP) {
+    const uint32_t nb = min(Sh::BP, n - q0);
+    // sides of pairs q0 .. q0 + nb - 1 into LDS; spare lanes and pairs past
+    // the chunk's end load pair q0's sides again (never read back)
 #pragma unroll
-    for (int j = 0; j < BP / (int)PPI; ++j) {
-      const uint32_t q = q0 + j * PPI;
-      uint32_t idx = q + lane / LPB;
-      if (idx >= n) idx = q0;
-      const uint32_t src = idx & 63;
-      const uint32_t ra = (uint32_t)__shfl((int)(upper ? w1.x : w0.x), (int)src) & OFF;
-      const uint32_t rb = (uint32_t)__shfl((int)(upper ? w1.y : w0.y), (int)src) & OFF;
-      __builtin_amdgcn_global_load_lds((const void*)(Fg + (size_t)ra * ROWB + offa),
-                                       (lds_void_t*)(Fs + j * PPI * 32), 16, 0, 0);
-      __builtin_amdgcn_global_load_lds((const void*)(Eg + (size_t)rb * ROWB + offb),
-                                       (lds_void_t*)(Es + j * PPI * 32), 16, 0, 0);
-    }
-    if (SAME) {
-#pragma unroll
-      for (int j = 0; j < BP / (int)JPPI; ++j) {
-        const uint32_t q = q0 + j * JPPI;
-        if (q >= n) break;
-        uint32_t idx = q + lane / JLPB;
-        if (idx >= n) idx = q0;
-        const uint32_t src = idx & 63;
-        const uint32_t ra = (uint32_t)__shfl((int)(upper ? w1.x : w0.x), (int)src) & OFF;
-        __builtin_amdgcn_global_load_lds((const void*)(Jg + (size_t)ra * JROWB + joffa),
-                                         (lds_void_t*)(JAs + j * JPPI * 16), 16, 0, 0);
-        if (!all_eq) {
-          const uint32_t rb = (uint32_t)__shfl((int)(upper ? w1.y : w0.y), (int)src) & OFF;
-          __builtin_amdgcn_global_load_lds((const void*)(Jg + (size_t)rb * JROWB + joffb),
-                                           (lds_void_t*)(JBs + j * JPPI * 16), 16, 0, 0);
-        }
-      }
+    for (uint32_t j = 0; j < Sh::INS; ++j) {
+      if (j * Sh::NPI >= nb) break;
+      uint32_t idx = q0 + j * Sh::NPI + slot;
+      if (slot >= Sh::NPI || idx >= n) idx = q0;
+      const int src = (int)(idx & 63);
+      const uint32_t xl = (uint32_t)__shfl((int)w0.x, src), xh = (uint32_t)__shfl((int)w1.x, src);
+      const uint32_t yl = (uint32_t)__shfl((int)w0.y, src), yh = (uint32_t)__shfl((int)w1.y, src);
+      const uint32_t ra = (idx < 64 ? xl : xh) & OFF, rb = (idx < 64 ? yl : yh) & OFF;
+      __builtin_amdgcn_global_load_lds((const void*)(recb + (size_t)ra * RSB + 8 * sizeof(T) + piece),
+                                       (lds_void_t*)(As + j * Sh::IE), 16, 0, 0);
+      __builtin_amdgcn_global_load_lds((const void*)(recb + (size_t)rb * RSB + piece),
+                                       (lds_void_t*)(Bs + j * Sh::IE), 16, 0, 0);
     }
     __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
-    const uint32_t nb = min((uint32_t)BP, n - q0);
-    const uint64_t sm = same_m[upper ? 1 : 0] >> (q0 & 63), em = eq_m[upper ? 1 : 0] >> (q0 & 63);
-    for (uint32_t u = 0; u < nb; u += 2) {
-      const uint32_t p = u + h;
+    // M of every pair of the batch, column-major: Ms[4 p + 2 c + c'] = M[c'][c]
+    if (lane < nb) {
+      const uint32_t idx = q0 + lane;
+      const bool same = (((idx < 64 ? same_m[0] : same_m[1]) >> (idx & 63)) & 1u) != 0;
+      const T* ya = As + Sh::at(lane) + 2 * DB;
+      const T* xb = Bs + Sh::at(lane);
+      const T one = same ? T(1) : T(0);
+      Ms[4 * lane + 0] = one - (ya[0] * xb[0] + ya[1] * xb[1] + ya[2] * xb[2]);
+      Ms[4 * lane + 1] = -(ya[3] * xb[0] + ya[4] * xb[1] + ya[5] * xb[2]);
+      Ms[4 * lane + 2] = -(ya[0] * xb[3] + ya[1] * xb[4] + ya[2] * xb[5]);
+      Ms[4 * lane + 3] = one - (ya[3] * xb[3] + ya[4] * xb[4] + ya[5] * xb[5]);
+    }
+    __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    for (uint32_t u = 0; u < nb; u += Sh::PPM) {
+      const uint32_t p = u + pi;
       const bool ok = p < nb;
-      const T fa = (ok && fa_on) ? Fs[p * 32 + i * 4 + k] : T(0);
-      const T eb = (ok && eb_on) ? Es[p * 32 + i * 4 + k] : T(0);
-      s = M::step(fa, eb, s);
-      if (SAME) {
-        if ((sm >> u) & 3u) {
-          const bool same = ok && ((sm >> p) & 1u);
-          const T* jbs = ((em >> p) & 1u) ? JAs : JBs;  // a == b: b's block is a's
-          const T ja = (same && ja_on) ? JAs[p * 16 + i * 2 + k] : T(0);
-          const T jb = (same && jb_on) ? jbs[p * 16 + i * 2 + k] : T(0);
-          b = M::step(ja, jb, b);
-        }
-        // right-hand side in pair order
-        if (rlane && ((em >> u) & 1u)) accR.add(Fs[u * 32 + lane * 4 + 3] - Es[u * 32 + lane * 4 + 3]);
-        if (rlane && u + 1 < nb && ((em >> (u + 1)) & 1u))
-          accR.add(Fs[(u + 1) * 32 + lane * 4 + 3] - Es[(u + 1) * 32 + lane * 4 + 3]);
-      }
-      if (sizeof(T) == 4 && (u & 2)) {
-        // FP32: at most two products per quadrant summed before FP64
+      const uint32_t pq = ok ? p : 0u;
+      const T* a = As + Sh::at(pq);
+      const T* b = Bs + Sh::at(pq);
+      const T m0 = Ms[4 * pq + 2 * c], m1 = Ms[4 * pq + 2 * c + 1];
+      const T j0 = a[2 * rr], j1 = a[2 * rr + 1];
+      const T av = (ok && a_on) ? j0 * m0 + j1 * m1 : T(0);
+      const T bv = (ok && b_on) ? b[8 + 2 * rr + c] : T(0);
+      s = M::step(av, bv, s);
+      if (sizeof(T) == 4) {
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          ss[c] += (double)s[c];
-          bs[c] += (double)b[c];
-          s[c] = T(0);
-          b[c] = T(0);
+        for (int q = 0; q < 4; ++q) {
+          ss[q] += (double)s[q];
+          s[q] = T(0);
+        }
+      }
+      if (diag) {
+        // right-hand side and damping diagonal, in pair order
+#pragma unroll
+        for (uint32_t t = 0; t < Sh::PPM; ++t) {
+          const uint32_t q = u + t, idx = q0 + q;
+          if (q < nb && (((idx < 64 ? eq_m[0] : eq_m[1]) >> (idx & 63)) & 1u) && rlane) {
+            const T* aq = As + Sh::at(q);
+            const T g0 = aq[2 * lane], g1 = aq[2 * lane + 1];
+            accR.add(g0 * aq[2 * DB + 6] + g1 * aq[2 * DB + 7]);
+            bdg += (double)g0 * (double)g0 + (double)g1 * (double)g1;
+          }
         }
       }
     }
     // the next batch's loads overwrite this one: every lane's reads first
     __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
   }
+  if (sizeof(T) == 8) {
 #pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    ss[c] += (double)s[c];
-    bs[c] += (double)b[c];
+    for (int q = 0; q < 4; ++q) ss[q] = (double)s[q];
   }
-  // Quadrant (8 + r, 8 + c) sits 40 lanes up in the same register (FP32
-  // layout) or 8 lanes up two registers on (FP64 layout).
+  if (Sh::QUAD) {
+    // Quadrant (8 + r, 8 + c) sits 40 lanes up in the same register (FP32
+    // layout, row 4 (l / 16) + v) or 8 lanes up two registers on (FP64
+    // layout, row l / 16 + 4 v); only the registers of rows 0-7 combine.
 #pragma unroll
-  for (int q = 0; q < (sizeof(T) == 4 ? 4 : 2); ++q) {
-    const int src = sizeof(T) == 4 ? q : q + 2;
-    const int dl = sizeof(T) == 4 ? 40 : 8;
-    ss[q] += __shfl(ss[src], (int)((lane + dl) & 63));
-    bs[q] += __shfl(bs[src], (int)((lane + dl) & 63));
+    for (int q = 0; q < (sizeof(T) == 4 ? 4 : 2); ++q) {
+      const int src = sizeof(T) == 4 ? q : q + 2;
+      const int dl = sizeof(T) == 4 ? 40 : 8;
+      ss[q] += __shfl(ss[src], (int)((lane + dl) & 63));
+    }
   }
   const uint32_t oa = v.grp_off[ga], ob = v.grp_off[gb];
-  const uint32_t c = lane & 15;
+  const uint32_t col = lane & 15;
+  const double bdc = __shfl(bdg, (int)col);  // damping diagonal of row col (diagonal blocks)
   if (ch.w == NONE) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      const uint32_t r = M::row(lane, q);
-      if (r >= da || c >= db || (diag && r < c)) continue;
-      double val = bs[q] - ss[q];
-      if (diag && r == c) val = (bs[q] + bs[q] * lambda) - ss[q];
-      v.S[(size_t)(ob + c) * v.ld + oa + r] = val;
+      const uint32_t row = M::row(lane, q);
+      if (row >= da || col >= db || (diag && row < col)) continue;
+      double val = ss[q];
+      if (diag && row == col) val += lambda * bdc;
+      v.S[(size_t)(ob + col) * v.ld + oa + row] = val;
     }
     if (rlane) v.S[(size_t)(oa + lane) * v.ld + v.Dp] = accR.value();
   } else {
     double* pp = parts + (size_t)ch.w * PART_STRIDE;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      const uint32_t r = M::row(lane, q);
-      if (r >= da || c >= db) continue;
-      pp[r + c * da] = bs[q];
-      pp[196 + r + c * da] = ss[q];
+      const uint32_t row = M::row(lane, q);
+      if (row >= da || col >= db) continue;
+      pp[row + col * da] = ss[q];
     }
-    if (rlane) pp[392 + lane] = accR.value();
+    if (rlane) {
+      pp[196 + lane] = bdg;
+      pp[392 + lane] = accR.value();
+    }
   }
 }
 

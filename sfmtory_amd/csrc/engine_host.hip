@@ -128,9 +128,11 @@ struct Pack {
 };
 
 // pair_chunk: most pairs one Schur chunk reduces (0: no limit, one chunk
-// per destination).
+// per destination).  rec_pairs: pair words name incidences, whose records
+// the single-problem solve reads, instead of first rows (the window
+// solver's row streams).
 static void build_pack(const sfmba_input* in, Pack& P, bool need_pairs,
-                       uint32_t pair_chunk = PAIR_CHUNK) {
+                       uint32_t pair_chunk = PAIR_CHUNK, bool rec_pairs = false) {
   P.n_img = (uint32_t)in->num_images;
   P.n_cam = (uint32_t)in->num_cameras;
   P.n_pt = (uint32_t)in->num_points;
@@ -316,11 +318,13 @@ static void build_pack(const sfmba_input* in, Pack& P, bool need_pairs,
           const uint64_t ga = P.inc_grp[a], gb = P.inc_grp[b];
           if (ga < gb) continue;
           const uint32_t q = pos[ga * (ga + 1) / 2 + gb]++;
-          // First word: a's first row, bit 31 "same observation".  Second
-          // word: b's first row, bit 31 "a == b".
+          // First word: a (or a's first row), bit 31 "same observation".
+          // Second word: b (or b's first row), bit 31 "a == b".
           const bool same = P.inc_obs[a] == P.inc_obs[b];
-          P.pairs[2 * (size_t)q] = P.inc_froff[a] | (same ? 0x80000000u : 0u);
-          P.pairs[2 * (size_t)q + 1] = P.inc_froff[b] | (a == b ? 0x80000000u : 0u);
+          const uint32_t wa = rec_pairs ? a : P.inc_froff[a];
+          const uint32_t wb = rec_pairs ? b : P.inc_froff[b];
+          P.pairs[2 * (size_t)q] = wa | (same ? 0x80000000u : 0u);
+          P.pairs[2 * (size_t)q + 1] = wb | (a == b ? 0x80000000u : 0u);
         }
     }
   }
@@ -375,6 +379,7 @@ static void upload_layout(const Pack& P, Problem& pr, hipStream_t s) {
   L.n_multi = (uint32_t)P.multi_dest.size();
   L.n_parts = P.n_parts;
   for (uint32_t g : P.grp_dim) L.max_grp_dim = std::max(L.max_grp_dim, g);
+  L.rec_db = rec_db_for(L.max_grp_dim);
   L.obs_img = A.upload(P.obs_img, s);
   L.obs_pt = A.upload(P.obs_pt, s);
   L.obs_xy = A.upload(P.obs_xy, s);
@@ -561,7 +566,7 @@ static sfmba_problem* create_problem(const sfmba_input* in, int device) {
   HIP_CHECK(hipHostMalloc((void**)&H->pinned, 64, hipHostMallocDefault));
   const auto t0 = clock::now();
   Pack P;
-  build_pack(in, P, true);
+  build_pack(in, P, true, PAIR_CHUNK, true);
   const auto t1 = clock::now();
   upload_layout(P, H->pr, H->s);
   H->init_poses = H->pr.arena.upload(P.poses, H->s);
@@ -640,9 +645,10 @@ static void alloc_work(sfmba_problem& H, Work<T>& W, double* shared_S = nullptr,
   if (L.D == 0) return;
   W.chain = A.alloc<uint32_t>(L.Dp / NB);
   HIP_CHECK(hipMemsetAsync(W.chain, 0, (L.Dp / NB) * sizeof(uint32_t), s));
-  W.J2 = A.alloc<T>(2 * ((size_t)L.n_rows + 8));
-  W.E4 = A.alloc<T>(4 * (size_t)L.n_rows);
-  W.F4 = A.alloc<T>(4 * (size_t)L.n_rows);
+  // records: zeroed once (rows past a block's dimension are never written)
+  const size_t rec_elems = (size_t)std::max<uint32_t>(L.n_inc, 1) * rec_values(L.rec_db);
+  W.rec = A.alloc<T>(rec_elems);
+  HIP_CHECK(hipMemsetAsync(W.rec, 0, rec_elems * sizeof(T), s));
   W.obs_pc = A.alloc<T>(9 * (size_t)L.n_obs);
   W.obs_valid = A.alloc<uint8_t>(L.n_obs);
   W.pt_C = A.alloc<double>(6 * (size_t)L.n_pt);

@@ -14,11 +14,11 @@ namespace sfmba {
 // ms, FP64 0.94 -> 0.81 ms against one lane; eight lanes: no gain), and
 // they also halve the FP64 kernel's registers (178 -> 111).
 constexpr int LINEARIZE_LANES = 4;
-template <typename T, int LPO>
+template <typename T, int LPO, int DB>
 __global__ __launch_bounds__(256) void k_linearize(View<T> v, int cur, int loss, double scale) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t o = t / LPO;
-  if (o < v.n_obs) linearize_obs<T, LPO>(v, o, cur, loss, scale, t % LPO);
+  if (o < v.n_obs) linearize_obs<T, LPO, DB>(v, o, cur, loss, scale, t % LPO);
 }
 
 // LPP lanes per point (neighbouring lanes read neighbouring observations).
@@ -36,67 +36,29 @@ __global__ __launch_bounds__(256) void k_point_prepare(View<T> v, double lambda)
   if (p < v.n_pt) point_prepare(v, p, lambda);
 }
 
-// LPI lanes per incidence (a power of two >= its group dimension), one row
-// per lane.
-template <typename T, int LPI>
-__global__ __launch_bounds__(256) void k_compute_F(View<T> v) {
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t a = t / LPI;
-  if (a < v.n_inc) compute_F_row(v, a, t % LPI);
+// Y and z of every incidence record, one lane per incidence.
+template <typename T, int DB>
+__global__ __launch_bounds__(256) void k_compute_Y(View<T> v) {
+  const uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a < v.n_inc) compute_Y<T, DB>(v, a);
 }
 
-// The Schur reduction on the matrix cores, one wavefront per chunk, chunks
-// in destination-tile order: the wavefronts in flight work on neighbouring
-// destinations and share their F and E rows in L2 and the MALL.
-// Steps in flight per wavefront and the occupancy asked for, measured on
-// the 1,000-camera scene (every block 8x8, so the packed form: two pairs
-// per step):
-//   FP32  two steps at seven waves per SIMD (68 VGPRs): 1.72 ms per
-//         assembly; eight waves spill (2.00 ms), one step at eight waves
-//         1.80 ms, three steps at six waves 1.78 ms, four steps at four
-//         waves 1.88 ms;
-//   FP64  two steps at six waves (77 VGPRs): 2.10 ms; the compiler's choice
-//         (five waves) 2.31 ms, three steps at five waves 2.22 ms, four
-//         steps 2.45 ms.
-// The one-pair form took 2.02 ms (FP32, eight waves) and 3.07 ms (FP64,
-// five waves) on the same scene; it keeps those settings for blocks above
-// 8x8.  The gathers' latency, not bandwidth or issue, bounds this kernel,
-// so the pairs in flight per SIMD are what count.
-template <typename T>
-constexpr int schur_pairs() {
-  return 2;
-}
-template <typename T, bool PACK>
-constexpr int schur_min_waves() {
-  return PACK ? (sizeof(T) == 4 ? 7 : 6) : (sizeof(T) == 4 ? 8 : 1);
-}
-// PACK: every block is at most 8x8 (the largest parameter block of the
-// problem has at most 8 unknowns), two pairs per MFMA.
-template <typename T, bool PACK>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(schur_min_waves<T, PACK>()))) void k_schur_mfma(
-    View<T> v, double lambda, double* __restrict__ parts) {
-  const uint32_t ci = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
-  if (ci >= v.n_chunks) return;
-  if (PACK)
-    schur_chunk_mfma8<T, schur_pairs<T>()>(v, v.chunks[ci], lambda, parts);
-  else
-    schur_chunk_mfma<T, schur_pairs<T>()>(v, v.chunks[ci], lambda, parts);
-}
-
-// The LDS-staged Schur reduction (every block at most 8x8), one wavefront
-// per chunk: SAME = false for the chunks without same-observation pairs,
-// true for the others.
-template <typename T, bool SAME>
-__global__ __launch_bounds__(256) void k_schur_lds(View<T> v, double lambda, double* __restrict__ parts) {
-  constexpr int PER_WAVE = lds_chunk_elems<T, SAME>();
-  __shared__ T stage[4 * PER_WAVE];
+// The Schur reduction from incidence records, one wavefront per chunk,
+// chunks in destination-tile order (the wavefronts in flight work on
+// neighbouring destinations and share their records in L2 and the MALL).
+// Four wavefronts per workgroup, each with its own LDS stage.
+template <typename T, int DB>
+__global__ __launch_bounds__(256) void k_schur_rec(View<T> v, double lambda, double* __restrict__ parts) {
+  constexpr uint32_t PER_WAVE = RecShape<T, DB>::LDS;
+  __shared__ __attribute__((aligned(16))) T stage[4 * PER_WAVE];
   const uint32_t w = threadIdx.x >> 6;
   const uint32_t ci = blockIdx.x * 4 + w;
   if (ci >= v.n_chunks) return;
-  schur_chunk_lds<T, SAME>(v, v.chunks[ci], lambda, parts, stage + w * PER_WAVE);
+  schur_chunk_rec<T, DB>(v, v.chunks[ci], lambda, parts, stage + w * PER_WAVE);
 }
 
-// Destinations split over several chunks: sum the partials in chunk order.
+// Destinations split over several chunks: sum the partials in chunk order
+// (sums, then on diagonal blocks the damping diagonal and the rhs).
 __global__ __launch_bounds__(256) void k_schur_reduce(
     uint32_t n_multi, const uint32_t* __restrict__ multi_dest,
     const uint32_t* __restrict__ dest_part0, const uint32_t* __restrict__ dest_nparts,
@@ -114,15 +76,14 @@ __global__ __launch_bounds__(256) void k_schur_reduce(
   for (uint32_t e = lane; e < ne; e += 64) {
     const uint32_t r = e % da, c = e / da;
     if (diag && r < c) continue;
-    double bv = 0.0, sv = 0.0;
-    for (uint32_t k = 0; k < np; ++k) {
-      const double* pp = parts + (size_t)(p0 + k) * PART_STRIDE;
-      bv += pp[e];
-      sv += pp[196 + e];
+    double sv = 0.0;
+    for (uint32_t k = 0; k < np; ++k) sv += parts[(size_t)(p0 + k) * PART_STRIDE + e];
+    if (diag && r == c) {
+      double bd = 0.0;
+      for (uint32_t k = 0; k < np; ++k) bd += parts[(size_t)(p0 + k) * PART_STRIDE + 196 + r];
+      sv += lambda * bd;
     }
-    double v = bv - sv;
-    if (diag && r == c) v = (bv + bv * lambda) - sv;
-    S[(size_t)(ob + c) * ld + oa + r] = v;
+    S[(size_t)(ob + c) * ld + oa + r] = sv;
   }
   if (diag && lane < da) {
     double rv = 0.0;
@@ -131,35 +92,33 @@ __global__ __launch_bounds__(256) void k_schur_reduce(
   }
 }
 
-// Point back-substitution in two passes.  First E_a^T delta_g per
-// incidence a: LPI lanes per incidence, one row per lane (its rows are
-// read as whole cache lines), summed across the lanes.
-template <typename T, int LPI>
+// Point back-substitution in two passes.  First E_a^T delta_g = X2^T
+// (J2_a^T delta_g) per incidence a: LPI lanes per incidence, one J2 row per
+// lane, summed across the lanes.
+template <typename T, int DB, int LPI>
 __global__ __launch_bounds__(256) void k_backsub_inc(View<T> v, double* __restrict__ inc_part) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t a = t / LPI, r = t % LPI;
-  double c0 = 0.0, c1 = 0.0, c2 = 0.0;
+  double t0 = 0.0, t1 = 0.0;
+  const T* rc = nullptr;
   if (a < v.n_inc) {
     const uint32_t g = v.inc_grp[a];
+    rc = v.rec + (size_t)a * rec_values(DB);
     if (r < v.grp_dim[g]) {
-      const auto e = v.E4[v.inc_roff[a] + r];
       const double d = v.delta[v.grp_off[g] + r];
-      c0 = (double)e.x * d;
-      c1 = (double)e.y * d;
-      c2 = (double)e.z * d;
+      t0 = (double)rc[8 + 2 * r] * d;
+      t1 = (double)rc[8 + 2 * r + 1] * d;
     }
   }
   // every lane of the wavefront takes part (LPI divides 64)
 #pragma unroll
   for (int o = LPI / 2; o > 0; o >>= 1) {
-    c0 += __shfl_xor(c0, o);
-    c1 += __shfl_xor(c1, o);
-    c2 += __shfl_xor(c2, o);
+    t0 += __shfl_xor(t0, o);
+    t1 += __shfl_xor(t1, o);
   }
   if (a < v.n_inc && r == 0) {
-    inc_part[3 * (size_t)a] = c0;
-    inc_part[3 * (size_t)a + 1] = c1;
-    inc_part[3 * (size_t)a + 2] = c2;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) inc_part[3 * (size_t)a + j] = (double)rc[j] * t0 + (double)rc[3 + j] * t1;
   }
 }
 
@@ -353,7 +312,6 @@ static inline uint32_t lanes_per_incidence(const Layout& L) {
 
 template <typename T>
 View<T> make_view(const Layout& L, Work<T>& W) {
-  using V4 = typename Vec4<T>::type;
   View<T> v{};
   v.n_img = L.n_img;
   v.n_cam = L.n_cam;
@@ -384,9 +342,7 @@ View<T> make_view(const Layout& L, Work<T>& W) {
   v.grp_dim = L.grp_dim;
   v.pairs = reinterpret_cast<const uint2*>(L.pairs);
   v.chunks = L.chunks;
-  v.J2 = reinterpret_cast<typename View<T>::V2*>(W.J2);
-  v.E4 = reinterpret_cast<V4*>(W.E4);
-  v.F4 = reinterpret_cast<V4*>(W.F4);
+  v.rec = W.rec;
   v.obs_pc = W.obs_pc;
   v.obs_valid = W.obs_valid;
   v.pt_C = W.pt_C;
@@ -408,8 +364,13 @@ template <typename T>
 void launch_linearize(const Layout& L, Work<T>& W, int cur, int loss, double scale, hipStream_t s) {
   if (L.n_obs == 0) return;
   constexpr int LPO = LINEARIZE_LANES;
-  hipLaunchKernelGGL((k_linearize<T, LPO>), dim3(blocks_for((uint64_t)L.n_obs * LPO, 256)), dim3(256), 0, s,
-                     make_view(L, W), cur, loss, scale);
+  const dim3 grid(blocks_for((uint64_t)L.n_obs * LPO, 256));
+  const View<T> v = make_view(L, W);
+  switch (L.rec_db) {
+    case 8: hipLaunchKernelGGL((k_linearize<T, LPO, 8>), grid, dim3(256), 0, s, v, cur, loss, scale); break;
+    case 12: hipLaunchKernelGGL((k_linearize<T, LPO, 12>), grid, dim3(256), 0, s, v, cur, loss, scale); break;
+    default: hipLaunchKernelGGL((k_linearize<T, LPO, 16>), grid, dim3(256), 0, s, v, cur, loss, scale); break;
+  }
   HIP_CHECK(hipGetLastError());
 }
 
@@ -428,13 +389,11 @@ void launch_point_prepare(const Layout& L, Work<T>& W, double lambda, hipStream_
   hipLaunchKernelGGL(k_point_prepare<T>, dim3(blocks_for(L.n_pt, 256)), dim3(256), 0, s, v, lambda);
   HIP_CHECK(hipGetLastError());
   if (L.n_inc == 0) return;
-  const uint32_t lpi = lanes_per_incidence(L);
-  const dim3 grid(blocks_for((uint64_t)L.n_inc * lpi, 256));
-  switch (lpi) {
-    case 2: hipLaunchKernelGGL((k_compute_F<T, 2>), grid, dim3(256), 0, s, v); break;
-    case 4: hipLaunchKernelGGL((k_compute_F<T, 4>), grid, dim3(256), 0, s, v); break;
-    case 8: hipLaunchKernelGGL((k_compute_F<T, 8>), grid, dim3(256), 0, s, v); break;
-    default: hipLaunchKernelGGL((k_compute_F<T, 16>), grid, dim3(256), 0, s, v); break;
+  const dim3 grid(blocks_for(L.n_inc, 256));
+  switch (L.rec_db) {
+    case 8: hipLaunchKernelGGL((k_compute_Y<T, 8>), grid, dim3(256), 0, s, v); break;
+    case 12: hipLaunchKernelGGL((k_compute_Y<T, 12>), grid, dim3(256), 0, s, v); break;
+    default: hipLaunchKernelGGL((k_compute_Y<T, 16>), grid, dim3(256), 0, s, v); break;
   }
   HIP_CHECK(hipGetLastError());
 }
@@ -444,24 +403,11 @@ void launch_schur(const Layout& L, Work<T>& W, double lambda, hipStream_t s) {
   const View<T> v = make_view(L, W);
   if (L.n_chunks) {
     if (L.max_grp_dim > 16) throw Error(SFMBA_ERR_INVALID_ARGUMENT, "group dimension above 16");
-    if (L.max_grp_dim <= 8) {
-      // LDS-staged: the chunks without same-observation pairs, then those
-      // with them (which stage the J blocks too)
-      View<T> vp = v, vr = v;
-      vp.n_chunks = L.n_chunks_plain;
-      vr.chunks = L.chunks + L.n_chunks_plain;
-      vr.n_chunks = L.n_chunks - L.n_chunks_plain;
-      if (vp.n_chunks) {
-        hipLaunchKernelGGL((k_schur_lds<T, false>), dim3(blocks_for(vp.n_chunks, 4)), dim3(256), 0, s, vp, lambda,
-                           W.parts);
-        HIP_CHECK(hipGetLastError());
-      }
-      if (vr.n_chunks)
-        hipLaunchKernelGGL((k_schur_lds<T, true>), dim3(blocks_for(vr.n_chunks, 4)), dim3(256), 0, s, vr, lambda,
-                           W.parts);
-    } else {
-      hipLaunchKernelGGL((k_schur_mfma<T, false>), dim3(blocks_for(L.n_chunks, 4)), dim3(256), 0, s, v, lambda,
-                         W.parts);
+    const dim3 grid(blocks_for(L.n_chunks, 4));
+    switch (L.rec_db) {
+      case 8: hipLaunchKernelGGL((k_schur_rec<T, 8>), grid, dim3(256), 0, s, v, lambda, W.parts); break;
+      case 12: hipLaunchKernelGGL((k_schur_rec<T, 12>), grid, dim3(256), 0, s, v, lambda, W.parts); break;
+      default: hipLaunchKernelGGL((k_schur_rec<T, 16>), grid, dim3(256), 0, s, v, lambda, W.parts); break;
     }
     HIP_CHECK(hipGetLastError());
   }
@@ -478,12 +424,20 @@ void launch_backsub_update(const Layout& L, Work<T>& W, int cur, hipStream_t s) 
   const View<T> v = make_view(L, W);
   if (L.n_pt) {
     if (L.n_inc) {
-      const dim3 grid(blocks_for((uint64_t)L.n_inc * lanes_per_incidence(L), 256));
-      switch (lanes_per_incidence(L)) {
-        case 2: hipLaunchKernelGGL((k_backsub_inc<T, 2>), grid, dim3(256), 0, s, v, W.inc_part); break;
-        case 4: hipLaunchKernelGGL((k_backsub_inc<T, 4>), grid, dim3(256), 0, s, v, W.inc_part); break;
-        case 8: hipLaunchKernelGGL((k_backsub_inc<T, 8>), grid, dim3(256), 0, s, v, W.inc_part); break;
-        default: hipLaunchKernelGGL((k_backsub_inc<T, 16>), grid, dim3(256), 0, s, v, W.inc_part); break;
+      const uint32_t lpi = lanes_per_incidence(L);
+      const dim3 grid(blocks_for((uint64_t)L.n_inc * lpi, 256));
+      const dim3 blk(256);
+      double* ip = W.inc_part;
+      if (L.rec_db == 8) {
+        switch (lpi) {
+          case 2: hipLaunchKernelGGL((k_backsub_inc<T, 8, 2>), grid, blk, 0, s, v, ip); break;
+          case 4: hipLaunchKernelGGL((k_backsub_inc<T, 8, 4>), grid, blk, 0, s, v, ip); break;
+          default: hipLaunchKernelGGL((k_backsub_inc<T, 8, 8>), grid, blk, 0, s, v, ip); break;
+        }
+      } else if (L.rec_db == 12) {
+        hipLaunchKernelGGL((k_backsub_inc<T, 12, 16>), grid, blk, 0, s, v, ip);
+      } else {
+        hipLaunchKernelGGL((k_backsub_inc<T, 16, 16>), grid, blk, 0, s, v, ip);
       }
       HIP_CHECK(hipGetLastError());
     }

@@ -63,15 +63,22 @@ def test_multi_block_reduced_system(n_img):
     assert_parity(sc.inp, ba.BaParams(max_iterations=15))
 
 
-@pytest.mark.parametrize("n_free", [1, 2, 3, 4])
-def test_parameter_block_sizes_on_both_schur_forms(n_free):
-    # A pose block (6) and its own camera's free SIMPLE_RADIAL parameters
-    # form one block: 7 or 8 unknowns take the Schur assembly's two-pairs-
-    # per-MFMA form (every block at most 8x8), 9 or 10 the one-pair form.
-    sc = scenes.generate("c0", num_points=400, seed=13 + n_free)
+@pytest.mark.parametrize("model,fixed", [
+    (2, (1, 2, 3)), (2, (2, 3)), (2, (3,)), (2, ()),  # SIMPLE_RADIAL: blocks of 7 to 10 unknowns
+    (4, (2, 3)), (4, (3,)), (4, ()),                  # OPENCV: blocks of 12, 13 and 14
+])
+def test_parameter_block_sizes_on_both_schur_forms(model, fixed):
+    # A pose block (6) and its own camera's free parameters form one block.
+    # The Schur assembly reads incidence records of 8, 12 or 16 rows (the
+    # largest block rounded up): blocks of at most 8 take its four-pairs-per-
+    # MFMA form, larger ones its two-pairs form.
+    n_params = 4 if model == 2 else 8
+    n_free = n_params - len(fixed)
+    sc = scenes.generate("c0", model_id=model, num_points=400, seed=13 + 7 * model + n_free,
+                         dist_sigma=0.01 if model >= 3 else 0.0)
     inp = sc.inp
-    inp.fixed_camera_params = [[k >= n_free for k in range(4)] for _ in range(len(inp.cameras))]
-    out, _ = assert_parity(inp, ba.BaParams(max_iterations=12))
+    inp.fixed_camera_params = [[k in fixed for k in range(n_params)] for _ in range(len(inp.cameras))]
+    out, _ = assert_parity(inp, ba.BaParams(max_iterations=12 if model == 2 else 8))
     # pose 0 is fixed, so its camera is a block of its own
     assert out.stats["dim"] == 15 * (6 + n_free) + n_free
 
