@@ -721,21 +721,26 @@ __device__ inline void compute_Y(const View<T>& v, uint32_t a) {
 // LPS lanes of 16 bytes; one global_load_lds instruction (64 lanes, 1 KB of
 // LDS) carries NPI sides, and a batch is INS instructions per side.  Blocks
 // of at most 8 unknowns take four pairs per MFMA (two per 8x8 diagonal
-// quadrant of the 16x16 tile), larger ones two (one per K-slot pair).
+// quadrant of the 16x16 tile), larger ones two (one per K-slot pair); the
+// last step of a batch may be partly empty.  INS, measured on the
+// 1,000-camera scene (FP64 / FP32 assembly per try): 1 -> 1.36 / 0.89 ms,
+// 2 -> 1.26 / 0.85 ms, 4 -> 1.45 / 1.07 ms, 8 (FP64) -> 1.97 ms.  Two keeps
+// 17 KB of LDS per four-wavefront workgroup, so eight wavefronts per SIMD
+// (the register limit) are resident; four halve that.
 template <typename T, int DB>
 struct RecShape {
   static constexpr uint32_t RS = rec_values(DB);
   static constexpr uint32_t SIDE = 8 + 2 * DB;
   static constexpr uint32_t LPS = SIDE * sizeof(T) / 16;
   static constexpr uint32_t NPI = 64 / LPS;
-  static constexpr uint32_t INS = 4;
+  static constexpr uint32_t INS = 2;
   static constexpr uint32_t BP = INS * NPI;            // pairs per batch
   static constexpr uint32_t IE = 1024 / sizeof(T);     // LDS elements per load instruction
   static constexpr bool QUAD = DB <= 8;
   static constexpr uint32_t PPM = QUAD ? 4 : 2;        // pairs per MFMA
   static constexpr uint32_t LDS = 2 * INS * IE + 4 * BP;  // elements per wavefront
   static_assert(SIDE * sizeof(T) % 16 == 0, "sides of whole 16-byte pieces");
-  static_assert(BP % PPM == 0 && BP <= 64, "batches of whole MFMA steps, one pair per lane");
+  static_assert(BP <= 64, "one pair per lane in the M pass");
   static_assert(LDS * sizeof(T) % 16 == 0, "16-byte aligned wavefront stages");
   // LDS offset of batch pair p's side (within its side's area)
   static __device__ inline uint32_t at(uint32_t p) { return (p / NPI) * IE + (p % NPI) * SIDE; }
