@@ -690,9 +690,21 @@ typedef __attribute__((address_space(3))) void lds_void_t;
 // and the rhs share F g_point are J2 Y and J2 (Y g_point), so the assembly
 // needs no F (or E) rows: E = J2 X2.
 template <typename T, int DB>
-__device__ inline void compute_Y(const View<T>& v, uint32_t a) {
+__device__ inline void compute_Y(const View<T>& v, uint32_t a, uint32_t q) {
+  // Four lanes per incidence (q = 0..3): lane q reads and writes 16 bytes
+  // (FP32) or 32 bytes (FP64) of the record, so a wavefront's loads and
+  // stores cover whole pieces of 16 consecutive records.  The lanes of one
+  // incidence share X2 and R2 by shuffles within the group.
   using V2 = typename Vec2<T>::type;
   V2* rc = reinterpret_cast<V2*>(v.rec + (size_t)a * rec_values(DB));
+  const V2 w = rc[q];  // q = 0: X2[0][0..1], 1: X2[0][2], X2[1][0], 2: X2[1][1..2], 3: R2
+  const int g4 = (int)(threadIdx.x & 63u) & ~3;
+  T x[8];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    x[2 * k] = __shfl(w.x, g4 + k);
+    x[2 * k + 1] = __shfl(w.y, g4 + k);
+  }
   const uint32_t pt = v.inc_pt[a];
   const double* ci = v.pt_cinv + 9 * (size_t)pt;
   T c[9];
@@ -700,20 +712,20 @@ __device__ inline void compute_Y(const View<T>& v, uint32_t a) {
   for (int k = 0; k < 9; ++k) c[k] = (T)ci[k];
   const T g0 = (T)v.pt_g[3 * (size_t)pt], g1 = (T)v.pt_g[3 * (size_t)pt + 1],
           g2 = (T)v.pt_g[3 * (size_t)pt + 2];
-  const V2 q0 = rc[0], q1 = rc[1], q2 = rc[2], q3 = rc[3];
-  const T x[6] = {q0.x, q0.y, q1.x, q1.y, q2.x, q2.y};
   T y[6];
 #pragma unroll
-  for (int q = 0; q < 2; ++q)
+  for (int r = 0; r < 2; ++r)
 #pragma unroll
-    for (int j = 0; j < 3; ++j) y[3 * q + j] = x[3 * q] * c[j] + x[3 * q + 1] * c[3 + j] + x[3 * q + 2] * c[6 + j];
-  const T z0 = (y[0] * g0 + y[1] * g1 + y[2] * g2) - q3.x;
-  const T z1 = (y[3] * g0 + y[4] * g1 + y[5] * g2) - q3.y;
-  V2* o = rc + 4 + DB;
-  o[0] = V2{y[0], y[1]};
-  o[1] = V2{y[2], y[3]};
-  o[2] = V2{y[4], y[5]};
-  o[3] = V2{z0, z1};
+    for (int j = 0; j < 3; ++j) y[3 * r + j] = x[3 * r] * c[j] + x[3 * r + 1] * c[3 + j] + x[3 * r + 2] * c[6 + j];
+  V2 o;
+  if (q == 3) {
+    o.x = (y[0] * g0 + y[1] * g1 + y[2] * g2) - x[6];
+    o.y = (y[3] * g0 + y[4] * g1 + y[5] * g2) - x[7];
+  } else {
+    o.x = q == 0 ? y[0] : q == 1 ? y[2] : y[4];
+    o.y = q == 0 ? y[1] : q == 1 ? y[3] : y[5];
+  }
+  rc[4 + DB + q] = o;
 }
 
 // Shapes of schur_chunk_rec for records of DB rows.  A pair's a side (J2

@@ -36,11 +36,13 @@ __global__ __launch_bounds__(256) void k_point_prepare(View<T> v, double lambda)
   if (p < v.n_pt) point_prepare(v, p, lambda);
 }
 
-// Y and z of every incidence record, one lane per incidence.
+// Y and z of every incidence record, four lanes per incidence (the groups
+// of four lanes of a wavefront are whole incidences or past the end).
 template <typename T, int DB>
 __global__ __launch_bounds__(256) void k_compute_Y(View<T> v) {
-  const uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
-  if (a < v.n_inc) compute_Y<T, DB>(v, a);
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t a = t >> 2;
+  if (a < v.n_inc) compute_Y<T, DB>(v, a, t & 3u);
 }
 
 // The Schur reduction from incidence records, one wavefront per chunk,
@@ -389,7 +391,7 @@ void launch_point_prepare(const Layout& L, Work<T>& W, double lambda, hipStream_
   hipLaunchKernelGGL(k_point_prepare<T>, dim3(blocks_for(L.n_pt, 256)), dim3(256), 0, s, v, lambda);
   HIP_CHECK(hipGetLastError());
   if (L.n_inc == 0) return;
-  const dim3 grid(blocks_for(L.n_inc, 256));
+  const dim3 grid(blocks_for(4 * (uint64_t)L.n_inc, 256));
   switch (L.rec_db) {
     case 8: hipLaunchKernelGGL((k_compute_Y<T, 8>), grid, dim3(256), 0, s, v); break;
     case 12: hipLaunchKernelGGL((k_compute_Y<T, 12>), grid, dim3(256), 0, s, v); break;
