@@ -464,7 +464,7 @@ def main() -> int:
     # (k_f = 2 free intrinsics per PINHOLE camera).
     bytes_lin = (112 + 8 * 2) * n_obs
     t_lin = r_mixed["fp32_linearize_ms"] * 1e-3
-    roof_lin = {"bound": "hbm", "kernel": "Jacobian stage: k_linearize + k_point_accumulate (FP32 linearizations of the "
+    roof_lin = {"bound": "hbm", "kernel": "Jacobian stage: k_linearize_staged + k_point_accumulate (FP32 linearizations of the "
                 "mixed solve)", "achieved": bytes_lin / t_lin / 1e9,
                 "peak": PEAK_HBM / 1e9, "unit": "GB/s", "frac": bytes_lin / t_lin / PEAK_HBM,
                 "traffic": measured_traffic(ASSEMBLY_TRAFFIC_PROFILE, "linearize_traffic_bytes"),

@@ -122,9 +122,12 @@ struct Acc<double> {
 // r + LPO, ... and obs_pc entries r, r + LPO, ..., so that one store
 // instruction writes whole rows of consecutive observations (a full cache
 // line per observation) instead of one 16- or 32-byte piece per lane.
+// pc_reg (LPO = 1): the point terms go there (registers of the caller)
+// instead of obs_pc.
 template <typename T, int LPO = 1, int DB = 0>
 __device__ inline void linearize_obs(const View<T>& v, uint32_t o, int cur, int loss, double scale,
-                                     uint32_t r = 0) {
+                                     uint32_t r = 0, typename Vec2<T>::type* stage = nullptr,
+                                     T* pc_reg = nullptr) {
   using V4 = typename Vec4<T>::type;
   using V2 = typename Vec2<T>::type;
   const uint32_t img = v.obs_img[o];
@@ -155,30 +158,49 @@ __device__ inline void linearize_obs(const View<T>& v, uint32_t o, int cur, int 
     if (inc0 != NONE) rec0 = reinterpret_cast<V2*>(v.rec) + (size_t)inc0 * RS2;
     if (two) rec1 = reinterpret_cast<V2*>(v.rec) + (size_t)inc1 * RS2;
   }
+  // With a stage (LPO = 1), an observation of one incidence leaves its
+  // record pieces there (V2 k of X2 R2 at k, J2 row s at 4 + s) for the
+  // caller to store; one of two incidences writes its records directly.
+  const bool staged = DB > 0 && stage != nullptr && !two;
+  auto put_xr = [&](uint32_t k, V2 val) {
+    if (staged) {
+      stage[k] = val;
+    } else {
+      if (rec0) rec0[k] = val;
+      if (rec1) rec1[k] = val;
+    }
+  };
+  auto put_row = [&](uint32_t s, V2 val) {
+    if (staged)
+      stage[4 + s] = val;
+    else
+      *((two && s >= 6) ? rec1 + 4 + (s - 6) : rec0 + 4 + s) = val;
+  };
 
   if (c2 <= 1e-9) {
     // Behind the camera: dropped for this iteration (lib.rs:339-341).
     if (r == 0) v.obs_valid[o] = 0;
 #pragma unroll
     for (uint32_t k0 = 0; k0 < 9; k0 += LPO)
-      if (k0 + r < 9) pc_out[k0 + r] = T(0);
+      if (k0 + r < 9) {
+        if (pc_reg)
+          pc_reg[k0 + r] = T(0);
+        else
+          pc_out[k0 + r] = T(0);
+      }
     const V4 z = {T(0), T(0), T(0), T(0)};
     const V2 z2 = {T(0), T(0)};
     if constexpr (DB > 0) {
 #pragma unroll
       for (uint32_t k0 = 0; k0 < 4; k0 += LPO)
-        if (k0 + r < 4) {
-          if (rec0) rec0[k0 + r] = z2;
-          if (rec1) rec1[k0 + r] = z2;
-        }
+        if (k0 + r < 4 && rec0) put_xr(k0 + r, z2);
     }
 #pragma unroll
     for (uint32_t s0 = 0; s0 < 14; s0 += LPO) {
       const uint32_t s = s0 + r;
       if (s >= ns) break;
       if constexpr (DB > 0) {
-        V2* dst = (two && s >= 6) ? rec1 + 4 + (s - 6) : rec0 + 4 + s;
-        *dst = z2;
+        put_row(s, z2);
       } else {
         const uint32_t row = (two && s >= 6) ? roff1 + (s - 6) : roff0 + s;
         v.J2[row] = z2;
@@ -230,7 +252,12 @@ __device__ inline void linearize_obs(const View<T>& v, uint32_t o, int cur, int 
 #pragma unroll
     for (uint32_t k = k0 + 1; k < k0 + LPO && k < 9; ++k)
       if (k == k0 + r) val = pc[k];
-    if (k0 + r < 9) pc_out[k0 + r] = val;
+    if (k0 + r < 9) {
+      if (pc_reg)
+        pc_reg[k0 + r] = val;
+      else
+        pc_out[k0 + r] = val;
+    }
   }
   if constexpr (DB > 0) {
     // X2 (three V2) and R2 into every record of the observation
@@ -240,8 +267,7 @@ __device__ inline void linearize_obs(const View<T>& v, uint32_t o, int cur, int 
       if (k < 4) {
         const T x0 = k == 0 ? sx[0] : k == 1 ? sx[2] : k == 2 ? sx[4] : sr0;
         const T x1 = k == 0 ? sx[1] : k == 1 ? sx[3] : k == 2 ? sx[5] : sr1;
-        if (rec0) rec0[k] = V2{x0, x1};
-        if (rec1) rec1[k] = V2{x0, x1};
+        if (rec0) put_xr(k, V2{x0, x1});
       }
     }
   }
@@ -302,8 +328,7 @@ __device__ inline void linearize_obs(const View<T>& v, uint32_t o, int cur, int 
       }
     const T j0 = sw * a0, j1 = sw * a1;
     if constexpr (DB > 0) {
-      V2* dst = (two && s >= 6) ? rec1 + 4 + (s - 6) : rec0 + 4 + s;
-      *dst = V2{j0, j1};
+      put_row(s, V2{j0, j1});
     } else {
       const uint32_t row = (two && s >= 6) ? roff1 + (s - 6) : roff0 + s;
       v.J2[row] = V2{j0, j1};

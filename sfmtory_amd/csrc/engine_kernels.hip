@@ -9,16 +9,70 @@
 
 namespace sfmba {
 
-// LPO lanes per observation (rows written as whole cache lines).  Four
-// lanes measured fastest on the 1,000-camera scene (FP32 rows 0.60 -> 0.51
-// ms, FP64 0.94 -> 0.81 ms against one lane; eight lanes: no gain), and
-// they also halve the FP64 kernel's registers (178 -> 111).
-constexpr int LINEARIZE_LANES = 4;
-template <typename T, int LPO, int DB>
-__global__ __launch_bounds__(256) void k_linearize(View<T> v, int cur, int loss, double scale) {
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t o = t / LPO;
-  if (o < v.n_obs) linearize_obs<T, LPO, DB>(v, o, cur, loss, scale, t % LPO);
+// One lane per observation, records and point terms stored through LDS:
+// each lane leaves its observation's record pieces (X2 and R2, then its J2
+// rows: P pieces of two values) in its wavefront's stage, then the
+// wavefront stores the pieces of 64 consecutive observations, lane l taking
+// piece l of each group of 64, so that one store instruction covers
+// consecutive pieces of neighbouring records instead of one piece in each
+// of 64 records; the point terms (nine values per observation, contiguous
+// over the wavefront's observations) follow through the same stage.  An
+// observation of two incidences writes its records itself.  Against four
+// lanes per observation that each compute the whole Jacobian and store a
+// quarter of its pieces: FP64 0.50 -> 0.47 ms, FP32 0.35 -> 0.27 ms per
+// linearization of the 1,000-camera scene (with the point accumulation).
+// The FP64 form keeps two wavefronts per SIMD (187 registers); asking for
+// three spilled the point terms (0.57 ms).
+template <typename T, int DB>
+__global__ __launch_bounds__(256) void k_linearize_staged(
+    View<T> v, int cur, int loss, double scale) {
+  using V2 = typename Vec2<T>::type;
+  constexpr uint32_t P = 4 + DB;
+  constexpr size_t RS2 = rec_values(DB) / 2;
+  __shared__ V2 stage[4 * 64 * P];
+  const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint32_t o = blockIdx.x * blockDim.x + threadIdx.x;
+  V2* sw = stage + w * 64 * P;
+  uint32_t inc = NONE, nrows = 0;  // the staged record and its J2 rows
+  T pc[9];
+  if (o < v.n_obs) {
+    linearize_obs<T, 1, DB>(v, o, cur, loss, scale, 0, sw + lane * P, pc);
+    const uint32_t i0 = v.obs_inc[2 * o];
+    if (i0 != NONE && v.obs_inc[2 * o + 1] == NONE) {
+      inc = i0;
+      nrows = v.grp_dim[v.inc_grp[i0]];
+    }
+  }
+  // the wavefront's stage is complete (one wavefront writes and reads it)
+  __asm__ volatile("s_waitcnt vmcnt(0) lgkmcnt(0)" ::: "memory");
+  V2* recv = reinterpret_cast<V2*>(v.rec);
+#pragma unroll
+  for (uint32_t j = 0; j < P; ++j) {
+    const uint32_t idx = j * 64 + lane;
+    const uint32_t slot = idx / P, piece = idx - slot * P;
+    const uint32_t sinc = (uint32_t)__shfl((int)inc, (int)slot);
+    const uint32_t srows = (uint32_t)__shfl((int)nrows, (int)slot);
+    if (sinc != NONE && (piece < 4 || piece - 4 < srows)) recv[(size_t)sinc * RS2 + piece] = sw[idx];
+  }
+  // point terms: values 9 o0 .. 9 o_end of obs_pc, in pieces of two
+  static_assert(64 * 9 <= 2 * 64 * P, "the point terms fit the stage");
+  __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+  T* st = reinterpret_cast<T*>(sw);
+#pragma unroll
+  for (int k = 0; k < 9; ++k) st[lane * 9 + k] = pc[k];
+  __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+  const uint32_t o0 = o - lane;
+  const uint32_t nv = o0 < v.n_obs ? 9 * min(64u, v.n_obs - o0) : 0u;
+  V2* pcv = reinterpret_cast<V2*>(v.obs_pc + 9 * (size_t)o0);
+  const V2* sv = reinterpret_cast<const V2*>(st);
+#pragma unroll
+  for (uint32_t j = 0; j < 5; ++j) {
+    const uint32_t q = j * 64 + lane;  // piece q: values 2q, 2q + 1
+    if (2 * q + 1 < nv)
+      pcv[q] = sv[q];
+    else if (2 * q < nv)  // an odd count's last value
+      v.obs_pc[9 * (size_t)o0 + 2 * q] = st[2 * q];
+  }
 }
 
 // LPP lanes per point (neighbouring lanes read neighbouring observations).
@@ -365,13 +419,12 @@ View<T> make_view(const Layout& L, Work<T>& W) {
 template <typename T>
 void launch_linearize(const Layout& L, Work<T>& W, int cur, int loss, double scale, hipStream_t s) {
   if (L.n_obs == 0) return;
-  constexpr int LPO = LINEARIZE_LANES;
-  const dim3 grid(blocks_for((uint64_t)L.n_obs * LPO, 256));
   const View<T> v = make_view(L, W);
+  const dim3 grid(blocks_for(L.n_obs, 256));
   switch (L.rec_db) {
-    case 8: hipLaunchKernelGGL((k_linearize<T, LPO, 8>), grid, dim3(256), 0, s, v, cur, loss, scale); break;
-    case 12: hipLaunchKernelGGL((k_linearize<T, LPO, 12>), grid, dim3(256), 0, s, v, cur, loss, scale); break;
-    default: hipLaunchKernelGGL((k_linearize<T, LPO, 16>), grid, dim3(256), 0, s, v, cur, loss, scale); break;
+    case 8: hipLaunchKernelGGL((k_linearize_staged<T, 8>), grid, dim3(256), 0, s, v, cur, loss, scale); break;
+    case 12: hipLaunchKernelGGL((k_linearize_staged<T, 12>), grid, dim3(256), 0, s, v, cur, loss, scale); break;
+    default: hipLaunchKernelGGL((k_linearize_staged<T, 16>), grid, dim3(256), 0, s, v, cur, loss, scale); break;
   }
   HIP_CHECK(hipGetLastError());
 }
