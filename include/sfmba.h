@@ -278,12 +278,13 @@ int sfmba_bundle_adjust_batch(const sfmba_input* inputs, size_t count,
 typedef struct sfmba_batch sfmba_batch;
 #define SFMBA_MAX_WINDOW_SHAPES 8
 typedef struct sfmba_batch_info {
-  int32_t threads_per_window;  /* the launch shape in use */
-  int32_t pairs_in_flight;
+  int32_t threads_per_window;  /* the launch shape in use: threads per window and */
+  int32_t pairs_in_flight;     /* the Schur reduction's batch depth (load instructions
+                                  per incidence-record side and batch) */
   int32_t tuned;               /* 1 when the autotuner chose it for this plan */
   int32_t num_shapes;          /* candidate shapes (below) */
   int32_t shape_threads[SFMBA_MAX_WINDOW_SHAPES];
-  int32_t shape_pairs[SFMBA_MAX_WINDOW_SHAPES];
+  int32_t shape_pairs[SFMBA_MAX_WINDOW_SHAPES];  /* batch depth of each candidate */
   double shape_seconds[SFMBA_MAX_WINDOW_SHAPES]; /* tuning run time per shape (0: not timed) */
   double create_seconds;       /* packing, upload, buffers */
   double capture_seconds;      /* graph capture and instantiation */

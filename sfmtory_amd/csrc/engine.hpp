@@ -6,15 +6,9 @@
 //   an observation touches at most two "incidences": the block of the
 //   reduced system its pose belongs to and the block its camera belongs to
 //   (one merged block when the camera is used by this image alone);
-//   the single-problem solve keeps one record per incidence (rec_values
-//   below: the scaled Jacobian rows of its block, the observation's scaled
-//   point Jacobian and residual, and per damping try Y = X2 Cinv);
-//   the window solver gives every incidence `gdim` consecutive "rows" (one
-//   per unknown of its block) in row-indexed Jacobian stores:
-//     J2[row] = sqrt(w) (J0, J1)       residual Jacobian column, scaled by the root
-//                                      of the observation's robust weight
-//     E4[row] = (E0, E1, E2, g)        E = w J^T Jx (camera-point coupling), g = w J^T r
-//     F4[row] = (F0, F1, F2, F.g)      F = E * Cinv(point), g the point gradient, per try;
+//   every incidence has one record (rec_values below: the scaled Jacobian
+//   rows of its block, the observation's scaled point Jacobian and
+//   residual, and per damping try Y = X2 Cinv), in incidence order;
 //   the reduced camera system S is dense, column-major, lower triangle,
 //   (Dp + NB) x Dp with Dp = D rounded up to NB; padding is the identity and
 //   row Dp carries the right-hand side, so the blocked Cholesky produces
@@ -110,7 +104,6 @@ inline int model_param_count_host(int model) {
 // Static structure of one problem on the device.
 struct Layout {
   uint32_t n_img = 0, n_cam = 0, n_pt = 0, n_obs = 0, n_inc = 0, n_grp = 0;
-  uint32_t n_rows = 0;      // total incidence rows
   uint32_t D = 0, Dp = 0, ld = 0;
   uint64_t n_pairs = 0;
   uint32_t n_chunks = 0, n_dests = 0, n_multi = 0, n_parts = 0;
@@ -127,9 +120,6 @@ struct Layout {
   uint32_t* inc_grp = nullptr;
   uint32_t* inc_pt = nullptr;
   uint32_t* inc_obs = nullptr;
-  uint32_t* inc_roff = nullptr;  // first row of J, E and F (group-major)
-  uint32_t* inc_froff = nullptr; // first row of F (group-major)
-  uint32_t* inc_row0 = nullptr;  // first entry of the observation's side vector
   uint32_t* pt_inc_ptr = nullptr;  // incidences of each point (CSR, n_pt + 1)
   // cameras and images
   uint32_t* cam_of_img = nullptr;

@@ -8,13 +8,12 @@
 //                      and the per-observation terms of hot loop B, 880-909
 //   point_accumulate   c_diag / c_rhs sums, 885-886; the points_to_obs test, 960
 //   point_prepare      damped point-block inverse, 958-965
-//   compute_F / schur_chunk (schur_chunk_mfma8: the same reduction on the
-//   matrix cores), for the window solver's row streams; compute_Y /
-//   schur_chunk_rec, from incidence records, for the single-problem solve
+//   compute_Y / schur_chunk_rec (the reduction from incidence records on
+//   the matrix cores)
 //                      damped diagonal and cross blocks, 923-956, and the
 //                      point elimination S -= E C^-1 E^T, rhs += E C^-1 g of
 //                      accumulate_point_schur, 112-211 / 967-1043
-//   backsub_point      delta_point = -C^-1 (g + E^T delta), 1096-1120
+//   backsub_point_rec  delta_point = -C^-1 (g + E^T delta), 1096-1120
 //   update_pose / update_cam
 //                      trial state, 1122-1164 (perturb_pose 263-269)
 //   obs_cost           one term of total_cost 727-745 / mean_reprojection_error 708-725
@@ -26,17 +25,6 @@
 #include "engine.hpp"
 
 namespace sfmba {
-
-template <typename T>
-struct Vec4;
-template <>
-struct Vec4<float> {
-  using type = float4;
-};
-template <>
-struct Vec4<double> {
-  using type = double4;
-};
 
 template <typename T>
 struct Vec2;
@@ -52,7 +40,6 @@ struct Vec2<double> {
 // Pointers and sizes of one problem on the device (layout + work arrays).
 template <typename T>
 struct View {
-  using V4 = typename Vec4<T>::type;
   using V2 = typename Vec2<T>::type;
   uint32_t n_img, n_cam, n_pt, n_obs, n_inc, n_chunks;
   uint32_t D, Dp, ld;
@@ -63,8 +50,6 @@ struct View {
   const uint32_t* pt_ptr;
   const uint32_t* inc_grp;
   const uint32_t* inc_pt;
-  const uint32_t* inc_roff;
-  const uint32_t* inc_froff;
   const uint32_t* pt_inc_ptr;
   const uint32_t* cam_of_img;
   const int32_t* cam_model;
@@ -76,9 +61,6 @@ struct View {
   const uint32_t* grp_dim;
   const uint2* pairs;
   const uint4* chunks;
-  V2* J2;
-  V4* E4;
-  V4* F4;
   T* rec;
   T* obs_pc;
   uint8_t* obs_valid;
@@ -114,21 +96,16 @@ struct Acc<double> {
 };
 
 // ---------------------------------------------------------------- linearize
-// Observation o (point-major order): for each of its incidences one J2 and
-// one E4 row per unknown (DB = 0), or its incidence records (DB > 0: X2 and
-// R2, and one J2 row per unknown; rec_values), and its contribution to its
-// point's 3x3 block and gradient (obs_pc).  With LPO > 1, LPO lanes share
-// observation o: each computes the same values and lane r writes rows r,
-// r + LPO, ... and obs_pc entries r, r + LPO, ..., so that one store
-// instruction writes whole rows of consecutive observations (a full cache
-// line per observation) instead of one 16- or 32-byte piece per lane.
-// pc_reg (LPO = 1): the point terms go there (registers of the caller)
-// instead of obs_pc.
-template <typename T, int LPO = 1, int DB = 0>
+// Observation o (point-major order): the records of its incidences
+// (rec_values: X2 and R2, and one J2 row per unknown of the block) and its
+// contribution to its point's 3x3 block and gradient (obs_pc).  With a
+// stage, an observation of one incidence leaves its record pieces there
+// (V2 k of X2 R2 at k, J2 row s at 4 + s) for the caller to store; one of
+// two incidences writes its records directly.  With pc_reg, the point
+// terms go there (the caller's registers) instead of obs_pc.
+template <typename T, int DB>
 __device__ inline void linearize_obs(const View<T>& v, uint32_t o, int cur, int loss, double scale,
-                                     uint32_t r = 0, typename Vec2<T>::type* stage = nullptr,
-                                     T* pc_reg = nullptr) {
-  using V4 = typename Vec4<T>::type;
+                                     typename Vec2<T>::type* stage = nullptr, T* pc_reg = nullptr) {
   using V2 = typename Vec2<T>::type;
   const uint32_t img = v.obs_img[o];
   const uint32_t pt = v.obs_pt[o];
@@ -145,23 +122,14 @@ __device__ inline void linearize_obs(const View<T>& v, uint32_t o, int cur, int 
   const uint32_t kf = cam_is_free ? v.cam_nfree[cam] : 0u;
   const uint32_t ns = (pose_free ? 6u : 0u) + kf;
   const uint32_t inc0 = v.obs_inc[2 * o], inc1 = v.obs_inc[2 * o + 1];
-  const uint32_t roff0 = inc0 != NONE ? v.inc_roff[inc0] : 0u;
   const bool two = inc1 != NONE;
-  const uint32_t roff1 = two ? v.inc_roff[inc1] : 0u;
   T* pc_out = v.obs_pc + 9 * (size_t)o;
-  // DB > 0: the incidences' records, in V2 units (an observation without a
-  // free unknown has none and writes no record)
-  V2* rec0 = nullptr;
-  V2* rec1 = nullptr;
-  if constexpr (DB > 0) {
-    constexpr size_t RS2 = rec_values(DB) / 2;
-    if (inc0 != NONE) rec0 = reinterpret_cast<V2*>(v.rec) + (size_t)inc0 * RS2;
-    if (two) rec1 = reinterpret_cast<V2*>(v.rec) + (size_t)inc1 * RS2;
-  }
-  // With a stage (LPO = 1), an observation of one incidence leaves its
-  // record pieces there (V2 k of X2 R2 at k, J2 row s at 4 + s) for the
-  // caller to store; one of two incidences writes its records directly.
-  const bool staged = DB > 0 && stage != nullptr && !two;
+  // the incidences' records, in V2 units (an observation without a free
+  // unknown has none and writes no record)
+  constexpr size_t RS2 = rec_values(DB) / 2;
+  V2* rec0 = inc0 != NONE ? reinterpret_cast<V2*>(v.rec) + (size_t)inc0 * RS2 : nullptr;
+  V2* rec1 = two ? reinterpret_cast<V2*>(v.rec) + (size_t)inc1 * RS2 : nullptr;
+  const bool staged = stage != nullptr && !two;
   auto put_xr = [&](uint32_t k, V2 val) {
     if (staged) {
       stage[k] = val;
@@ -176,40 +144,30 @@ __device__ inline void linearize_obs(const View<T>& v, uint32_t o, int cur, int 
     else
       *((two && s >= 6) ? rec1 + 4 + (s - 6) : rec0 + 4 + s) = val;
   };
+  auto put_pc = [&](int k, T val) {
+    if (pc_reg)
+      pc_reg[k] = val;
+    else
+      pc_out[k] = val;
+  };
 
   if (c2 <= 1e-9) {
     // Behind the camera: dropped for this iteration (lib.rs:339-341).
-    if (r == 0) v.obs_valid[o] = 0;
+    v.obs_valid[o] = 0;
 #pragma unroll
-    for (uint32_t k0 = 0; k0 < 9; k0 += LPO)
-      if (k0 + r < 9) {
-        if (pc_reg)
-          pc_reg[k0 + r] = T(0);
-        else
-          pc_out[k0 + r] = T(0);
-      }
-    const V4 z = {T(0), T(0), T(0), T(0)};
+    for (int k = 0; k < 9; ++k) put_pc(k, T(0));
     const V2 z2 = {T(0), T(0)};
-    if constexpr (DB > 0) {
+    if (rec0)
 #pragma unroll
-      for (uint32_t k0 = 0; k0 < 4; k0 += LPO)
-        if (k0 + r < 4 && rec0) put_xr(k0 + r, z2);
-    }
+      for (uint32_t k = 0; k < 4; ++k) put_xr(k, z2);
 #pragma unroll
-    for (uint32_t s0 = 0; s0 < 14; s0 += LPO) {
-      const uint32_t s = s0 + r;
+    for (uint32_t s = 0; s < 14; ++s) {
       if (s >= ns) break;
-      if constexpr (DB > 0) {
-        put_row(s, z2);
-      } else {
-        const uint32_t row = (two && s >= 6) ? roff1 + (s - 6) : roff0 + s;
-        v.J2[row] = z2;
-        v.E4[row] = z;
-      }
+      put_row(s, z2);
     }
     return;
   }
-  if (r == 0) v.obs_valid[o] = 1;
+  v.obs_valid[o] = 1;
   const double2 xy = v.obs_xy[o];
   double r0, r1;
   linearization_residual(model, P, c0, c1, c2, xy.x, xy.y, r0, r1);
@@ -236,40 +194,21 @@ __device__ inline void linearize_obs(const View<T>& v, uint32_t o, int cur, int 
   for (int q = 0; q < 6; ++q) sx[q] = sw * jx[q];
 
   // Point terms: X2^T X2 (upper 6), X2^T R2.
-  T pc[9];
-  pc[0] = sx[0] * sx[0] + sx[3] * sx[3];
-  pc[1] = sx[0] * sx[1] + sx[3] * sx[4];
-  pc[2] = sx[0] * sx[2] + sx[3] * sx[5];
-  pc[3] = sx[1] * sx[1] + sx[4] * sx[4];
-  pc[4] = sx[1] * sx[2] + sx[4] * sx[5];
-  pc[5] = sx[2] * sx[2] + sx[5] * sx[5];
-  pc[6] = sx[0] * sr0 + sx[3] * sr1;
-  pc[7] = sx[1] * sr0 + sx[4] * sr1;
-  pc[8] = sx[2] * sr0 + sx[5] * sr1;
-#pragma unroll
-  for (uint32_t k0 = 0; k0 < 9; k0 += LPO) {
-    T val = pc[k0];
-#pragma unroll
-    for (uint32_t k = k0 + 1; k < k0 + LPO && k < 9; ++k)
-      if (k == k0 + r) val = pc[k];
-    if (k0 + r < 9) {
-      if (pc_reg)
-        pc_reg[k0 + r] = val;
-      else
-        pc_out[k0 + r] = val;
-    }
-  }
-  if constexpr (DB > 0) {
-    // X2 (three V2) and R2 into every record of the observation
-#pragma unroll
-    for (uint32_t k0 = 0; k0 < 4; k0 += LPO) {
-      const uint32_t k = k0 + r;
-      if (k < 4) {
-        const T x0 = k == 0 ? sx[0] : k == 1 ? sx[2] : k == 2 ? sx[4] : sr0;
-        const T x1 = k == 0 ? sx[1] : k == 1 ? sx[3] : k == 2 ? sx[5] : sr1;
-        if (rec0) put_xr(k, V2{x0, x1});
-      }
-    }
+  put_pc(0, sx[0] * sx[0] + sx[3] * sx[3]);
+  put_pc(1, sx[0] * sx[1] + sx[3] * sx[4]);
+  put_pc(2, sx[0] * sx[2] + sx[3] * sx[5]);
+  put_pc(3, sx[1] * sx[1] + sx[4] * sx[4]);
+  put_pc(4, sx[1] * sx[2] + sx[4] * sx[5]);
+  put_pc(5, sx[2] * sx[2] + sx[5] * sx[5]);
+  put_pc(6, sx[0] * sr0 + sx[3] * sr1);
+  put_pc(7, sx[1] * sr0 + sx[4] * sr1);
+  put_pc(8, sx[2] * sr0 + sx[5] * sr1);
+  // X2 (three V2) and R2 into every record of the observation
+  if (rec0) {
+    put_xr(0, V2{sx[0], sx[1]});
+    put_xr(1, V2{sx[2], sx[3]});
+    put_xr(2, V2{sx[4], sx[5]});
+    put_xr(3, V2{sr0, sr1});
   }
 
   // Side vector: free pose columns, then the camera's free parameters.
@@ -316,24 +255,9 @@ __device__ inline void linearize_obs(const View<T>& v, uint32_t o, int cur, int 
     }
   }
 #pragma unroll
-  for (uint32_t q0 = 0; q0 < 14; q0 += LPO) {
-    const uint32_t s = q0 + r;
+  for (uint32_t s = 0; s < 14; ++s) {
     if (s >= ns) break;
-    T a0 = s0[q0], a1 = s1[q0];
-#pragma unroll
-    for (uint32_t q = q0 + 1; q < q0 + LPO && q < 14; ++q)
-      if (q == s) {
-        a0 = s0[q];
-        a1 = s1[q];
-      }
-    const T j0 = sw * a0, j1 = sw * a1;
-    if constexpr (DB > 0) {
-      put_row(s, V2{j0, j1});
-    } else {
-      const uint32_t row = (two && s >= 6) ? roff1 + (s - 6) : roff0 + s;
-      v.J2[row] = V2{j0, j1};
-      v.E4[row] = V4{j0 * sx[0] + j1 * sx[3], j0 * sx[1] + j1 * sx[4], j0 * sx[2] + j1 * sx[5], j0 * sr0 + j1 * sr1};
-    }
+    put_row(s, V2{sw * s0[s], sw * s1[s]});
   }
 }
 
@@ -407,134 +331,8 @@ __device__ inline void point_prepare(const View<T>& v, uint32_t p, double lambda
   v.pt_ok[p] = ok ? 1 : 0;
 }
 
-// F = E Cinv and F . g (the incidence's rhs share) for incidence a; F is
-// stored group-major.
-template <typename T>
-__device__ inline void compute_F(const View<T>& v, uint32_t a) {
-  using V4 = typename Vec4<T>::type;
-  const uint32_t pt = v.inc_pt[a];
-  const double* ci = v.pt_cinv + 9 * (size_t)pt;
-  T c[9];
-#pragma unroll
-  for (int k = 0; k < 9; ++k) c[k] = (T)ci[k];
-  const T g0 = (T)v.pt_g[3 * (size_t)pt], g1 = (T)v.pt_g[3 * (size_t)pt + 1],
-          g2 = (T)v.pt_g[3 * (size_t)pt + 2];
-  const uint32_t r0 = v.inc_roff[a], f0ff = v.inc_froff[a], n = v.grp_dim[v.inc_grp[a]];
-  for (uint32_t r = 0; r < n; ++r) {
-    const V4 e = v.E4[r0 + r];
-    const T f0 = e.x * c[0] + e.y * c[3] + e.z * c[6];
-    const T f1 = e.x * c[1] + e.y * c[4] + e.z * c[7];
-    const T f2 = e.x * c[2] + e.y * c[5] + e.z * c[8];
-    // w: this row's share of rhs, F . g_point = E Cinv g_point
-    v.F4[f0ff + r] = V4{f0, f1, f2, f0 * g0 + f1 * g1 + f2 * g2};
-  }
-}
-
 constexpr int PART_STRIDE = 2 * 196 + 16;
 
-// One wavefront reduces one chunk of a destination block (ga >= gb) of the
-// reduced system; lane l owns entries l, l + 64, ... (NQ of them) of the
-// da x db block.  Sums, over the chunk's point pairs (a, b), B = w J_a^T J_b
-// for pairs of the same observation and Schur = F_a E_b^T = E_a Cinv E_b^T;
-// on diagonal blocks also the right-hand side F_a g_point - w J_a^T r over
-// pairs a == b (F4.w - E4.w).  A pair is two words: a's first row with bit
-// 31 "same observation", and b's first row with bit 31 "a == b" (J, E and F
-// rows share one group-major row numbering).  U pairs are processed per
-// step with all their loads issued together.  A chunk (ch.w == NONE) that
-// covers its whole destination is written straight into S (with the
-// Marquardt damping of the diagonal); others leave partials.
-// UMAX: pairs in flight per step (the grid kernel's default measured best
-// on gfx950 at full occupancy).
-template <typename T, int NQ, int UMAX = (sizeof(T) == 4 ? 4 : 2)>
-__device__ inline void schur_chunk(const View<T>& v, const uint4 ch, double lambda, double* parts) {
-  using V4 = typename Vec4<T>::type;
-  constexpr uint32_t HI = 0x80000000u, OFF = 0x7FFFFFFFu;
-  constexpr int U = UMAX / NQ > 1 ? UMAX / NQ : 1;
-  const uint32_t lane = threadIdx.x & 63;
-  const uint32_t ga = ch.x >> 16, gb = ch.x & 0xFFFFu;
-  const uint32_t da = v.grp_dim[ga], db = v.grp_dim[gb];
-  const uint32_t ne = da * db;
-  const bool diag = ga == gb;
-  Acc<T> accS[NQ], accB[NQ], accR;
-  uint32_t er[NQ], ec[NQ];
-  bool on[NQ];
-#pragma unroll
-  for (int q = 0; q < NQ; ++q) {
-    const uint32_t e = lane + 64 * q;
-    on[q] = e < ne;
-    er[q] = on[q] ? e % da : 0;
-    ec[q] = on[q] ? e / da : 0;
-  }
-  const bool rlane = diag && lane < da;
-  for (uint32_t p = ch.y; p < ch.z; p += U) {
-    uint2 pr[U];
-    bool ok[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      ok[u] = p + u < ch.z;
-      pr[u] = ok[u] ? v.pairs[p + u] : make_uint2(0u, 0u);
-    }
-    V4 f[U][NQ], e[U][NQ];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const uint32_t ra = pr[u].x & OFF, rb = pr[u].y & OFF;
-#pragma unroll
-      for (int q = 0; q < NQ; ++q) {
-        if (ok[u] && on[q]) {
-          f[u][q] = v.F4[ra + er[q]];
-          e[u][q] = v.E4[rb + ec[q]];
-        }
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      if (!ok[u]) continue;
-      const uint32_t ra = pr[u].x & OFF, rb = pr[u].y & OFF;
-#pragma unroll
-      for (int q = 0; q < NQ; ++q)
-        if (on[q]) accS[q].add(f[u][q].x * e[u][q].x + f[u][q].y * e[u][q].y + f[u][q].z * e[u][q].z);
-      if (pr[u].x & HI) {
-        // same observation: w J_a^T J_b from the J rows of a and b
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-          if (on[q]) {
-            const auto ja = v.J2[ra + er[q]];
-            const auto jb = v.J2[rb + ec[q]];
-            accB[q].add(ja.x * jb.x + ja.y * jb.y);
-          }
-        }
-      }
-      if (rlane && (pr[u].y & HI)) accR.add(v.F4[ra + lane].w - v.E4[rb + lane].w);
-    }
-  }
-  const uint32_t oa = v.grp_off[ga], ob = v.grp_off[gb];
-  if (ch.w == NONE) {
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-      if (on[q]) {
-        const uint32_t r = er[q], c = ec[q];
-        if (diag && r < c) continue;
-        const double bv = accB[q].value();
-        double val = bv - accS[q].value();
-        if (diag && r == c) val = (bv + bv * lambda) - accS[q].value();
-        v.S[(size_t)(ob + c) * v.ld + oa + r] = val;
-      }
-    }
-    if (rlane) v.S[(size_t)(oa + lane) * v.ld + v.Dp] = accR.value();
-  } else {
-    double* pp = parts + (size_t)ch.w * PART_STRIDE;
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-      if (on[q]) {
-        pp[lane + 64 * q] = accB[q].value();
-        pp[196 + lane + 64 * q] = accS[q].value();
-      }
-    }
-    if (rlane) pp[392 + lane] = accR.value();
-  }
-}
-
-// ----------------------------------------------------- MFMA Schur chunk
 // 16x16x4 matrix-core product for T: lane l supplies A[l % 16][l / 16] and
 // B[l / 16][l % 16]; register v of lane l holds D[row(l, v)][l % 16], with
 // row 4 (l / 16) + v in FP32 and l / 16 + 4 v in FP64 (checked on gfx950
@@ -561,149 +359,6 @@ struct Mfma<double> {
     return (lane >> 4) + 4u * (uint32_t)v;
   }
 };
-
-// The destination-block reduction of schur_chunk on the matrix cores, for
-// blocks of at most 8x8 (da, db <= 8; the window solver's row-stream
-// layout).  Per pair, lane l supplies one component, F_a[row][k] and
-// E_b[row][k] for the three point coordinates (K slot 3 zero), so one
-// 16x16x4 MFMA adds F_a E_b^T; a pair of the same observation adds
-// w J_a^T J_b with a second MFMA over the two residual components.  Two
-// pairs per MFMA: rows 0-7 of both operands carry the even pair of a step, rows 8-15
-// the odd pair, so the 16x16 tile's two diagonal quadrants sum the even and
-// the odd pairs of the chunk and its off-diagonal quadrants (cross terms)
-// are discarded.  Lane l loads row l % 8 of pair 2u + (l / 8) % 2,
-// component l / 16: per load instruction twice the pairs of the one-pair
-// form, at the same registers.  The quadrants are added at the end (even +
-// odd); the FP32 instantiation flushes after every U steps, so no more than
-// U terms are summed in FP32 per quadrant.
-// LONG: chunks of any length (the window solver's one chunk per
-// destination), taken 128 pairs at a time; otherwise at most PAIR_CHUNK.
-template <typename T, int U, bool LONG = false>
-__device__ inline void schur_chunk_mfma8(const View<T>& v, const uint4 ch, double lambda, double* parts) {
-  using M = Mfma<T>;
-  using acc_t = typename M::acc_t;
-  constexpr uint32_t HI = 0x80000000u, OFF = 0x7FFFFFFFu;
-  static_assert(PAIR_CHUNK <= 128, "a chunk's pair words are held in two registers per lane");
-  const uint32_t lane = threadIdx.x & 63;
-  const uint32_t i = lane & 7, h = (lane >> 3) & 1, k = lane >> 4;
-  const uint32_t ga = ch.x >> 16, gb = ch.x & 0xFFFFu;
-  const uint32_t da = v.grp_dim[ga], db = v.grp_dim[gb];
-  const bool diag = ga == gb;
-  const bool fa_on = i < da && k < 3, eb_on = i < db && k < 3;
-  const bool ja_on = i < da && k < 2, jb_on = i < db && k < 2;
-  const T* F = reinterpret_cast<const T*>(v.F4);
-  const T* E = reinterpret_cast<const T*>(v.E4);
-  const T* J = reinterpret_cast<const T*>(v.J2);
-  const uint32_t n_all = ch.z - ch.y;
-  acc_t s = {T(0), T(0), T(0), T(0)}, b = {T(0), T(0), T(0), T(0)};
-  double ss[4] = {0.0, 0.0, 0.0, 0.0}, bs[4] = {0.0, 0.0, 0.0, 0.0};
-  Acc<T> accR;
-  const bool rlane = diag && lane < da;
-  for (uint32_t base = 0; base < (LONG ? n_all : 1u); base += 128) {
-    const uint32_t n = LONG ? min(128u, n_all - base) : n_all;
-    const uint32_t p0 = ch.y + base;
-    const uint2 w0 = lane < n ? v.pairs[p0 + lane] : make_uint2(0u, 0u);
-    const uint2 w1 = lane + 64 < n ? v.pairs[p0 + 64 + lane] : make_uint2(0u, 0u);
-    // Pair q's words (zero for q >= n).
-    auto word = [&](uint32_t q, uint32_t& x, uint32_t& y) {
-      x = 0u;
-      y = 0u;
-      if (q < n) {
-        const uint32_t sl = q & 63;
-        x = __builtin_amdgcn_readlane(q < 64 ? w0.x : w1.x, sl);
-        y = __builtin_amdgcn_readlane(q < 64 ? w0.y : w1.y, sl);
-      }
-    };
-    struct Stage {
-      T fa[U], eb[U], ja[U], jb[U];
-    };
-    // Operands of U steps (2U pairs) starting at pair q.
-    auto load = [&](uint32_t q, Stage& st) {
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        uint32_t x0, y0, x1, y1;
-        word(q + 2 * u, x0, y0);
-        word(q + 2 * u + 1, x1, y1);
-        const bool ok = q + 2 * u + h < n;
-        const uint32_t px = h ? x1 : x0, py = h ? y1 : y0;
-        const uint32_t ra = px & OFF, rb = py & OFF;
-        st.fa[u] = (ok && fa_on) ? F[4 * (size_t)(ra + i) + k] : T(0);
-        st.eb[u] = (ok && eb_on) ? E[4 * (size_t)(rb + i) + k] : T(0);
-        const bool same = ok && (px & HI);
-        st.ja[u] = (same && ja_on) ? J[2 * (size_t)(ra + i) + k] : T(0);
-        st.jb[u] = (same && jb_on) ? J[2 * (size_t)(rb + i) + k] : T(0);
-      }
-    };
-    Stage cur, nxt;
-    load(0, cur);
-    for (uint32_t q = 0; q < n; q += 2 * U) {
-      if (q + 2 * U < n) load(q + 2 * U, nxt);
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        if (q + 2 * u >= n) break;
-        uint32_t x0, y0, x1, y1;
-        word(q + 2 * u, x0, y0);
-        word(q + 2 * u + 1, x1, y1);
-        s = M::step(cur.fa[u], cur.eb[u], s);
-        if ((x0 | x1) & HI) b = M::step(cur.ja[u], cur.jb[u], b);
-        // right-hand side in pair order
-        if (rlane && (y0 & HI)) accR.add(v.F4[(x0 & OFF) + lane].w - v.E4[(y0 & OFF) + lane].w);
-        if (rlane && (y1 & HI)) accR.add(v.F4[(x1 & OFF) + lane].w - v.E4[(y1 & OFF) + lane].w);
-      }
-      if (sizeof(T) == 4) {
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          ss[c] += (double)s[c];
-          bs[c] += (double)b[c];
-          s[c] = T(0);
-          b[c] = T(0);
-        }
-      }
-      cur = nxt;
-    }
-  }  // 128-pair blocks
-  if (sizeof(T) == 8) {
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      ss[c] = (double)s[c];
-      bs[c] = (double)b[c];
-    }
-  }
-  // Quadrant (8 + r, 8 + c) sits 40 lanes up in the same register (FP32
-  // layout, row 4 (l / 16) + v) or 8 lanes up two registers on (FP64
-  // layout, row l / 16 + 4 v).
-  // Only registers holding rows 0-7 are combined (FP64: registers 0 and 1).
-#pragma unroll
-  for (int q = 0; q < (sizeof(T) == 4 ? 4 : 2); ++q) {
-    const int src = sizeof(T) == 4 ? q : q + 2;
-    const int dl = sizeof(T) == 4 ? 40 : 8;
-    ss[q] += __shfl(ss[src], (int)((lane + dl) & 63));
-    bs[q] += __shfl(bs[src], (int)((lane + dl) & 63));
-  }
-  const uint32_t oa = v.grp_off[ga], ob = v.grp_off[gb];
-  const uint32_t c = lane & 15;
-  if (ch.w == NONE) {
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const uint32_t r = M::row(lane, q);
-      if (r >= da || c >= db || (diag && r < c)) continue;
-      double val = bs[q] - ss[q];
-      if (diag && r == c) val = (bs[q] + bs[q] * lambda) - ss[q];
-      v.S[(size_t)(ob + c) * v.ld + oa + r] = val;
-    }
-    if (rlane) v.S[(size_t)(oa + lane) * v.ld + v.Dp] = accR.value();
-  } else {
-    double* pp = parts + (size_t)ch.w * PART_STRIDE;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const uint32_t r = M::row(lane, q);
-      if (r >= da || c >= db) continue;
-      pp[r + c * da] = bs[q];
-      pp[196 + r + c * da] = ss[q];
-    }
-    if (rlane) pp[392 + lane] = accR.value();
-  }
-}
 
 // ------------------------------------------------ incidence records
 typedef __attribute__((address_space(3))) void lds_void_t;
@@ -764,13 +419,13 @@ __device__ inline void compute_Y(const View<T>& v, uint32_t a, uint32_t q) {
 // 2 -> 1.26 / 0.85 ms, 4 -> 1.45 / 1.07 ms, 8 (FP64) -> 1.97 ms.  Two keeps
 // 17 KB of LDS per four-wavefront workgroup, so eight wavefronts per SIMD
 // (the register limit) are resident; four halve that.
-template <typename T, int DB>
+template <typename T, int DB, int INS_ = 2>
 struct RecShape {
   static constexpr uint32_t RS = rec_values(DB);
   static constexpr uint32_t SIDE = 8 + 2 * DB;
   static constexpr uint32_t LPS = SIDE * sizeof(T) / 16;
   static constexpr uint32_t NPI = 64 / LPS;
-  static constexpr uint32_t INS = 2;
+  static constexpr uint32_t INS = INS_;
   static constexpr uint32_t BP = INS * NPI;            // pairs per batch
   static constexpr uint32_t IE = 1024 / sizeof(T);     // LDS elements per load instruction
   static constexpr bool QUAD = DB <= 8;
@@ -799,12 +454,12 @@ struct RecShape {
 // FP64 after every step (two products per entry).  A chunk covering its
 // whole destination writes S (with the Marquardt damping of the diagonal,
 // lib.rs:923-956); others leave partials (sums, damping diagonal, rhs) for
-// k_schur_reduce_rec.  lds: RecShape<T, DB>::LDS elements of this wavefront.
-template <typename T, int DB>
+// k_schur_reduce.  lds: RecShape<T, DB, INS>::LDS elements of this wavefront.
+template <typename T, int DB, bool LONG = false, int INS = 2>
 __device__ inline void schur_chunk_rec(const View<T>& v, const uint4 ch, double lambda, double* parts, T* lds) {
   using M = Mfma<T>;
   using acc_t = typename M::acc_t;
-  using Sh = RecShape<T, DB>;
+  using Sh = RecShape<T, DB, INS>;
   constexpr uint32_t HI = 0x80000000u, OFF = 0x7FFFFFFFu;
   static_assert(PAIR_CHUNK <= 128, "a chunk's pair words are held in two registers per lane");
   const uint32_t lane = threadIdx.x & 63;
@@ -824,19 +479,24 @@ __device__ inline void schur_chunk_rec(const View<T>& v, const uint4 ch, double 
   T* Ms = lds + 2 * Sh::INS * Sh::IE;
   const char* recb = reinterpret_cast<const char*>(v.rec);
   constexpr size_t RSB = Sh::RS * sizeof(T);
-  const uint32_t n = ch.z - ch.y;
-  const uint2 w0 = lane < n ? v.pairs[ch.y + lane] : make_uint2(0u, 0u);
-  const uint2 w1 = lane + 64 < n ? v.pairs[ch.y + 64 + lane] : make_uint2(0u, 0u);
-  // per-pair flags of the chunk (bit q % 64 of half q / 64): same
-  // observation, and a == b
-  const uint64_t same_m[2] = {__ballot((w0.x & HI) != 0), __ballot((w1.x & HI) != 0)};
-  const uint64_t eq_m[2] = {__ballot((w0.y & HI) != 0), __ballot((w1.y & HI) != 0)};
   // this lane's side slot and 16-byte piece in each load instruction
   const uint32_t slot = lane / Sh::LPS, piece = (lane % Sh::LPS) * 16;
   acc_t s = {T(0), T(0), T(0), T(0)};
   double ss[4] = {0.0, 0.0, 0.0, 0.0};
   Acc<T> accR;
   double bdg = 0.0;
+  const uint32_t n_all = ch.z - ch.y;
+  // LONG: a chunk of any length (the window solver's one chunk per
+  // destination), 128 pairs at a time, in pair order
+  for (uint32_t base = 0; base < (LONG ? n_all : 1u); base += 128) {
+  const uint32_t n = LONG ? min(128u, n_all - base) : n_all;
+  const uint32_t p0 = ch.y + base;
+  const uint2 w0 = lane < n ? v.pairs[p0 + lane] : make_uint2(0u, 0u);
+  const uint2 w1 = lane + 64 < n ? v.pairs[p0 + 64 + lane] : make_uint2(0u, 0u);
+  // per-pair flags of the block (bit q % 64 of half q / 64): same
+  // observation, and a == b
+  const uint64_t same_m[2] = {__ballot((w0.x & HI) != 0), __ballot((w1.x & HI) != 0)};
+  const uint64_t eq_m[2] = {__ballot((w0.y & HI) != 0), __ballot((w1.y & HI) != 0)};
   for (uint32_t q0 = 0; q0 < n; q0 += Sh::BP) {
     const uint32_t nb = min(Sh::BP, n - q0);
     // sides of pairs q0 .. q0 + nb - 1 into LDS; spare lanes and pairs past
@@ -904,6 +564,7 @@ __device__ inline void schur_chunk_rec(const View<T>& v, const uint4 ch, double 
     // the next batch's loads overwrite this one: every lane's reads first
     __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
   }
+  }  // 128-pair blocks
   if (sizeof(T) == 8) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) ss[q] = (double)s[q];
@@ -948,8 +609,10 @@ __device__ inline void schur_chunk_rec(const View<T>& v, const uint4 ch, double 
 }
 
 // ------------------------------------------------- back-substitution + update
-template <typename T>
-__device__ inline void backsub_point(const View<T>& v, uint32_t p, int cur) {
+// The point back-substitution from incidence records: E_a^T delta = X2^T (J2_a^T delta)
+// per incidence, in incidence order.
+template <typename T, int DB>
+__device__ inline void backsub_point_rec(const View<T>& v, uint32_t p, int cur) {
   const double* X = v.pts[cur] + 3 * (size_t)p;
   double* Y = v.pts[1 - cur] + 3 * (size_t)p;
   if (!v.pt_ok[p]) {
@@ -961,18 +624,17 @@ __device__ inline void backsub_point(const View<T>& v, uint32_t p, int cur) {
   double acc0 = v.pt_g[3 * (size_t)p], acc1 = v.pt_g[3 * (size_t)p + 1], acc2 = v.pt_g[3 * (size_t)p + 2];
   for (uint32_t a = v.pt_inc_ptr[p]; a < v.pt_inc_ptr[p + 1]; ++a) {
     const uint32_t g = v.inc_grp[a];
-    const uint32_t r0 = v.inc_roff[a], n = v.grp_dim[g], off = v.grp_off[g];
-    double c0 = 0.0, c1 = 0.0, c2 = 0.0;
+    const uint32_t n = v.grp_dim[g], off = v.grp_off[g];
+    const T* rc = v.rec + (size_t)a * rec_values(DB);
+    double t0 = 0.0, t1 = 0.0;
     for (uint32_t r = 0; r < n; ++r) {
-      const auto e = v.E4[r0 + r];
       const double d = v.delta[off + r];
-      c0 += (double)e.x * d;
-      c1 += (double)e.y * d;
-      c2 += (double)e.z * d;
+      t0 += (double)rc[8 + 2 * r] * d;
+      t1 += (double)rc[8 + 2 * r + 1] * d;
     }
-    acc0 += c0;
-    acc1 += c1;
-    acc2 += c2;
+    acc0 += (double)rc[0] * t0 + (double)rc[3] * t1;
+    acc1 += (double)rc[1] * t0 + (double)rc[4] * t1;
+    acc2 += (double)rc[2] * t0 + (double)rc[5] * t1;
   }
   const double* ci = v.pt_cinv + 9 * (size_t)p;
   Y[0] = X[0] + -(ci[0] * acc0 + ci[1] * acc1 + ci[2] * acc2);

@@ -113,10 +113,10 @@ static void validate(const sfmba_input* in) {
 // Host-built structure of one problem (see engine.hpp for the layout).
 struct Pack {
   uint32_t n_img = 0, n_cam = 0, n_pt = 0, n_obs = 0;
-  uint32_t D = 0, Dp = 0, ld = 0, n_grp = 0, n_rows = 0;
+  uint32_t D = 0, Dp = 0, ld = 0, n_grp = 0;
   std::vector<uint32_t> obs_img, obs_pt, obs_inc, pt_ptr;
   std::vector<double2> obs_xy;
-  std::vector<uint32_t> inc_grp, inc_pt, inc_obs, inc_roff, inc_froff, inc_row0, pt_inc_ptr;
+  std::vector<uint32_t> inc_grp, inc_pt, inc_obs, pt_inc_ptr;
   std::vector<uint32_t> cam_of_img, img_slot, cam_slot, cam_nfree, grp_off, grp_dim;
   std::vector<int32_t> cam_model;
   std::vector<uint8_t> cam_free;
@@ -128,11 +128,9 @@ struct Pack {
 };
 
 // pair_chunk: most pairs one Schur chunk reduces (0: no limit, one chunk
-// per destination).  rec_pairs: pair words name incidences, whose records
-// the single-problem solve reads, instead of first rows (the window
-// solver's row streams).
+// per destination).
 static void build_pack(const sfmba_input* in, Pack& P, bool need_pairs,
-                       uint32_t pair_chunk = PAIR_CHUNK, bool rec_pairs = false) {
+                       uint32_t pair_chunk = PAIR_CHUNK) {
   P.n_img = (uint32_t)in->num_images;
   P.n_cam = (uint32_t)in->num_cameras;
   P.n_pt = (uint32_t)in->num_points;
@@ -240,53 +238,28 @@ static void build_pack(const sfmba_input* in, Pack& P, bool need_pairs,
 
   // Incidences, in observation (hence point) order.
   P.pt_inc_ptr.assign((size_t)P.n_pt + 1, 0);
-  uint32_t rows = 0;
-  auto add_inc = [&](uint32_t s, uint32_t g, uint32_t row0, int slot) {
+  auto add_inc = [&](uint32_t s, uint32_t g, int slot) {
     const uint32_t a = (uint32_t)P.inc_grp.size();
     P.inc_grp.push_back(g);
     P.inc_pt.push_back(P.obs_pt[s]);
     P.inc_obs.push_back(s);
-    P.inc_roff.push_back(rows);
-    P.inc_row0.push_back(row0);
-    rows += P.grp_dim[g];
     P.obs_inc[2 * (size_t)s + slot] = a;
   };
   for (uint32_t s = 0; s < P.n_obs; ++s) {
     const uint32_t img = P.obs_img[s], c = P.cam_of_img[img];
     const bool pf = pose_free[img], cf = cam_is_free[c];
     if (pf && cf && grp_of_cam[c] == grp_of_pose[img]) {
-      add_inc(s, grp_of_pose[img], 0, 0);
+      add_inc(s, grp_of_pose[img], 0);
     } else {
       int slot = 0;
-      if (pf) add_inc(s, grp_of_pose[img], 0, slot++);
-      if (cf) add_inc(s, grp_of_cam[c], pf ? 6u : 0u, slot++);
+      if (pf) add_inc(s, grp_of_pose[img], slot++);
+      if (cf) add_inc(s, grp_of_cam[c], slot++);
     }
     P.pt_inc_ptr[P.obs_pt[s] + 1] = (uint32_t)P.inc_grp.size();
   }
   for (uint32_t p = 0; p < P.n_pt; ++p) P.pt_inc_ptr[p + 1] = std::max(P.pt_inc_ptr[p + 1], P.pt_inc_ptr[p]);
-  P.n_rows = rows;
-  if (rows >= 0x80000000u) throw Error(SFMBA_ERR_UNSUPPORTED, "too many Jacobian rows for 31-bit offsets");
-  // J, E and F rows are stored group-major: each group's incidences
-  // contiguous, in point order, so a destination block (ga, gb) of the Schur
-  // assembly reads two compact row ranges (group ga's F and J rows, group
-  // gb's E and J rows) that stay in L2 while its pairs are reduced.
-  {
-    std::vector<uint32_t> gstart(P.n_grp, 0);
-    for (uint32_t g : P.inc_grp) gstart[g] += P.grp_dim[g];
-    uint32_t acc = 0;
-    for (uint32_t g = 0; g < P.n_grp; ++g) {
-      const uint32_t n = gstart[g];
-      gstart[g] = acc;
-      acc += n;
-    }
-    P.inc_froff.resize(P.inc_grp.size());
-    for (size_t a = 0; a < P.inc_grp.size(); ++a) {
-      const uint32_t g = P.inc_grp[a];
-      P.inc_froff[a] = gstart[g];
-      gstart[g] += P.grp_dim[g];
-    }
-    P.inc_roff = P.inc_froff;
-  }
+  if (P.inc_grp.size() >= 0x80000000u)
+    throw Error(SFMBA_ERR_UNSUPPORTED, "too many incidences for 31-bit pair words");
   if (!need_pairs || D == 0) return;
   if (P.n_grp > 0xFFFFu) throw Error(SFMBA_ERR_UNSUPPORTED, "more than 65535 parameter blocks");
 
@@ -318,13 +291,11 @@ static void build_pack(const sfmba_input* in, Pack& P, bool need_pairs,
           const uint64_t ga = P.inc_grp[a], gb = P.inc_grp[b];
           if (ga < gb) continue;
           const uint32_t q = pos[ga * (ga + 1) / 2 + gb]++;
-          // First word: a (or a's first row), bit 31 "same observation".
-          // Second word: b (or b's first row), bit 31 "a == b".
+          // First word: incidence a, bit 31 "same observation".  Second
+          // word: incidence b, bit 31 "a == b".
           const bool same = P.inc_obs[a] == P.inc_obs[b];
-          const uint32_t wa = rec_pairs ? a : P.inc_froff[a];
-          const uint32_t wb = rec_pairs ? b : P.inc_froff[b];
-          P.pairs[2 * (size_t)q] = wa | (same ? 0x80000000u : 0u);
-          P.pairs[2 * (size_t)q + 1] = wb | (a == b ? 0x80000000u : 0u);
+          P.pairs[2 * (size_t)q] = a | (same ? 0x80000000u : 0u);
+          P.pairs[2 * (size_t)q + 1] = b | (a == b ? 0x80000000u : 0u);
         }
     }
   }
@@ -370,7 +341,6 @@ static void upload_layout(const Pack& P, Problem& pr, hipStream_t s) {
   L.n_obs = P.n_obs;
   L.n_inc = (uint32_t)P.inc_grp.size();
   L.n_grp = P.n_grp;
-  L.n_rows = P.n_rows;
   L.D = P.D;
   L.Dp = P.Dp;
   L.ld = P.ld;
@@ -388,9 +358,6 @@ static void upload_layout(const Pack& P, Problem& pr, hipStream_t s) {
   L.inc_grp = A.upload(P.inc_grp, s);
   L.inc_pt = A.upload(P.inc_pt, s);
   L.inc_obs = A.upload(P.inc_obs, s);
-  L.inc_roff = A.upload(P.inc_roff, s);
-  L.inc_froff = A.upload(P.inc_froff, s);
-  L.inc_row0 = A.upload(P.inc_row0, s);
   L.pt_inc_ptr = A.upload(P.pt_inc_ptr, s);
   L.cam_of_img = A.upload(P.cam_of_img, s);
   L.cam_model = A.upload(P.cam_model, s);
@@ -566,7 +533,7 @@ static sfmba_problem* create_problem(const sfmba_input* in, int device) {
   HIP_CHECK(hipHostMalloc((void**)&H->pinned, 64, hipHostMallocDefault));
   const auto t0 = clock::now();
   Pack P;
-  build_pack(in, P, true, PAIR_CHUNK, true);
+  build_pack(in, P, true);
   const auto t1 = clock::now();
   upload_layout(P, H->pr, H->s);
   H->init_poses = H->pr.arena.upload(P.poses, H->s);
@@ -1010,10 +977,10 @@ namespace sfmba {
 
 // Offsets of one window's arrays inside the batch allocation.
 struct WinPlan {
-  size_t obs_img, obs_pt, obs_xy, obs_inc, pt_ptr, inc_grp, inc_pt, inc_roff, inc_froff, pt_inc_ptr;
+  size_t obs_img, obs_pt, obs_xy, obs_inc, pt_ptr, inc_grp, inc_pt, pt_inc_ptr;
   size_t cam_of_img, cam_model, img_slot, cam_slot, cam_free, cam_nfree, grp_off, grp_dim, pairs, chunks;
   size_t state;  // poses[0], cams[0], pts[0], poses[1], cams[1], pts[1] (doubles)
-  size_t J2, E4, F4, obs_pc, obs_valid, pt_C, pt_g, pt_active, pt_cinv, pt_ok, S, delta;
+  size_t rec, obs_pc, obs_valid, pt_C, pt_g, pt_active, pt_cinv, pt_ok, S, delta;
 };
 
 struct Planner {
@@ -1051,7 +1018,6 @@ struct WindowBatch {
 template <typename T>
 static void batch_prepare(WindowBatch<T>& B, const sfmba_input* inputs, size_t count, const sfmba_params* prm,
                           hipStream_t s) {
-  using V4 = typename Vec4<T>::type;
   for (size_t w = 0; w < count; ++w) validate(&inputs[w]);
   B.packs.resize(count);
   B.cams_in.resize(count);
@@ -1103,8 +1069,6 @@ static void batch_prepare(WindowBatch<T>& B, const sfmba_input* inputs, size_t c
     q.pt_ptr = up.take(4 * ((size_t)P.n_pt + 1));
     q.inc_grp = up.take(4 * P.inc_grp.size());
     q.inc_pt = up.take(4 * P.inc_pt.size());
-    q.inc_roff = up.take(4 * P.inc_roff.size());
-    q.inc_froff = up.take(4 * P.inc_froff.size());
     q.pt_inc_ptr = up.take(4 * P.pt_inc_ptr.size());
     q.cam_of_img = up.take(4 * (size_t)P.n_img);
     q.cam_model = up.take(4 * (size_t)P.n_cam);
@@ -1129,9 +1093,7 @@ static void batch_prepare(WindowBatch<T>& B, const sfmba_input* inputs, size_t c
   for (size_t k = 0; k < nw; ++k) {
     const Pack& P = B.packs[B.win[k]];
     WinPlan& q = B.plan[k];
-    q.J2 = wk.take(2 * sizeof(T) * ((size_t)P.n_rows + 8));
-    q.E4 = wk.take(sizeof(V4) * (size_t)P.n_rows);
-    q.F4 = wk.take(sizeof(V4) * (size_t)P.n_rows);
+    q.rec = wk.take(sizeof(T) * rec_values(rec_db_for(B.max_dim)) * std::max<size_t>(P.inc_grp.size(), 1));
     q.obs_pc = wk.take(9 * sizeof(T) * (size_t)P.n_obs);
     q.obs_valid = wk.take(P.n_obs);
     q.pt_C = wk.take(6 * 8 * (size_t)P.n_pt);
@@ -1158,8 +1120,6 @@ static void batch_prepare(WindowBatch<T>& B, const sfmba_input* inputs, size_t c
     put(q.pt_ptr, P.pt_ptr.data(), 4 * P.pt_ptr.size());
     put(q.inc_grp, P.inc_grp.data(), 4 * P.inc_grp.size());
     put(q.inc_pt, P.inc_pt.data(), 4 * P.inc_pt.size());
-    put(q.inc_roff, P.inc_roff.data(), 4 * P.inc_roff.size());
-    put(q.inc_froff, P.inc_froff.data(), 4 * P.inc_froff.size());
     put(q.pt_inc_ptr, P.pt_inc_ptr.data(), 4 * P.pt_inc_ptr.size());
     put(q.cam_of_img, P.cam_of_img.data(), 4 * (size_t)P.n_img);
     put(q.cam_model, P.cam_model.data(), 4 * (size_t)P.n_cam);
@@ -1194,8 +1154,6 @@ static void batch_prepare(WindowBatch<T>& B, const sfmba_input* inputs, size_t c
     v.pt_ptr = reinterpret_cast<const uint32_t*>(at(q.pt_ptr));
     v.inc_grp = reinterpret_cast<const uint32_t*>(at(q.inc_grp));
     v.inc_pt = reinterpret_cast<const uint32_t*>(at(q.inc_pt));
-    v.inc_roff = reinterpret_cast<const uint32_t*>(at(q.inc_roff));
-    v.inc_froff = reinterpret_cast<const uint32_t*>(at(q.inc_froff));
     v.pt_inc_ptr = reinterpret_cast<const uint32_t*>(at(q.pt_inc_ptr));
     v.cam_of_img = reinterpret_cast<const uint32_t*>(at(q.cam_of_img));
     v.cam_model = reinterpret_cast<const int32_t*>(at(q.cam_model));
@@ -1207,9 +1165,7 @@ static void batch_prepare(WindowBatch<T>& B, const sfmba_input* inputs, size_t c
     v.grp_dim = reinterpret_cast<const uint32_t*>(at(q.grp_dim));
     v.pairs = reinterpret_cast<const uint2*>(at(q.pairs));
     v.chunks = reinterpret_cast<const uint4*>(at(q.chunks));
-    v.J2 = reinterpret_cast<typename View<T>::V2*>(at(q.J2));
-    v.E4 = reinterpret_cast<V4*>(at(q.E4));
-    v.F4 = reinterpret_cast<V4*>(at(q.F4));
+    v.rec = reinterpret_cast<T*>(at(q.rec));
     v.obs_pc = reinterpret_cast<T*>(at(q.obs_pc));
     v.obs_valid = at(q.obs_valid);
     v.pt_C = reinterpret_cast<double*>(at(q.pt_C));
@@ -1295,10 +1251,8 @@ static std::array<uint32_t, 4> tune_key(const WindowBatch<T>& B) {
     while ((1ull << l) < x) ++l;
     return l;
   };
-  const uint32_t ne = B.max_dim * B.max_dim;
-  const uint32_t nq = ne <= 64 ? 1 : ne <= 128 ? 2 : ne <= 192 ? 3 : 4;
   const uint64_t nw = std::max<size_t>(B.win.size(), 1);
-  return {(uint32_t)sizeof(T), nq, lg(nw), lg(std::max<uint64_t>(B.total_obs / nw, 1))};
+  return {(uint32_t)sizeof(T), rec_db_for(B.max_dim), lg(nw), lg(std::max<uint64_t>(B.total_obs / nw, 1))};
 }
 
 template <typename T>
@@ -1464,7 +1418,7 @@ static void batch_plan_build(sfmba_batch& Q, WindowBatch<T>& B, const sfmba_inpu
   Q.info.num_shapes = N_WINDOW_SHAPES;
   for (int k = 0; k < N_WINDOW_SHAPES; ++k) {
     Q.info.shape_threads[k] = WINDOW_SHAPES[k].threads;
-    Q.info.shape_pairs[k] = WINDOW_SHAPES[k].pairs;
+    Q.info.shape_pairs[k] = WINDOW_SHAPES[k].batch;
   }
   if (autotune && nw) {
     Q.shape = tune_batch(B, Q.d_pristine, s, autotune, Q.info.shape_seconds);
@@ -1473,7 +1427,7 @@ static void batch_plan_build(sfmba_batch& Q, WindowBatch<T>& B, const sfmba_inpu
     Q.shape = lookup_shape(B);
   }
   Q.info.threads_per_window = WINDOW_SHAPES[Q.shape].threads;
-  Q.info.pairs_in_flight = WINDOW_SHAPES[Q.shape].pairs;
+  Q.info.pairs_in_flight = WINDOW_SHAPES[Q.shape].batch;
   // capture (a batch without windows for the kernel has no graph)
   const auto t1 = clock::now();
   if (nw) {

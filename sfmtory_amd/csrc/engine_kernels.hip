@@ -36,7 +36,7 @@ __global__ __launch_bounds__(256) void k_linearize_staged(
   uint32_t inc = NONE, nrows = 0;  // the staged record and its J2 rows
   T pc[9];
   if (o < v.n_obs) {
-    linearize_obs<T, 1, DB>(v, o, cur, loss, scale, 0, sw + lane * P, pc);
+    linearize_obs<T, DB>(v, o, cur, loss, scale, sw + lane * P, pc);
     const uint32_t i0 = v.obs_inc[2 * o];
     if (i0 != NONE && v.obs_inc[2 * o + 1] == NONE) {
       inc = i0;
@@ -385,8 +385,6 @@ View<T> make_view(const Layout& L, Work<T>& W) {
   v.pt_ptr = L.pt_ptr;
   v.inc_grp = L.inc_grp;
   v.inc_pt = L.inc_pt;
-  v.inc_roff = L.inc_roff;
-  v.inc_froff = L.inc_froff;
   v.pt_inc_ptr = L.pt_inc_ptr;
   v.cam_of_img = L.cam_of_img;
   v.cam_model = L.cam_model;

@@ -32,19 +32,20 @@ struct WindowResult {
 constexpr uint32_t WINDOW_MAX_DIM = 1024;
 
 // Launch shapes of the window kernel: threads per window (one workgroup)
-// and Schur point pairs in flight per wavefront.  The autotuner
-// (engine_host.hip) times them on a batch; without a tuned choice the
-// default per precision is used (the best of a measured sweep over 256
-// windows of 10-40 cameras on gfx950: FP32 1024/8, FP64 512/4).
+// and the Schur reduction's batch depth (load instructions per record side
+// and batch, schur_chunk_rec's INS).  The autotuner (engine_host.hip) times
+// them on a batch; without a tuned choice the default per precision is
+// used (the best of a measured sweep over 256 windows of 10-40 cameras on
+// gfx950).
 struct WindowShape {
-  int threads, pairs;
+  int threads, batch;
 };
 constexpr int N_WINDOW_SHAPES = 6;
-constexpr WindowShape WINDOW_SHAPES[N_WINDOW_SHAPES] = {{1024, 8}, {1024, 4}, {512, 8},
-                                                        {512, 4},  {256, 8},  {256, 4}};
+constexpr WindowShape WINDOW_SHAPES[N_WINDOW_SHAPES] = {{1024, 2}, {1024, 1}, {512, 2},
+                                                        {512, 1},  {256, 2},  {256, 1}};
 template <typename T>
 constexpr int window_default_shape() {
-  return sizeof(T) == 4 ? 0 : 3;
+  return sizeof(T) == 4 ? 0 : 2;
 }
 
 // One workgroup per problem; outer_costs holds max_iterations doubles per

@@ -4,22 +4,22 @@
 
 namespace sfmba {
 
-SFMBA_WINDOW_SHAPE(float, 1024, 8)
-SFMBA_WINDOW_SHAPE(float, 1024, 4)
-SFMBA_WINDOW_SHAPE(double, 1024, 8)
-SFMBA_WINDOW_SHAPE(double, 1024, 4)
+SFMBA_WINDOW_SHAPE(float, 1024, 2)
+SFMBA_WINDOW_SHAPE(float, 1024, 1)
+SFMBA_WINDOW_SHAPE(double, 1024, 2)
+SFMBA_WINDOW_SHAPE(double, 1024, 1)
 
 template <typename T>
 void launch_window_lm(const View<T>* views, uint32_t n_win, uint32_t max_grp_dim, const WindowParams& prm,
                       WindowResult* res, double* outer_costs, int shape, hipStream_t s) {
   if (n_win == 0) return;
   switch (shape) {
-    case 0: launch_window_shape<T, 1024, 8>(views, n_win, max_grp_dim, prm, res, outer_costs, s); break;
-    case 1: launch_window_shape<T, 1024, 4>(views, n_win, max_grp_dim, prm, res, outer_costs, s); break;
-    case 2: launch_window_shape<T, 512, 8>(views, n_win, max_grp_dim, prm, res, outer_costs, s); break;
-    case 3: launch_window_shape<T, 512, 4>(views, n_win, max_grp_dim, prm, res, outer_costs, s); break;
-    case 4: launch_window_shape<T, 256, 8>(views, n_win, max_grp_dim, prm, res, outer_costs, s); break;
-    case 5: launch_window_shape<T, 256, 4>(views, n_win, max_grp_dim, prm, res, outer_costs, s); break;
+    case 0: launch_window_shape<T, 1024, 2>(views, n_win, max_grp_dim, prm, res, outer_costs, s); break;
+    case 1: launch_window_shape<T, 1024, 1>(views, n_win, max_grp_dim, prm, res, outer_costs, s); break;
+    case 2: launch_window_shape<T, 512, 2>(views, n_win, max_grp_dim, prm, res, outer_costs, s); break;
+    case 3: launch_window_shape<T, 512, 1>(views, n_win, max_grp_dim, prm, res, outer_costs, s); break;
+    case 4: launch_window_shape<T, 256, 2>(views, n_win, max_grp_dim, prm, res, outer_costs, s); break;
+    case 5: launch_window_shape<T, 256, 1>(views, n_win, max_grp_dim, prm, res, outer_costs, s); break;
     default: throw Error(SFMBA_ERR_INVALID_ARGUMENT, "window launch shape out of range");
   }
   HIP_CHECK(hipGetLastError());

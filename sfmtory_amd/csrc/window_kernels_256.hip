@@ -2,8 +2,8 @@
 #include "window_lm.hpp"
 
 namespace sfmba {
-SFMBA_WINDOW_SHAPE(float, 256, 8)
-SFMBA_WINDOW_SHAPE(float, 256, 4)
-SFMBA_WINDOW_SHAPE(double, 256, 8)
-SFMBA_WINDOW_SHAPE(double, 256, 4)
+SFMBA_WINDOW_SHAPE(float, 256, 2)
+SFMBA_WINDOW_SHAPE(float, 256, 1)
+SFMBA_WINDOW_SHAPE(double, 256, 2)
+SFMBA_WINDOW_SHAPE(double, 256, 1)
 }  // namespace sfmba

@@ -2,8 +2,8 @@
 #include "window_lm.hpp"
 
 namespace sfmba {
-SFMBA_WINDOW_SHAPE(float, 512, 8)
-SFMBA_WINDOW_SHAPE(float, 512, 4)
-SFMBA_WINDOW_SHAPE(double, 512, 8)
-SFMBA_WINDOW_SHAPE(double, 512, 4)
+SFMBA_WINDOW_SHAPE(float, 512, 2)
+SFMBA_WINDOW_SHAPE(float, 512, 1)
+SFMBA_WINDOW_SHAPE(double, 512, 2)
+SFMBA_WINDOW_SHAPE(double, 512, 1)
 }  // namespace sfmba

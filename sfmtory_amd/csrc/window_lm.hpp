@@ -1,6 +1,6 @@
 // Persistent batched solver for many small problems (local-BA windows):
 // the kernel template, instantiated per launch shape (threads per window,
-// Schur pairs in flight) by window_kernels*.hip.
+// batch depth of the Schur reduction) by window_kernels*.hip.
 //
 // One workgroup runs the whole Levenberg-Marquardt solve of
 // one problem: the reference's bundle_adjust (crates/sfm-ba/src/lib.rs:
@@ -299,41 +299,45 @@ __device__ inline bool wg_lu_solve(double* S, uint32_t D, double* x, WgShared& s
   return ok;
 }
 
-template <typename T, int NQ, int NP>
-__device__ inline void wg_assemble(const View<T>& v, double lam) {
+// The reduced system at damping lam from the incidence records: point
+// inverses, Y of every record (four lanes per incidence), then the
+// record-based reduction (engine_device.hpp schur_chunk_rec), one chunk per
+// destination, each wavefront with its own part of the LDS stage.
+template <typename T, int DB, int INS>
+__device__ inline void wg_assemble(const View<T>& v, double lam, T* stage) {
   for (uint32_t p = threadIdx.x; p < v.n_pt; p += WG) point_prepare(v, p, lam);
   __syncthreads();
-  for (uint32_t a = threadIdx.x; a < v.n_inc; a += WG) compute_F(v, a);
+  for (uint32_t t = threadIdx.x; t < 4 * v.n_inc; t += WG) compute_Y<T, DB>(v, t >> 2, t & 3u);
   __syncthreads();
-  // Blocks of at most 8x8 (NQ == 1: at most 64 entries): the matrix-core
-  // form with two pairs per MFMA (chunks of any length); otherwise the
-  // vector form with NP pairs in flight per wavefront.
-  for (uint32_t ci = threadIdx.x >> 6; ci < v.n_chunks; ci += NWAVE) {
-    if constexpr (NQ == 1)
-      schur_chunk_mfma8<T, 2, true>(v, v.chunks[ci], lam, nullptr);
-    else
-      schur_chunk<T, NQ, NP>(v, v.chunks[ci], lam, nullptr);
-  }
+  const uint32_t wave = threadIdx.x >> 6;
+  T* lds = stage + wave * RecShape<T, DB, INS>::LDS;
+  for (uint32_t ci = wave; ci < v.n_chunks; ci += NWAVE)
+    schur_chunk_rec<T, DB, true, INS>(v, v.chunks[ci], lam, nullptr, lds);
   __syncthreads();
 }
 
-template <typename T>
+template <typename T, int DB>
 __device__ inline void wg_step_and_cost(const View<T>& v, int cur, int loss, double scale,
                                         WgShared& sh, double& trial, double& n_front) {
-  for (uint32_t p = threadIdx.x; p < v.n_pt; p += WG) backsub_point(v, p, cur);
+  for (uint32_t p = threadIdx.x; p < v.n_pt; p += WG) backsub_point_rec<T, DB>(v, p, cur);
   for (uint32_t i = threadIdx.x; i < v.n_img; i += WG) update_pose(v, i, cur);
   for (uint32_t c = threadIdx.x; c < v.n_cam; c += WG) update_cam(v, c, cur);
   __syncthreads();
   wg_cost(v, 1 - cur, loss, scale, sh, trial, n_front);
 }
 
-// WGS threads per window, NP Schur pairs in flight per wavefront.
-template <typename T, int NQ, int WGS, int NP>
+// WGS threads per window, records of DB rows, INS load instructions per
+// record side and batch of the Schur reduction.  The Cholesky's LDS panel
+// doubles as the reduction's stage (the two never run at once).
+template <typename T, int DB, int WGS, int INS>
 __global__ __launch_bounds__(WGS) void k_window_lm(const View<T>* __restrict__ views,
                                                   WindowParams prm, WindowResult* __restrict__ res,
                                                   double* __restrict__ outer_costs) {
+  static_assert((WGS / 64) * RecShape<T, DB, INS>::LDS * sizeof(T) <= PANEL_ROWS * PB * sizeof(double),
+                "the reduction's stage fits the panel");
   __shared__ WgShared sh;
-  __shared__ double panel[PANEL_ROWS * PB];
+  __shared__ __attribute__((aligned(16))) double panel[PANEL_ROWS * PB];
+  T* stage = reinterpret_cast<T*>(panel);
   const View<T>& v = views[blockIdx.x];
   WindowResult& R = res[blockIdx.x];
   double* oc = outer_costs + (size_t)blockIdx.x * prm.max_iterations;
@@ -353,7 +357,7 @@ __global__ __launch_bounds__(WGS) void k_window_lm(const View<T>* __restrict__ v
       ++iterations;
       const double cost_before = cost;
       if (n_front == 0.0) break;  // no observation in front of its camera (lib.rs:855-857)
-      for (uint32_t o = threadIdx.x; o < v.n_obs; o += WG) linearize_obs(v, o, cur, loss, scale);
+      for (uint32_t o = threadIdx.x; o < v.n_obs; o += WG) linearize_obs<T, DB>(v, o, cur, loss, scale);
       __syncthreads();
       for (uint32_t p = threadIdx.x; p < v.n_pt; p += WG) point_accumulate(v, p);
       __syncthreads();
@@ -365,7 +369,7 @@ __global__ __launch_bounds__(WGS) void k_window_lm(const View<T>* __restrict__ v
         bool have_step = false, stepped = false;
         // FP32-assembled systems are damped at least at their resolution
         const double lam0 = mixed ? fmax(lambda, MIXED_MIN_LAMBDA) : lambda;
-        wg_assemble<T, NQ, NP>(v, lam0);
+        wg_assemble<T, DB, INS>(v, lam0, stage);
         if (wg_cholesky(v.S, v.D, panel, sh)) {
           have_step = true;
         } else {
@@ -376,17 +380,17 @@ __global__ __launch_bounds__(WGS) void k_window_lm(const View<T>* __restrict__ v
             // carry no information)
           } else {
             // not positive definite: rebuild S and solve by LU (lib.rs:1083-1093)
-            wg_assemble<T, NQ, NP>(v, lam0);
+            wg_assemble<T, DB, INS>(v, lam0, stage);
             ++n_lu;
             if (wg_lu_solve(v.S, v.D, v.delta, sh)) {
-              wg_step_and_cost(v, cur, loss, scale, sh, trial, trial_front);
+              wg_step_and_cost<T, DB>(v, cur, loss, scale, sh, trial, trial_front);
               have_step = stepped = true;
             }
           }
         }
         if (have_step && !stepped) {
           wg_backsolve(v.S, v.D, v.delta);
-          wg_step_and_cost(v, cur, loss, scale, sh, trial, trial_front);
+          wg_step_and_cost<T, DB>(v, cur, loss, scale, sh, trial, trial_front);
         }
         if (!have_step) {
           lambda *= nu;
@@ -427,25 +431,26 @@ __global__ __launch_bounds__(WGS) void k_window_lm(const View<T>* __restrict__ v
   }
 }
 
-// Launch of one shape, dispatched on the destination block size class.
-template <typename T, int WGS, int NP>
+// Launch of one shape, dispatched on the record size (the batch's largest
+// block, rec_db_for).
+template <typename T, int WGS, int INS>
 void launch_window_shape(const View<T>* views, uint32_t n_win, uint32_t max_grp_dim, const WindowParams& prm,
                          WindowResult* res, double* outer_costs, hipStream_t s) {
-  const uint32_t ne = max_grp_dim * max_grp_dim;
   auto go = [&](auto kern) {
     hipLaunchKernelGGL(kern, dim3(n_win), dim3(WGS), 0, s, views, prm, res, outer_costs);
   };
-  if (ne <= 64) go(k_window_lm<T, 1, WGS, NP>);
-  else if (ne <= 128) go(k_window_lm<T, 2, WGS, NP>);
-  else if (ne <= 192) go(k_window_lm<T, 3, WGS, NP>);
-  else go(k_window_lm<T, 4, WGS, NP>);
+  switch (rec_db_for(max_grp_dim)) {
+    case 8: go(k_window_lm<T, 8, WGS, INS>); break;
+    case 12: go(k_window_lm<T, 12, WGS, INS>); break;
+    default: go(k_window_lm<T, 16, WGS, INS>); break;
+  }
 }
 
 #undef WG
 #undef NWAVE
 
-#define SFMBA_WINDOW_SHAPE(T, WGS, NP)                                                                       \
-  template void launch_window_shape<T, WGS, NP>(const View<T>*, uint32_t, uint32_t, const WindowParams&,   \
-                                                WindowResult*, double*, hipStream_t);
+#define SFMBA_WINDOW_SHAPE(T, WGS, INS)                                                                      \
+  template void launch_window_shape<T, WGS, INS>(const View<T>*, uint32_t, uint32_t, const WindowParams&,  \
+                                                 WindowResult*, double*, hipStream_t);
 
 }  // namespace sfmba
