@@ -48,6 +48,7 @@ inline size_t dense_workspace_doubles(uint32_t Dp) {
 // LM schedule itself is unchanged).
 constexpr double MIXED_MIN_LAMBDA = SFMBA_MIXED_MIN_LAMBDA;
 constexpr int PAIR_CHUNK = 128;   // max point pairs reduced by one wavefront
+constexpr uint32_t LINEARIZE_SPAN = 64;  // observations per linearization wavefront
 constexpr uint64_t SCHUR_TILE = 64; // destination tile edge (parameter blocks) of the chunk order
 
 // Incidence record of the single-problem solve: everything the Schur
@@ -116,6 +117,9 @@ struct Layout {
   double2* obs_xy = nullptr;
   uint32_t* obs_inc = nullptr;  // 2 per obs, NONE if absent
   uint32_t* pt_ptr = nullptr;   // n_pt + 1
+  uint32_t n_pt_fix = 0;
+  uint32_t* pt_fix = nullptr;   // points with no observation or observations in two
+                                // LINEARIZE_SPAN spans (summed by k_point_accumulate)
   // incidences
   uint32_t* inc_grp = nullptr;
   uint32_t* inc_pt = nullptr;

@@ -102,9 +102,10 @@ struct Acc<double> {
 // stage, an observation of one incidence leaves its record pieces there
 // (V2 k of X2 R2 at k, J2 row s at 4 + s) for the caller to store; one of
 // two incidences writes its records directly.  With pc_reg, the point
-// terms go there (the caller's registers) instead of obs_pc.
+// terms go there (the caller's registers) instead of obs_pc.  Returns
+// whether the point is in front of the camera (obs_valid).
 template <typename T, int DB>
-__device__ inline void linearize_obs(const View<T>& v, uint32_t o, int cur, int loss, double scale,
+__device__ inline bool linearize_obs(const View<T>& v, uint32_t o, int cur, int loss, double scale,
                                      typename Vec2<T>::type* stage = nullptr, T* pc_reg = nullptr) {
   using V2 = typename Vec2<T>::type;
   const uint32_t img = v.obs_img[o];
@@ -165,7 +166,7 @@ __device__ inline void linearize_obs(const View<T>& v, uint32_t o, int cur, int 
       if (s >= ns) break;
       put_row(s, z2);
     }
-    return;
+    return false;
   }
   v.obs_valid[o] = 1;
   const double2 xy = v.obs_xy[o];
@@ -259,6 +260,7 @@ __device__ inline void linearize_obs(const View<T>& v, uint32_t o, int cur, int 
     if (s >= ns) break;
     put_row(s, V2{sw * s0[s], sw * s1[s]});
   }
+  return true;
 }
 
 // ------------------------------------------------------- point accumulation

@@ -115,6 +115,7 @@ struct Pack {
   uint32_t n_img = 0, n_cam = 0, n_pt = 0, n_obs = 0;
   uint32_t D = 0, Dp = 0, ld = 0, n_grp = 0;
   std::vector<uint32_t> obs_img, obs_pt, obs_inc, pt_ptr;
+  std::vector<uint32_t> pt_fix;  // points left to k_point_accumulate
   std::vector<double2> obs_xy;
   std::vector<uint32_t> inc_grp, inc_pt, inc_obs, pt_inc_ptr;
   std::vector<uint32_t> cam_of_img, img_slot, cam_slot, cam_nfree, grp_off, grp_dim;
@@ -223,6 +224,14 @@ static void build_pack(const sfmba_input* in, Pack& P, bool need_pairs,
   P.pt_ptr.assign((size_t)P.n_pt + 1, 0);
   for (uint32_t o = 0; o < P.n_obs; ++o) ++P.pt_ptr[in->observations[o].point_idx + 1];
   for (uint32_t p = 0; p < P.n_pt; ++p) P.pt_ptr[p + 1] += P.pt_ptr[p];
+  // The linearization sums the point terms of a point whose observations
+  // lie in one aligned span of LINEARIZE_SPAN; the others (and points
+  // without observations) are summed by the accumulation pass.
+  P.pt_fix.clear();
+  for (uint32_t p = 0; p < P.n_pt; ++p) {
+    const uint32_t b = P.pt_ptr[p], e = P.pt_ptr[p + 1];
+    if (b == e || b / LINEARIZE_SPAN != (e - 1) / LINEARIZE_SPAN) P.pt_fix.push_back(p);
+  }
   std::vector<uint32_t> fill(P.pt_ptr.begin(), P.pt_ptr.end() - 1), order(P.n_obs);
   for (uint32_t o = 0; o < P.n_obs; ++o) order[fill[in->observations[o].point_idx]++] = o;
   P.obs_img.resize(P.n_obs);
@@ -355,6 +364,8 @@ static void upload_layout(const Pack& P, Problem& pr, hipStream_t s) {
   L.obs_xy = A.upload(P.obs_xy, s);
   L.obs_inc = A.upload(P.obs_inc, s);
   L.pt_ptr = A.upload(P.pt_ptr, s);
+  L.n_pt_fix = (uint32_t)P.pt_fix.size();
+  L.pt_fix = A.upload(P.pt_fix, s);
   L.inc_grp = A.upload(P.inc_grp, s);
   L.inc_pt = A.upload(P.inc_pt, s);
   L.inc_obs = A.upload(P.inc_obs, s);

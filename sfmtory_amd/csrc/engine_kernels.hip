@@ -15,9 +15,10 @@ namespace sfmba {
 // wavefront stores the pieces of 64 consecutive observations, lane l taking
 // piece l of each group of 64, so that one store instruction covers
 // consecutive pieces of neighbouring records instead of one piece in each
-// of 64 records; the point terms (nine values per observation, contiguous
-// over the wavefront's observations) follow through the same stage.  An
-// observation of two incidences writes its records itself.  Against four
+// of 64 records; the point terms (nine values per observation) follow
+// through the same stage and are summed there into the point blocks of the
+// points seen whole by the wavefront.  An observation of two incidences
+// writes its records itself.  Against four
 // lanes per observation that each compute the whole Jacobian and store a
 // quarter of its pieces: FP64 0.50 -> 0.47 ms, FP32 0.35 -> 0.27 ms per
 // linearization of the 1,000-camera scene (with the point accumulation).
@@ -33,15 +34,22 @@ __global__ __launch_bounds__(256) void k_linearize_staged(
   const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const uint32_t o = blockIdx.x * blockDim.x + threadIdx.x;
   V2* sw = stage + w * 64 * P;
+  static_assert(LINEARIZE_SPAN == 64, "one wavefront per span");
   uint32_t inc = NONE, nrows = 0;  // the staged record and its J2 rows
+  uint32_t pb = 0, pe = 0, p = 0;  // the observation's point and its observations
+  bool valid = false, act = false;
   T pc[9];
   if (o < v.n_obs) {
-    linearize_obs<T, DB>(v, o, cur, loss, scale, sw + lane * P, pc);
+    valid = linearize_obs<T, DB>(v, o, cur, loss, scale, sw + lane * P, pc);
     const uint32_t i0 = v.obs_inc[2 * o];
+    act = valid && i0 != NONE;
     if (i0 != NONE && v.obs_inc[2 * o + 1] == NONE) {
       inc = i0;
       nrows = v.grp_dim[v.inc_grp[i0]];
     }
+    p = v.obs_pt[o];
+    pb = v.pt_ptr[p];
+    pe = v.pt_ptr[p + 1];
   }
   // the wavefront's stage is complete (one wavefront writes and reads it)
   __asm__ volatile("s_waitcnt vmcnt(0) lgkmcnt(0)" ::: "memory");
@@ -54,7 +62,10 @@ __global__ __launch_bounds__(256) void k_linearize_staged(
     const uint32_t srows = (uint32_t)__shfl((int)nrows, (int)slot);
     if (sinc != NONE && (piece < 4 || piece - 4 < srows)) recv[(size_t)sinc * RS2 + piece] = sw[idx];
   }
-  // point terms: values 9 o0 .. 9 o_end of obs_pc, in pieces of two
+  // Point terms: through the stage, then summed here for the points whose
+  // observations all lie in this span (the lane of a point's last
+  // observation adds the terms of its valid observations in order) and
+  // stored in obs_pc for the others (k_point_accumulate sums those).
   static_assert(64 * 9 <= 2 * 64 * P, "the point terms fit the stage");
   __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
   T* st = reinterpret_cast<T*>(sw);
@@ -62,26 +73,45 @@ __global__ __launch_bounds__(256) void k_linearize_staged(
   for (int k = 0; k < 9; ++k) st[lane * 9 + k] = pc[k];
   __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
   const uint32_t o0 = o - lane;
-  const uint32_t nv = o0 < v.n_obs ? 9 * min(64u, v.n_obs - o0) : 0u;
-  V2* pcv = reinterpret_cast<V2*>(v.obs_pc + 9 * (size_t)o0);
-  const V2* sv = reinterpret_cast<const V2*>(st);
+  const bool in_span = o < v.n_obs && pb / LINEARIZE_SPAN == (pe - 1) / LINEARIZE_SPAN;
+  const uint64_t split = __ballot(o < v.n_obs && !in_span);
+  const uint64_t vmask = __ballot(valid), amask = __ballot(act);
+  if (split != 0) {
 #pragma unroll
-  for (uint32_t j = 0; j < 5; ++j) {
-    const uint32_t q = j * 64 + lane;  // piece q: values 2q, 2q + 1
-    if (2 * q + 1 < nv)
-      pcv[q] = sv[q];
-    else if (2 * q < nv)  // an odd count's last value
-      v.obs_pc[9 * (size_t)o0 + 2 * q] = st[2 * q];
+    for (uint32_t k = 0; k < 9; ++k) {
+      const uint32_t idx = k * 64 + lane;  // value idx of the span's terms
+      if ((split >> (idx / 9)) & 1u) v.obs_pc[9 * (size_t)o0 + idx] = st[idx];
+    }
+  }
+  if (in_span && o + 1 == pe) {
+    double a[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) a[k] = 0.0;
+    for (uint32_t j = pb - o0; j <= lane; ++j) {
+      if (!((vmask >> j) & 1u)) continue;
+#pragma unroll
+      for (int k = 0; k < 9; ++k) a[k] += (double)st[j * 9 + k];
+    }
+    const uint32_t h = pb - o0;
+    const uint64_t range = (lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1)) & ~((1ull << h) - 1);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) v.pt_C[6 * (size_t)p + k] = a[k];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) v.pt_g[3 * (size_t)p + k] = a[6 + k];
+    v.pt_active[p] = (amask & range) != 0 ? 1 : 0;
   }
 }
 
-// LPP lanes per point (neighbouring lanes read neighbouring observations).
+// The points of a list (those the linearization leaves: no observation, or
+// observations in two spans), LPP lanes per point (neighbouring lanes read
+// neighbouring observations).
 constexpr int ACCUMULATE_LANES = 4;
 template <typename T>
-__global__ __launch_bounds__(256) void k_point_accumulate(View<T> v) {
+__global__ __launch_bounds__(256) void k_point_accumulate(View<T> v, const uint32_t* __restrict__ list,
+                                                          uint32_t n) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t p = t / ACCUMULATE_LANES;
-  if (p < v.n_pt) point_accumulate<T, ACCUMULATE_LANES>(v, p, t % ACCUMULATE_LANES);
+  const uint32_t i = t / ACCUMULATE_LANES;
+  if (i < n) point_accumulate<T, ACCUMULATE_LANES>(v, list[i], t % ACCUMULATE_LANES);
 }
 
 template <typename T>
@@ -429,9 +459,9 @@ void launch_linearize(const Layout& L, Work<T>& W, int cur, int loss, double sca
 
 template <typename T>
 void launch_point_accumulate(const Layout& L, Work<T>& W, hipStream_t s) {
-  if (L.n_pt == 0) return;
-  hipLaunchKernelGGL(k_point_accumulate<T>, dim3(blocks_for((uint64_t)L.n_pt * ACCUMULATE_LANES, 256)), dim3(256), 0, s,
-                     make_view(L, W));
+  if (L.n_pt_fix == 0) return;
+  hipLaunchKernelGGL(k_point_accumulate<T>, dim3(blocks_for((uint64_t)L.n_pt_fix * ACCUMULATE_LANES, 256)), dim3(256),
+                     0, s, make_view(L, W), (const uint32_t*)L.pt_fix, L.n_pt_fix);
   HIP_CHECK(hipGetLastError());
 }
 
