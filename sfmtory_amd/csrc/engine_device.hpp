@@ -100,10 +100,11 @@ struct Acc<double> {
 // (rec_values: X2 and R2, and one J2 row per unknown of the block) and its
 // contribution to its point's 3x3 block and gradient (obs_pc).  With a
 // stage, an observation of one incidence leaves its record pieces there
-// (V2 k of X2 R2 at k, J2 row s at 4 + s) for the caller to store; one of
-// two incidences writes its records directly.  With pc_reg, the point
-// terms go there (the caller's registers) instead of obs_pc.  Returns
-// whether the point is in front of the camera (obs_valid).
+// (V2 k of X2 R2 at k, J2 row s at 4 + s, zero rows up to DB) for the
+// caller to store; one of two incidences writes its records directly.
+// With pc_reg, the point terms go there (the caller's registers) instead
+// of obs_pc.  Returns whether the point is in front of the camera
+// (obs_valid).
 template <typename T, int DB>
 __device__ inline bool linearize_obs(const View<T>& v, uint32_t o, int cur, int loss, double scale,
                                      typename Vec2<T>::type* stage = nullptr, T* pc_reg = nullptr) {
@@ -166,6 +167,10 @@ __device__ inline bool linearize_obs(const View<T>& v, uint32_t o, int cur, int 
       if (s >= ns) break;
       put_row(s, z2);
     }
+    if (staged)
+#pragma unroll
+      for (uint32_t s = 0; s < DB; ++s)
+        if (s >= ns) stage[4 + s] = z2;
     return false;
   }
   v.obs_valid[o] = 1;
@@ -260,6 +265,10 @@ __device__ inline bool linearize_obs(const View<T>& v, uint32_t o, int cur, int 
     if (s >= ns) break;
     put_row(s, V2{sw * s0[s], sw * s1[s]});
   }
+  if (staged)
+#pragma unroll
+    for (uint32_t s = 0; s < DB; ++s)
+      if (s >= ns) stage[4 + s] = V2{T(0), T(0)};
   return true;
 }
 

@@ -35,7 +35,7 @@ __global__ __launch_bounds__(256) void k_linearize_staged(
   const uint32_t o = blockIdx.x * blockDim.x + threadIdx.x;
   V2* sw = stage + w * 64 * P;
   static_assert(LINEARIZE_SPAN == 64, "one wavefront per span");
-  uint32_t inc = NONE, nrows = 0;  // the staged record and its J2 rows
+  uint32_t inc = NONE;  // the staged record (all DB rows: those past the block's are zero)
   uint32_t pb = 0, pe = 0, p = 0;  // the observation's point and its observations
   bool valid = false, act = false;
   T pc[9];
@@ -43,10 +43,7 @@ __global__ __launch_bounds__(256) void k_linearize_staged(
     valid = linearize_obs<T, DB>(v, o, cur, loss, scale, sw + lane * P, pc);
     const uint32_t i0 = v.obs_inc[2 * o];
     act = valid && i0 != NONE;
-    if (i0 != NONE && v.obs_inc[2 * o + 1] == NONE) {
-      inc = i0;
-      nrows = v.grp_dim[v.inc_grp[i0]];
-    }
+    if (i0 != NONE && v.obs_inc[2 * o + 1] == NONE) inc = i0;
     p = v.obs_pt[o];
     pb = v.pt_ptr[p];
     pe = v.pt_ptr[p + 1];
@@ -59,8 +56,7 @@ __global__ __launch_bounds__(256) void k_linearize_staged(
     const uint32_t idx = j * 64 + lane;
     const uint32_t slot = idx / P, piece = idx - slot * P;
     const uint32_t sinc = (uint32_t)__shfl((int)inc, (int)slot);
-    const uint32_t srows = (uint32_t)__shfl((int)nrows, (int)slot);
-    if (sinc != NONE && (piece < 4 || piece - 4 < srows)) recv[(size_t)sinc * RS2 + piece] = sw[idx];
+    if (sinc != NONE) recv[(size_t)sinc * RS2 + piece] = sw[idx];
   }
   // Point terms: through the stage, then summed here for the points whose
   // observations all lie in this span (the lane of a point's last
