@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "engine_device.hpp"
 
@@ -30,7 +31,7 @@ __global__ __launch_bounds__(256) void k_linearize_staged(
   using V2 = typename Vec2<T>::type;
   constexpr uint32_t P = 4 + DB;
   constexpr size_t RS2 = rec_values(DB) / 2;
-  __shared__ V2 stage[4 * 64 * P];
+  __shared__ __attribute__((aligned(16))) V2 stage[4 * 64 * P];
   const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const uint32_t o = blockIdx.x * blockDim.x + threadIdx.x;
   V2* sw = stage + w * 64 * P;
@@ -50,13 +51,20 @@ __global__ __launch_bounds__(256) void k_linearize_staged(
   }
   // the wavefront's stage is complete (one wavefront writes and reads it)
   __asm__ volatile("s_waitcnt vmcnt(0) lgkmcnt(0)" ::: "memory");
-  V2* recv = reinterpret_cast<V2*>(v.rec);
+  // 16-byte units: two pieces of an FP32 record, one of an FP64 record
+  using U = typename std::conditional<sizeof(T) == 4, float4, double2>::type;
+  constexpr uint32_t PPU = sizeof(U) / sizeof(V2);
+  static_assert(P % PPU == 0 && RS2 % PPU == 0, "records in whole 16-byte units");
+  constexpr uint32_t PU = P / PPU;
+  constexpr size_t RSU = RS2 / PPU;
+  U* recu = reinterpret_cast<U*>(v.rec);
+  const U* su = reinterpret_cast<const U*>(sw);
 #pragma unroll
-  for (uint32_t j = 0; j < P; ++j) {
+  for (uint32_t j = 0; j < PU; ++j) {
     const uint32_t idx = j * 64 + lane;
-    const uint32_t slot = idx / P, piece = idx - slot * P;
+    const uint32_t slot = idx / PU, piece = idx - slot * PU;
     const uint32_t sinc = (uint32_t)__shfl((int)inc, (int)slot);
-    if (sinc != NONE) recv[(size_t)sinc * RS2 + piece] = sw[idx];
+    if (sinc != NONE) recu[(size_t)sinc * RSU + piece] = su[idx];
   }
   // Point terms: through the stage, then summed here for the points whose
   // observations all lie in this span (the lane of a point's last
