@@ -121,6 +121,31 @@ def test_behind_camera_and_unobserved_points():
     assert_parity(inp, ba.BaParams(max_iterations=10))
 
 
+def test_long_tracks_across_linearization_spans():
+    # Tracks of 60-139 observations, whose observations (in point order)
+    # cross one or two boundaries of the linearization's 64-observation
+    # spans, so their blocks are summed by the separate accumulation pass;
+    # 30 points of three observations after them, most of which lie in one
+    # span (summed inside the linearization); and an unobserved point.
+    sc = scenes.generate("c0", num_images=140, num_points=100, seed=77, track_kind=0, track_min=60,
+                         track_max=139)
+    inp = sc.inp
+    obs, pts = inp.observations, np.asarray(inp.points)
+    extra = []
+    for k in range(30):
+        o = obs[obs["point_idx"] == k][:3].copy()
+        o["point_idx"] = 100 + k
+        extra.append(o)
+    inp.observations = np.concatenate([obs] + extra)
+    inp.points = np.vstack([pts, pts[:30], [[0.1, 0.2, 0.3]]])
+    counts = np.bincount(inp.observations["point_idx"].astype(int), minlength=len(inp.points))
+    end = np.cumsum(counts)
+    begin = end - counts
+    whole = (counts > 0) & (begin // 64 == (end - 1) // 64)
+    assert (counts > 128).sum() > 0 and whole.sum() > 20 and counts[-1] == 0
+    assert_parity(inp, ba.BaParams(max_iterations=5))
+
+
 def test_duplicate_observations():
     sc = scenes.generate("c0", num_points=200, seed=12)
     inp = sc.inp
