@@ -184,21 +184,27 @@ __global__ __launch_bounds__(256) void k_schur_reduce(
 
 // Point back-substitution in two passes.  First E_a^T delta_g = X2^T
 // (J2_a^T delta_g) per incidence a: LPI lanes per incidence, one J2 row per
-// lane, summed across the lanes.
+// lane, summed across the lanes.  The record's row and X2 are loaded before
+// the block's offset and size are known (rows past the block's are zero),
+// so the record loads do not wait on the chain incidence -> block -> delta.
 template <typename T, int DB, int LPI>
 __global__ __launch_bounds__(256) void k_backsub_inc(View<T> v, double* __restrict__ inc_part) {
+  using V2 = typename Vec2<T>::type;
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t a = t / LPI, r = t % LPI;
   double t0 = 0.0, t1 = 0.0;
-  const T* rc = nullptr;
+  T x2[6];
   if (a < v.n_inc) {
+    const T* rc = v.rec + (size_t)a * rec_values(DB);
+    V2 row = {T(0), T(0)};
+    if (r < (uint32_t)DB) row = *reinterpret_cast<const V2*>(rc + 8 + 2 * r);
+    if (r == 0)
+#pragma unroll
+      for (int j = 0; j < 6; ++j) x2[j] = rc[j];
     const uint32_t g = v.inc_grp[a];
-    rc = v.rec + (size_t)a * rec_values(DB);
-    if (r < v.grp_dim[g]) {
-      const double d = v.delta[v.grp_off[g] + r];
-      t0 = (double)rc[8 + 2 * r] * d;
-      t1 = (double)rc[8 + 2 * r + 1] * d;
-    }
+    const double d = r < v.grp_dim[g] ? v.delta[v.grp_off[g] + r] : 0.0;
+    t0 = (double)row.x * d;
+    t1 = (double)row.y * d;
   }
   // every lane of the wavefront takes part (LPI divides 64)
 #pragma unroll
@@ -208,7 +214,7 @@ __global__ __launch_bounds__(256) void k_backsub_inc(View<T> v, double* __restri
   }
   if (a < v.n_inc && r == 0) {
 #pragma unroll
-    for (int j = 0; j < 3; ++j) inc_part[3 * (size_t)a + j] = (double)rc[j] * t0 + (double)rc[3 + j] * t1;
+    for (int j = 0; j < 3; ++j) inc_part[3 * (size_t)a + j] = (double)x2[j] * t0 + (double)x2[3 + j] * t1;
   }
 }
 
