@@ -49,6 +49,8 @@ inline size_t dense_workspace_doubles(uint32_t Dp) {
 constexpr double MIXED_MIN_LAMBDA = SFMBA_MIXED_MIN_LAMBDA;
 constexpr int PAIR_CHUNK = 128;   // max point pairs reduced by one wavefront
 constexpr uint32_t LINEARIZE_SPAN = 64;  // observations per linearization wavefront
+constexpr int COST_BLOCK = 256;          // threads per block of the cost kernel
+constexpr int COST_ITEMS = 8;            // observations per thread of the cost kernel
 constexpr uint64_t SCHUR_TILE = 64; // destination tile edge (parameter blocks) of the chunk order
 
 // Incidence record of the single-problem solve: everything the Schur

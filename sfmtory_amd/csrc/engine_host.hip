@@ -616,7 +616,7 @@ static void alloc_work(sfmba_problem& H, Work<T>& W, double* shared_S = nullptr,
     W.cams[k] = A.alloc<double>(8 * (size_t)L.n_cam);
     W.pts[k] = A.alloc<double>(3 * (size_t)L.n_pt);
   }
-  const uint32_t n_cost_part = (L.n_obs + 2047) / 2048 + 1;
+  const uint32_t n_cost_part = (L.n_obs + COST_BLOCK * COST_ITEMS - 1) / (COST_BLOCK * COST_ITEMS) + 1;
   W.cost_part = A.alloc<double>(2 * (size_t)n_cost_part);
   W.cost = A.alloc<double>(8);
   W.flags = A.alloc<int>(4);
@@ -1572,7 +1572,7 @@ static double mean_error(const sfmba_input* in, int device) {
   Pack P;
   double *po, *ca, *pt;
   upload_state(in, pr, P, sg.s, &po, &ca, &pt);
-  double* part = pr.arena.alloc<double>(2 * ((size_t)pr.L.n_obs / 2048 + 2));
+  double* part = pr.arena.alloc<double>(2 * ((size_t)pr.L.n_obs / (COST_BLOCK * COST_ITEMS) + 2));
   double* res = pr.arena.alloc<double>(2);
   launch_cost(pr.L, po, ca, pt, SFMBA_LOSS_NONE, 1.0, part, res, 1, sg.s);
   double h[2];
@@ -1900,7 +1900,7 @@ int sfmba_total_cost(const sfmba_input* input, const sfmba_params* params, doubl
     Pack P;
     double *po, *ca, *pt;
     upload_state(input, pr, P, sg.s, &po, &ca, &pt);
-    double* part = pr.arena.alloc<double>(2 * ((size_t)pr.L.n_obs / 2048 + 2));
+    double* part = pr.arena.alloc<double>(2 * ((size_t)pr.L.n_obs / (COST_BLOCK * COST_ITEMS) + 2));
     double* res = pr.arena.alloc<double>(2);
     launch_cost(pr.L, po, ca, pt, params->robust_loss, params->loss_scale_px, part, res, 0, sg.s);
     double h[2];

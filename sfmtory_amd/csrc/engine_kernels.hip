@@ -260,8 +260,6 @@ __global__ __launch_bounds__(256) void k_update_cams(View<T> v, int cur) {
 // mode 0: sum w(|r|) |r|^2 (total_cost); mode 1: sum |r| (the numerator of
 // mean_reprojection_error).  Both also count the observations in front of
 // their camera.  Deterministic two-level reduction.
-constexpr int COST_BLOCK = 256;
-constexpr int COST_ITEMS = 8;
 
 __global__ __launch_bounds__(COST_BLOCK) void k_cost(
     uint32_t n_obs, const uint32_t* __restrict__ obs_img, const uint32_t* __restrict__ obs_pt,
