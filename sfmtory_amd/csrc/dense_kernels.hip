@@ -62,7 +62,6 @@ constexpr int DIAG_TILES = 36;
 constexpr int DIAG_IP = DIAG_TILES * 256;
 constexpr int DIAG_COL = DIAG_IP + NB;  // per-wavefront 16-double broadcast buffers
 constexpr int SMEM_DIAG = DIAG_COL + 4 * PW;
-constexpr int SMEM_INV = NB * NB + NB * PW + (NB - PW) * PW;  // block, X11 blocks, trtri scratch
 
 __device__ __attribute__((always_inline)) inline int tile_index(int ti, int tj) {
   return ti * (ti + 1) / 2 + tj;
@@ -375,75 +374,6 @@ __device__ __attribute__((always_inline)) inline void diag_factor(double* A, dou
   }
 }
 
-// In-place inverse of the lower triangle L held in LDS (stride NB), block
-// columns right to left: A21 <- -(X22 A21) X11 with X22 the already
-// inverted trailing triangle and X11 the diagonal blocks' inverses (Xd,
-// column-major PW x PW each), then the diagonal block <- X11.
-// tmp holds (NB - PW) x PW doubles.
-__device__ __attribute__((always_inline)) inline void lds_trtri(double* A, const double* Xd, double* tmp) {
-  const int t = threadIdx.x;
-  for (int j0 = NB - PW; j0 >= 0; j0 -= PW) {
-    const double* X11 = Xd + (j0 / PW) * PW * PW;
-    const int h = NB - j0 - PW;
-    if (h > 0) {
-      for (int e = t; e < h * PW; e += blockDim.x) {
-        const int c = e / h, rr = e - c * h;
-        tmp[c * h + rr] = A[(j0 + c) * NB + j0 + PW + rr];
-      }
-      __syncthreads();
-      // A21 <- X22 * A21, 4x4 output tiles (X22 lower, already inverted)
-      const int nt = (h / 4) * (PW / 4);
-      for (int e = t; e < nt; e += blockDim.x) {
-        const int ti = e / (PW / 4), tj = e - ti * (PW / 4);
-        const int r0 = 4 * ti, c0 = 4 * tj;
-        double acc[4][4];
-#pragma unroll
-        for (int p = 0; p < 4; ++p)
-#pragma unroll
-          for (int q = 0; q < 4; ++q) acc[p][q] = 0.0;
-#pragma unroll 4
-        for (int k = 0; k < r0 + 4; ++k) {
-          double xr[4], tc[4];
-#pragma unroll
-          for (int p = 0; p < 4; ++p) xr[p] = k <= r0 + p ? A[(j0 + PW + k) * NB + j0 + PW + r0 + p] : 0.0;
-#pragma unroll
-          for (int q = 0; q < 4; ++q) tc[q] = tmp[(c0 + q) * h + k];
-#pragma unroll
-          for (int p = 0; p < 4; ++p)
-#pragma unroll
-            for (int q = 0; q < 4; ++q) acc[p][q] += xr[p] * tc[q];
-        }
-#pragma unroll
-        for (int p = 0; p < 4; ++p)
-#pragma unroll
-          for (int q = 0; q < 4; ++q) A[(j0 + c0 + q) * NB + j0 + PW + r0 + p] = acc[p][q];
-      }
-      __syncthreads();
-      // A21 <- -A21 X11: y_c = -sum_{k >= c} a_k X11[k][c], in place from the first column
-      if (t < h) {
-        const int rr = j0 + PW + t;
-        double a[PW];
-#pragma unroll
-        for (int k = 0; k < PW; ++k) a[k] = A[(j0 + k) * NB + rr];
-#pragma unroll
-        for (int c = 0; c < PW; ++c) {
-          double s = 0.0;
-#pragma unroll
-          for (int k = c; k < PW; ++k) s += a[k] * X11[c * PW + k];
-          a[c] = -s;
-        }
-#pragma unroll
-        for (int k = 0; k < PW; ++k) A[(j0 + k) * NB + rr] = a[k];
-      }
-    }
-    for (int e = t; e < PW * PW; e += blockDim.x) {
-      const int c = e / PW, rr = e - c * PW;
-      if (rr >= c) A[(j0 + c) * NB + j0 + rr] = X11[c * PW + rr];
-    }
-    __syncthreads();
-  }
-}
-
 __global__ __launch_bounds__(256) void k_potrf_block(double* __restrict__ S, size_t ld, int k,
                                                      uint32_t D, double* __restrict__ Top,
                                                      int* __restrict__ flags) {
@@ -459,32 +389,26 @@ __global__ __launch_bounds__(256) void k_potrf_block(double* __restrict__ S, siz
   diag_factor<4>(smem, S, ld, k, D, Top, flags);
 }
 
-// L_ik = A_ik L_kk^{-T} for row blocks i = k+1 .. nblk-1 and the rhs row
-// (block nblk, first 16 rows), one wavefront per 16-row strip, entirely in
-// registers: for each 16-column sub-panel p,
+// One 16-row strip of the panel solve X = A L_kk^{-T}, one wavefront,
+// entirely in registers: for each 16-column sub-panel p,
 //   R_p = A_p - sum_{q<p} X_q L_pq^T,   X_p = R_p X11_p^T,
 // computed transposed on v_mfma_f64_16x16x4f64, whose accumulator layout
 // for X_p^T is exactly the B-operand layout X_p^T needs in the later
 // sub-panels (lane l, register v: X_p[l%16][l/16 + 4v]).  The L_pq and
 // X11_p operands come from Top in lane order (one coalesced 512-byte load
-// per operand).
-__global__ __launch_bounds__(64) void k_trsm_panel(double* __restrict__ S, size_t ld, int k,
-                                                   const double* __restrict__ Top, uint32_t D,
-                                                   uint32_t Dp) {
-  const int lane = threadIdx.x;
-  const int i = k + 1 + (int)(blockIdx.x >> 3), rt = blockIdx.x & 7;
-  const uint32_t row0 = (uint32_t)i * NB + 16 * rt;
-  const uint32_t grow = row0 + (lane & 15);
-  const bool row_ok = grow < D || grow == Dp;
-  const double* T = Top + (size_t)k * TOP_BLOCK + lane;
-  double* Acol = S + (size_t)k * NB * ld + grow;
-  const int lk = lane >> 4;
+// per operand): T is block k's Top plus the lane.  Lane l's row of the
+// strip is row l % 16: load(c) gives its A entry in column c, store(c, x)
+// takes its X entry.
+template <typename Load, typename Store>
+__device__ __attribute__((always_inline)) inline void panel_strip(Load load, const double* __restrict__ T,
+                                                                   Store store) {
+  const int lk = (threadIdx.x & 63) >> 4;
   double xb[8][4];
 #pragma unroll
   for (int p = 0; p < 8; ++p) {
     double4_t r;
 #pragma unroll
-    for (int v = 0; v < 4; ++v) r[v] = Acol[(size_t)(PW * p + lk + 4 * v) * ld];
+    for (int v = 0; v < 4; ++v) r[v] = load(PW * p + lk + 4 * v);
 #pragma unroll
     for (int q = 0; q < p; ++q)
 #pragma unroll
@@ -498,10 +422,27 @@ __global__ __launch_bounds__(64) void k_trsm_panel(double* __restrict__ S, size_
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       xb[p][v] = x[v];
-      const uint32_t col = (uint32_t)k * NB + PW * p + lk + 4 * v;
-      if (row_ok && col < D) Acol[(size_t)(PW * p + lk + 4 * v) * ld] = x[v];
+      store(PW * p + lk + 4 * v, x[v]);
     }
   }
+}
+
+// L_ik = A_ik L_kk^{-T} in place for row blocks i = k+1 .. nblk-1 and the
+// rhs row (block nblk, first 16 rows), one wavefront per strip.
+__global__ __launch_bounds__(64) void k_trsm_panel(double* __restrict__ S, size_t ld, int k,
+                                                   const double* __restrict__ Top, uint32_t D,
+                                                   uint32_t Dp) {
+  const int lane = threadIdx.x;
+  const int i = k + 1 + (int)(blockIdx.x >> 3), rt = blockIdx.x & 7;
+  const uint32_t row0 = (uint32_t)i * NB + 16 * rt;
+  const uint32_t grow = row0 + (lane & 15);
+  const bool row_ok = grow < D || grow == Dp;
+  double* Acol = S + (size_t)k * NB * ld + grow;
+  const uint32_t col0 = (uint32_t)k * NB;
+  panel_strip([&](int c) { return Acol[(size_t)c * ld]; }, Top + (size_t)k * TOP_BLOCK + lane,
+              [&](int c, double x) {
+                if (row_ok && col0 + (uint32_t)c < D) Acol[(size_t)c * ld] = x;
+              });
 }
 
 // Tile t of a trailing update, in column-major order of its lower
@@ -744,36 +685,24 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 }
 
 // L^{-1} of every diagonal block, all blocks at once after the
-// factorization (only the triangular solves use it): Linv holds, per block,
-// L^{-1} (column-major, identity on padding) followed by its transpose.
-__global__ __launch_bounds__(256) void k_block_inverse(const double* __restrict__ S, size_t ld,
-                                                       uint32_t D, const double* __restrict__ Top,
+// factorization (only the triangular solves use it), as the panel solve of
+// the identity: strip s of I L_kk^{-T} is rows 16 s .. 16 s + 15 of
+// L_kk^{-T}, that is columns of L_kk^{-1}.  One workgroup per block, one
+// strip per wavefront.  Linv holds, per block, L^{-1} (column-major,
+// identity on padding) followed by its transpose.
+__global__ __launch_bounds__(512) void k_block_inverse(uint32_t D, const double* __restrict__ Top,
                                                        double* __restrict__ Linv) {
-  __shared__ double smem[SMEM_INV];
-  const int t = threadIdx.x;
+  const int lane = threadIdx.x & 63, strip = threadIdx.x >> 6;
   const int k = blockIdx.x;
-  const uint32_t base = (uint32_t)k * NB;
-  double* A = smem;
-  double* Xd = smem + NB * NB;
-  double* tmp = Xd + NB * PW;
-  const double* G = S + (size_t)k * NB * ld + (size_t)k * NB;
-  for (int idx = t; idx < NB * NB; idx += blockDim.x) {
-    const int c = idx >> 7, r = idx & 127;
-    const bool pad = base + r >= D || base + c >= D;
-    A[c * NB + r] = pad ? (r == c ? 1.0 : 0.0) : (r >= c ? G[(size_t)c * ld + r] : 0.0);
-  }
-  // X11_p (column-major 16x16) from the panel-solve operands.
-  const double* T = Top + (size_t)k * TOP_BLOCK + 28 * 256;
-  for (int idx = t; idx < NB * PW; idx += blockDim.x) Xd[idx] = T[idx];
-  __syncthreads();
-  lds_trtri(A, Xd, tmp);
+  const int rho = 16 * strip + (lane & 15);  // row of L^{-T}, column of L^{-1}
+  const bool pad_rho = (uint32_t)k * NB + rho >= D;
   double* X = Linv + (size_t)k * LINV_BLOCK;
-  for (int idx = t; idx < NB * NB; idx += blockDim.x) {
-    const int c = idx >> 7, r = idx & 127;
-    const bool pad = base + r >= D || base + c >= D;
-    X[idx] = pad ? (r == c ? 1.0 : 0.0) : (r >= c ? A[c * NB + r] : 0.0);
-    X[NB * NB + idx] = pad ? (r == c ? 1.0 : 0.0) : (c >= r ? A[r * NB + c] : 0.0);
-  }
+  panel_strip([&](int c) { return c == rho ? 1.0 : 0.0; }, Top + (size_t)k * TOP_BLOCK + lane,
+              [&](int c, double x) {
+                if (pad_rho || (uint32_t)k * NB + c >= D) x = c == rho ? 1.0 : 0.0;
+                X[(size_t)rho * NB + c] = x;                    // L^{-1}[c][rho]
+                X[(size_t)NB * NB + (size_t)c * NB + rho] = x;  // the transpose
+              });
 }
 
 // ------------------------------------------------------ triangular solves
@@ -1106,7 +1035,7 @@ void dense_cholesky(double* S, double* Linv, uint32_t Dp, uint32_t ld, uint32_t 
       k += 1;
     }
   }
-  hipLaunchKernelGGL(k_block_inverse, dim3(nblk), dim3(256), 0, s, S, (size_t)ld, D, Top, Linv);
+  hipLaunchKernelGGL(k_block_inverse, dim3(nblk), dim3(512), 0, s, D, Top, Linv);
   HIP_CHECK(hipGetLastError());
 }
 
