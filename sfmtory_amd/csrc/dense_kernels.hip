@@ -688,16 +688,26 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 // factorization (only the triangular solves use it), as the panel solve of
 // the identity: strip s of I L_kk^{-T} is rows 16 s .. 16 s + 15 of
 // L_kk^{-T}, that is columns of L_kk^{-1}.  One workgroup per block, one
-// strip per wavefront.  Linv holds, per block, L^{-1} (column-major,
-// identity on padding) followed by its transpose.
+// strip per wavefront, the block's operands staged in LDS first (read from
+// memory by each step of the chains, 63 blocks took 59 us).  Linv holds,
+// per block, L^{-1} (column-major, identity on padding) followed by its
+// transpose.
 __global__ __launch_bounds__(512) void k_block_inverse(uint32_t D, const double* __restrict__ Top,
                                                        double* __restrict__ Linv) {
+  __shared__ __attribute__((aligned(16))) double Tl[TOP_BLOCK];
   const int lane = threadIdx.x & 63, strip = threadIdx.x >> 6;
   const int k = blockIdx.x;
+  {
+    const double2* src = reinterpret_cast<const double2*>(Top + (size_t)k * TOP_BLOCK);
+    double2* dst = reinterpret_cast<double2*>(Tl);
+#pragma unroll
+    for (int i = threadIdx.x; i < (int)TOP_BLOCK / 2; i += 512) dst[i] = src[i];
+  }
+  __syncthreads();
   const int rho = 16 * strip + (lane & 15);  // row of L^{-T}, column of L^{-1}
   const bool pad_rho = (uint32_t)k * NB + rho >= D;
   double* X = Linv + (size_t)k * LINV_BLOCK;
-  panel_strip([&](int c) { return c == rho ? 1.0 : 0.0; }, Top + (size_t)k * TOP_BLOCK + lane,
+  panel_strip([&](int c) { return c == rho ? 1.0 : 0.0; }, Tl + lane,
               [&](int c, double x) {
                 if (pad_rho || (uint32_t)k * NB + c >= D) x = c == rho ? 1.0 : 0.0;
                 X[(size_t)rho * NB + c] = x;                    // L^{-1}[c][rho]
