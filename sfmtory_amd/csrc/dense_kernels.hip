@@ -700,8 +700,9 @@ __global__ __launch_bounds__(512) void k_block_inverse(uint32_t D, const double*
   {
     const double2* src = reinterpret_cast<const double2*>(Top + (size_t)k * TOP_BLOCK);
     double2* dst = reinterpret_cast<double2*>(Tl);
+    static_assert(TOP_BLOCK % (2 * 512) == 0, "whole rounds of 512 threads");
 #pragma unroll
-    for (int i = threadIdx.x; i < (int)TOP_BLOCK / 2; i += 512) dst[i] = src[i];
+    for (int it = 0; it < (int)TOP_BLOCK / (2 * 512); ++it) dst[it * 512 + threadIdx.x] = src[it * 512 + threadIdx.x];
   }
   __syncthreads();
   const int rho = 16 * strip + (lane & 15);  // row of L^{-T}, column of L^{-1}
