@@ -478,8 +478,11 @@ struct Persist {
   static __device__ __attribute__((always_inline)) inline int row(int n, int lane) {
     return ((threadIdx.x >> 6) & 3) * 32 + n * 16 + (lane & 15);
   }
-  static __device__ __attribute__((always_inline)) inline int col(int m, int v, int lane) {
-    return (threadIdx.x >> 8) * 64 + m * 16 + (lane >> 4) + 4 * v;
+  // NM blocks of 16 columns per wavefront from column cb: NM = 4 covers the
+  // whole tile (cb = 0), NM = 2 one half of it (cb = 0 or 64).
+  template <int NM>
+  static __device__ __attribute__((always_inline)) inline int col(int m, int v, int lane, int cb) {
+    return cb + (threadIdx.x >> 8) * (16 * NM) + m * 16 + (lane >> 4) + 4 * v;
   }
 
   // Chunk k0 of panels A (column) and B (row) into LDS: row k of the chunk
@@ -501,14 +504,15 @@ struct Persist {
 
   // acc += A B^T over the staged chunk; the operands of the next four
   // columns are read from LDS before the products of the current four.
-  static __device__ __attribute__((always_inline)) inline void mma(double4_t (&acc)[4][2], const double* Ac) {
+  template <int NM>
+  static __device__ __attribute__((always_inline)) inline void mma(double4_t (&acc)[NM][2], const double* Ac, int cb) {
     const int lane = threadIdx.x & 63;
     const int wr = threadIdx.x >> 8, wc = (threadIdx.x >> 6) & 3;
-    const double* pa = Ac + (lane >> 4) * KS + wr * 64 + (lane & 15);
+    const double* pa = Ac + (lane >> 4) * KS + cb + wr * (16 * NM) + (lane & 15);
     const double* pb = Ac + PK * KS + (lane >> 4) * KS + wc * 32 + (lane & 15);
-    double a[2][4], b[2][2];
+    double a[2][NM], b[2][2];
 #pragma unroll
-    for (int m = 0; m < 4; ++m) a[0][m] = pa[m * 16];
+    for (int m = 0; m < NM; ++m) a[0][m] = pa[m * 16];
 #pragma unroll
     for (int n = 0; n < 2; ++n) b[0][n] = pb[n * 16];
 #pragma unroll
@@ -516,28 +520,28 @@ struct Persist {
       const int cur = s & 1;
       if (s + 1 < PK / 4) {
 #pragma unroll
-        for (int m = 0; m < 4; ++m) a[cur ^ 1][m] = pa[(s + 1) * 4 * KS + m * 16];
+        for (int m = 0; m < NM; ++m) a[cur ^ 1][m] = pa[(s + 1) * 4 * KS + m * 16];
 #pragma unroll
         for (int n = 0; n < 2; ++n) b[cur ^ 1][n] = pb[(s + 1) * 4 * KS + n * 16];
       }
 #pragma unroll
-      for (int m = 0; m < 4; ++m)
+      for (int m = 0; m < NM; ++m)
 #pragma unroll
         for (int n = 0; n < 2; ++n)
           acc[m][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[cur][m], b[cur][n], acc[m][n], 0, 0, 0);
     }
   }
 
-  // C group G (8 of the 32 values per lane: m = G) into the prefetch
+  // C group G (8 of the 8 NM values per lane: m = G) into the prefetch
   // registers.
-  template <int G>
-  static __device__ __attribute__((always_inline)) inline void c_group(double (&cr)[4][2][4], const double* C,
-                                                                      size_t ld) {
+  template <int NM, int G>
+  static __device__ __attribute__((always_inline)) inline void c_group(double (&cr)[NM][2][4], const double* C,
+                                                                      size_t ld, int cb) {
     const int lane = threadIdx.x & 63;
 #pragma unroll
     for (int v = 0; v < 4; ++v)
 #pragma unroll
-      for (int n = 0; n < 2; ++n) cr[G][n][v] = C[(size_t)col(G, v, lane) * ld + row(n, lane)];
+      for (int n = 0; n < 2; ++n) cr[G][n][v] = C[(size_t)col<NM>(G, v, lane, cb) * ld + row(n, lane)];
   }
 
   // One step: stage the next chunk (A = An, B = Bn; none if An is null),
@@ -548,20 +552,20 @@ struct Persist {
   // are complete at the barrier, so the next step may stage into it.
   // (s_barrier alone: the fence of __syncthreads() would wait for the C
   // group too.)
-  template <int G>
-  static __device__ __attribute__((always_inline)) inline void step(double4_t (&acc)[4][2], double (&cr)[4][2][4],
+  template <int NM, int G>
+  static __device__ __attribute__((always_inline)) inline void step(double4_t (&acc)[NM][2], double (&cr)[NM][2][4],
                                                                     const double* C, size_t ld, const double* An,
                                                                     const double* Bn, int k_next, double* smem,
-                                                                    int& buf) {
+                                                                    int& buf, int cb) {
     if (An) stage(An, Bn, ld, k_next, smem + (buf ^ 1) * BUF);
     if constexpr (G >= 0) {
       __asm__ volatile("" ::: "memory");
-      c_group<G>(cr, C, ld);
+      c_group<NM, G>(cr, C, ld, cb);
       __asm__ volatile("" ::: "memory");
     }
-    mma(acc, smem + buf * BUF);
+    mma<NM>(acc, smem + buf * BUF, cb);
     if constexpr (G >= 0) {
-      static_assert(G < 4, "four C groups of eight loads");
+      static_assert(G < NM, "NM C groups of eight loads");
       __asm__ volatile("s_waitcnt vmcnt(8) lgkmcnt(0)" ::: "memory");
     } else {
       __asm__ volatile("s_waitcnt vmcnt(0) lgkmcnt(0)" ::: "memory");
@@ -576,24 +580,68 @@ struct Persist {
   // of the rhs block keep their initial values, so a failed (NaN)
   // factorization cannot leak into storage the next assembly does not
   // rewrite (NaN * 0 is NaN).
-  static __device__ __attribute__((always_inline)) inline void store(const double4_t (&acc)[4][2], double* C,
+  template <int NM>
+  static __device__ __attribute__((always_inline)) inline void store(const double4_t (&acc)[NM][2], double* C,
                                                                      size_t ldc, bool lower_only, uint32_t row0,
-                                                                     uint32_t col0, uint32_t D, uint32_t Dp) {
+                                                                     uint32_t col0, uint32_t D, uint32_t Dp, int cb) {
     const int lane = threadIdx.x & 63;
     const bool full = !lower_only && row0 + NB <= D && col0 + NB <= D;
 #pragma unroll
-    for (int m = 0; m < 4; ++m)
+    for (int m = 0; m < NM; ++m)
 #pragma unroll
       for (int n = 0; n < 2; ++n)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-          const int r = row(n, lane), c = col(m, v, lane);
+          const int r = row(n, lane), c = col<NM>(m, v, lane, cb);
           const uint32_t g = row0 + r;
           if (full || ((!lower_only || r >= c) && (g < D || g == Dp) && col0 + c < D))
             C[(size_t)c * ldc + r] = acc[m][n][v];
         }
   }
 };
+
+// One tile of a trailing update (NM = 4) or one column half of it (NM = 2,
+// columns cb .. cb + 63): C - A_i A_j^T over the K / PK chunks, the first
+// of which is staged in buffer buf; the next work item's first chunk (An =
+// Ajn, Bn = Ain; none if Ajn is null) is staged during the last step.
+template <int NM>
+__device__ __attribute__((always_inline)) inline void persist_tile(double* C, size_t ld, const double* Aj,
+                                                                    const double* Ai, const double* Ajn,
+                                                                    const double* Ain, int nch, double* smem,
+                                                                    int& buf, bool lower_only, uint32_t row0,
+                                                                    uint32_t col0, uint32_t D, uint32_t Dp, int cb) {
+  constexpr int PK = Persist::PK;
+  double4_t acc[NM][2];
+#pragma unroll
+  for (int m = 0; m < NM; ++m)
+#pragma unroll
+    for (int n = 0; n < 2; ++n) acc[m][n] = double4_t{0.0, 0.0, 0.0, 0.0};
+  double cr[NM][2][4];
+  // C arrives in NM groups during the first NM steps (nch >= 4)
+  Persist::step<NM, 0>(acc, cr, C, ld, Aj, Ai, 1 * PK, smem, buf, cb);
+  Persist::step<NM, 1>(acc, cr, C, ld, Aj, Ai, 2 * PK, smem, buf, cb);
+  Persist::step<NM, (2 < NM ? 2 : -1)>(acc, cr, C, ld, Aj, Ai, 3 * PK, smem, buf, cb);
+  Persist::step<NM, (3 < NM ? 3 : -1)>(acc, cr, C, ld, 4 < nch ? Aj : Ajn, 4 < nch ? Ai : Ain,
+                                       4 < nch ? 4 * PK : 0, smem, buf, cb);
+#pragma unroll 1
+  for (int ch = 4; ch < nch; ++ch) {
+    const bool inner = ch + 1 < nch;
+    Persist::step<NM, -1>(acc, cr, C, ld, inner ? Aj : Ajn, inner ? Ai : Ain, inner ? (ch + 1) * PK : 0, smem, buf,
+                          cb);
+  }
+  // C - sum, all before the first store: the stores then issue back to
+  // back (a difference left next to its predicated store would wait for
+  // every earlier store too).
+#pragma unroll
+  for (int m = 0; m < NM; ++m)
+#pragma unroll
+    for (int n = 0; n < 2; ++n) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc[m][n][v] = cr[m][n][v] - acc[m][n][v];
+      __asm__ volatile("" : "+v"(acc[m][n]));
+    }
+  Persist::store<NM>(acc, C, ld, lower_only, row0, col0, D, Dp, cb);
+}
 
 // A_ij -= sum over the panels kp .. kp + K/NB - 1 of L_ip L_jp^T for
 // j0 <= j <= i, j < nblk, i <= nblk (i == nblk: the rhs block); ncol == 1
@@ -618,15 +666,36 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     diag_factor<8>(smem, S, ld, j0, D, Top, flags);
     return;
   }
-  // Workgroup b >= 1 takes tiles b, b + G - 1, ... (G = gridDim.x).
+  // Workgroup b >= 1 takes tiles b, b + G1, ... of tiles 1 .. ntiles - 1
+  // (G1 = gridDim.x - 1).  When the last round would leave more than half
+  // of the workgroups idle, its tiles go as column halves to two
+  // workgroups each (same products, same order: the same results).
   const int nrow = nblk - j0 + 1;
   const int ntiles = ncol * nrow - ncol * (ncol - 1) / 2;
-  const int stride = (int)gridDim.x - 1;
-  int t = (int)blockIdx.x;
-  if (t >= ntiles) return;
-  int r, c;
-  trail_tile(t, nrow, r, c);
-  int i = j0 + r, j = j0 + c;
+  const int G1 = (int)gridDim.x - 1, b = (int)blockIdx.x;
+  const int rounds = (ntiles - 1) / G1, rem = ntiles - 1 - rounds * G1;
+  const bool split = rem > 0 && 2 * rem <= G1;
+  const int nfull = rounds + (!split && b <= rem ? 1 : 0);
+  const int nitem = nfull + (split && b <= 2 * rem ? 1 : 0);
+  if (nitem == 0) return;
+  // work item q: block row and column of its tile, and the column half
+  // (-1: the whole tile)
+  auto item = [&](int q, int& ti, int& tj, int& half) {
+    int t;
+    if (q < nfull) {
+      t = b + q * G1;
+      half = -1;
+    } else {
+      t = rounds * G1 + 1 + (b - 1) / 2;
+      half = (b - 1) & 1;
+    }
+    int r, c;
+    trail_tile(t, nrow, r, c);
+    ti = j0 + r;
+    tj = j0 + c;
+  };
+  int i, j, half;
+  item(0, i, j, half);
   const double* Ai = P + (size_t)i * NB;
   const double* Aj = P + (size_t)j * NB;
   Persist::stage(Aj, Ai, ld, 0, smem);
@@ -634,51 +703,21 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
   __syncthreads();
   int buf = 0;
   const int nch = K / Persist::PK;
-  for (;;) {
-    const int tn = t + stride;
-    const bool more = tn < ntiles;
-    int in = 0, jn = 0;
-    if (more) {
-      trail_tile(tn, nrow, r, c);
-      in = j0 + r;
-      jn = j0 + c;
-    }
+  for (int q = 0; q < nitem; ++q) {
+    const bool more = q + 1 < nitem;
+    int in = 0, jn = 0, halfn = -1;
+    if (more) item(q + 1, in, jn, halfn);
     const double* Ain = P + (size_t)in * NB;
     const double* Ajn = more ? P + (size_t)jn * NB : nullptr;
     double* C = S + (size_t)j * NB * ld + (size_t)i * NB;
-    double4_t acc[4][2];
-#pragma unroll
-    for (int m = 0; m < 4; ++m)
-#pragma unroll
-      for (int n = 0; n < 2; ++n) acc[m][n] = double4_t{0.0, 0.0, 0.0, 0.0};
-    double cr[4][2][4];
-    // C arrives in four groups during the first four steps (nch >= 4)
-    Persist::step<0>(acc, cr, C, ld, Aj, Ai, 1 * Persist::PK, smem, buf);
-    Persist::step<1>(acc, cr, C, ld, Aj, Ai, 2 * Persist::PK, smem, buf);
-    Persist::step<2>(acc, cr, C, ld, Aj, Ai, 3 * Persist::PK, smem, buf);
-    Persist::step<3>(acc, cr, C, ld, 4 < nch ? Aj : Ajn, 4 < nch ? Ai : Ain, 4 < nch ? 4 * Persist::PK : 0, smem, buf);
-#pragma unroll 1
-    for (int ch = 4; ch < nch; ++ch) {
-      const bool inner = ch + 1 < nch;
-      Persist::step<-1>(acc, cr, C, ld, inner ? Aj : Ajn, inner ? Ai : Ain, inner ? (ch + 1) * Persist::PK : 0, smem,
-                        buf);
-    }
-    // C - sum, all before the first store: the stores then issue back to
-    // back (a difference left next to its predicated store would wait for
-    // every earlier store too).
-#pragma unroll
-    for (int m = 0; m < 4; ++m)
-#pragma unroll
-      for (int n = 0; n < 2; ++n) {
-#pragma unroll
-        for (int v = 0; v < 4; ++v) acc[m][n][v] = cr[m][n][v] - acc[m][n][v];
-        __asm__ volatile("" : "+v"(acc[m][n]));
-      }
-    Persist::store(acc, C, ld, i == j, (uint32_t)i * NB, (uint32_t)j * NB, D, Dp);
-    if (!more) break;
-    t = tn;
+    if (half < 0)
+      persist_tile<4>(C, ld, Aj, Ai, Ajn, Ain, nch, smem, buf, i == j, (uint32_t)i * NB, (uint32_t)j * NB, D, Dp, 0);
+    else
+      persist_tile<2>(C, ld, Aj, Ai, Ajn, Ain, nch, smem, buf, i == j, (uint32_t)i * NB, (uint32_t)j * NB, D, Dp,
+                      64 * half);
     i = in;
     j = jn;
+    half = halfn;
     Ai = Ain;
     Aj = Ajn;
   }
