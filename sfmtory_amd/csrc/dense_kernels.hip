@@ -23,14 +23,16 @@
 //                   eight-wavefront workgroup per CU walking its tiles,
 //                   panels staged through LDS by global_load_lds (double
 //                   buffered), C loaded while the products run; its first
-//                   workgroup updates and factors the next diagonal block.
+//                   workgroup updates and factors the next diagonal block;
+//                   a last round of tiles that would leave most workgroups
+//                   idle is split into column halves.
 // While the trailing matrix is large, block columns go in pairs: the update
 // of column block k+1 alone, its factorization and panel solve, then one
 // update of the rest with both panels (K = 2 NB), which halves the trailing
 // matrix's HBM traffic per FLOP.  Near the end, where the diagonal chain
 // bounds each step, single steps keep that chain short.
 // L_kk^{-1} (for the triangular solves) is formed for all blocks at once
-// after the factorization, off the critical path.
+// after the factorization, as panel solves of the identity.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
