@@ -697,22 +697,21 @@ __device__ inline void update_cam(const View<T>& v, uint32_t c, int cur) {
 }
 
 // ------------------------------------------------------------------- cost
-// Observation o's term: mode 0 w(|r|) |r|^2 (total_cost), mode 1 |r| (the
-// numerator of mean_reprojection_error); in_front is 1 for an observation
-// in front of its camera.
-__device__ inline void obs_cost(const uint32_t* obs_img, const uint32_t* obs_pt, const double2* obs_xy,
-                                const uint32_t* cam_of_img, const int32_t* cam_model,
-                                const double* poses, const double* cams, const double* pts,
-                                uint32_t o, int loss, double scale, int mode, double& term,
-                                double& in_front) {
+// The term of an observation of point pt in image img at xy: mode 0
+// w(|r|) |r|^2 (total_cost), mode 1 |r| (the numerator of
+// mean_reprojection_error); in_front is 1 for an observation in front of
+// its camera.
+__device__ inline void obs_cost_at(uint32_t img, uint32_t pt, double2 xy, const uint32_t* cam_of_img,
+                                   const int32_t* cam_model, const double* poses, const double* cams,
+                                   const double* pts, int loss, double scale, int mode, double& term,
+                                   double& in_front) {
   term = 0.0;
   in_front = 0.0;
-  const uint32_t img = obs_img[o], pt = obs_pt[o], cam = cam_of_img[img];
+  const uint32_t cam = cam_of_img[img];
   double c0, c1, c2;
   transform_point(poses + 7 * (size_t)img, pts[3 * (size_t)pt], pts[3 * (size_t)pt + 1],
                   pts[3 * (size_t)pt + 2], c0, c1, c2);
   if (c2 <= 1e-9) return;
-  const double2 xy = obs_xy[o];
   double r0, r1;
   project_residual(cam_model[cam], cams + 8 * (size_t)cam, c0, c1, c2, xy.x, xy.y, r0, r1);
   const double n = sqrt(r0 * r0 + r1 * r1);
@@ -723,6 +722,16 @@ __device__ inline void obs_cost(const uint32_t* obs_img, const uint32_t* obs_pt,
     term = n;
   }
   in_front = 1.0;
+}
+
+// Observation o's term (obs_cost_at).
+__device__ inline void obs_cost(const uint32_t* obs_img, const uint32_t* obs_pt, const double2* obs_xy,
+                                const uint32_t* cam_of_img, const int32_t* cam_model,
+                                const double* poses, const double* cams, const double* pts,
+                                uint32_t o, int loss, double scale, int mode, double& term,
+                                double& in_front) {
+  obs_cost_at(obs_img[o], obs_pt[o], obs_xy[o], cam_of_img, cam_model, poses, cams, pts, loss, scale, mode, term,
+              in_front);
 }
 
 }  // namespace sfmba

@@ -271,13 +271,25 @@ __global__ __launch_bounds__(COST_BLOCK) void k_cost(
   __shared__ double red1[COST_BLOCK];
   double s0 = 0.0, s1 = 0.0;
   const uint32_t base = blockIdx.x * COST_BLOCK * COST_ITEMS + threadIdx.x;
+  // every item's observation record first, so that the items' dependent
+  // loads (image -> camera, pose, point) do not each wait for it in turn
+  uint32_t img[COST_ITEMS], pt[COST_ITEMS];
+  double2 xy[COST_ITEMS];
+#pragma unroll
+  for (int it = 0; it < COST_ITEMS; ++it) {
+    const uint32_t o = base + it * COST_BLOCK;
+    if (o < n_obs) {
+      img[it] = obs_img[o];
+      pt[it] = obs_pt[o];
+      xy[it] = obs_xy[o];
+    }
+  }
 #pragma unroll
   for (int it = 0; it < COST_ITEMS; ++it) {
     const uint32_t o = base + it * COST_BLOCK;
     if (o < n_obs) {
       double term, front;
-      obs_cost(obs_img, obs_pt, obs_xy, cam_of_img, cam_model, poses, cams, pts, o, loss, scale, mode,
-               term, front);
+      obs_cost_at(img[it], pt[it], xy[it], cam_of_img, cam_model, poses, cams, pts, loss, scale, mode, term, front);
       s0 += term;
       s1 += front;
     }
