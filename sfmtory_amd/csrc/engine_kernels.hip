@@ -224,24 +224,32 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_backsub(View<T> v, int cur, const double* __restrict__ inc_part) {
   const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= v.n_pt) return;
+  // the point's own values are loaded before its flag is tested, so that
+  // the incidence sums wait on one load (the incidence range), not two
   const double* X = v.pts[cur] + 3 * (size_t)p;
   double* Y = v.pts[1 - cur] + 3 * (size_t)p;
-  if (!v.pt_ok[p]) {
-    Y[0] = X[0];
-    Y[1] = X[1];
-    Y[2] = X[2];
+  const double x0 = X[0], x1 = X[1], x2 = X[2];
+  const uint8_t ok = v.pt_ok[p];
+  const uint32_t a0 = v.pt_inc_ptr[p], a1 = v.pt_inc_ptr[p + 1];
+  double acc0 = v.pt_g[3 * (size_t)p], acc1 = v.pt_g[3 * (size_t)p + 1], acc2 = v.pt_g[3 * (size_t)p + 2];
+  const double* ci = v.pt_cinv + 9 * (size_t)p;
+  double c[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) c[k] = ci[k];
+  if (!ok) {
+    Y[0] = x0;
+    Y[1] = x1;
+    Y[2] = x2;
     return;
   }
-  double acc0 = v.pt_g[3 * (size_t)p], acc1 = v.pt_g[3 * (size_t)p + 1], acc2 = v.pt_g[3 * (size_t)p + 2];
-  for (uint32_t a = v.pt_inc_ptr[p]; a < v.pt_inc_ptr[p + 1]; ++a) {
+  for (uint32_t a = a0; a < a1; ++a) {
     acc0 += inc_part[3 * (size_t)a];
     acc1 += inc_part[3 * (size_t)a + 1];
     acc2 += inc_part[3 * (size_t)a + 2];
   }
-  const double* ci = v.pt_cinv + 9 * (size_t)p;
-  Y[0] = X[0] + -(ci[0] * acc0 + ci[1] * acc1 + ci[2] * acc2);
-  Y[1] = X[1] + -(ci[3] * acc0 + ci[4] * acc1 + ci[5] * acc2);
-  Y[2] = X[2] + -(ci[6] * acc0 + ci[7] * acc1 + ci[8] * acc2);
+  Y[0] = x0 + -(c[0] * acc0 + c[1] * acc1 + c[2] * acc2);
+  Y[1] = x1 + -(c[3] * acc0 + c[4] * acc1 + c[5] * acc2);
+  Y[2] = x2 + -(c[6] * acc0 + c[7] * acc1 + c[8] * acc2);
 }
 
 template <typename T>
