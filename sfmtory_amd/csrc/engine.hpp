@@ -143,7 +143,7 @@ struct Layout {
   uint32_t* multi_dest = nullptr; // destinations reduced from partials
   uint32_t* dest_part0 = nullptr; // first partial of each multi destination
   uint32_t* dest_nparts = nullptr;
-  // host copies needed by the driver
+  // host copies needed by the host-side LM loop
   std::vector<uint32_t> h_img_slot, h_cam_slot, h_cam_nfree, h_grp_off, h_grp_dim;
   std::vector<uint8_t> h_cam_free;
 };
