@@ -481,7 +481,8 @@ struct Persist {
     return ((threadIdx.x >> 6) & 3) * 32 + n * 16 + (lane & 15);
   }
   // NM blocks of 16 columns per wavefront from column cb: NM = 4 covers the
-  // whole tile (cb = 0), NM = 2 one half of it (cb = 0 or 64).
+  // whole tile (cb = 0), NM = 2 one half of it (cb = 0 or 64), NM = 1 one
+  // quarter (cb = 0, 32, 64 or 96).
   template <int NM>
   static __device__ __attribute__((always_inline)) inline int col(int m, int v, int lane, int cb) {
     return cb + (threadIdx.x >> 8) * (16 * NM) + m * 16 + (lane >> 4) + 4 * v;
@@ -602,8 +603,8 @@ struct Persist {
   }
 };
 
-// One tile of a trailing update (NM = 4) or one column half of it (NM = 2,
-// columns cb .. cb + 63): C - A_i A_j^T over the K / PK chunks, the first
+// One tile of a trailing update (NM = 4) or one column part of it (NM = 2:
+// a half, columns cb .. cb + 63; NM = 1: a quarter, cb .. cb + 31): C - A_i A_j^T over the K / PK chunks, the first
 // of which is staged in buffer buf; the next work item's first chunk (An =
 // Ajn, Bn = Ain; none if Ajn is null) is staged during the last step.
 template <int NM>
@@ -621,7 +622,7 @@ __device__ __attribute__((always_inline)) inline void persist_tile(double* C, si
   double cr[NM][2][4];
   // C arrives in NM groups during the first NM steps (nch >= 4)
   Persist::step<NM, 0>(acc, cr, C, ld, Aj, Ai, 1 * PK, smem, buf, cb);
-  Persist::step<NM, 1>(acc, cr, C, ld, Aj, Ai, 2 * PK, smem, buf, cb);
+  Persist::step<NM, (1 < NM ? 1 : -1)>(acc, cr, C, ld, Aj, Ai, 2 * PK, smem, buf, cb);
   Persist::step<NM, (2 < NM ? 2 : -1)>(acc, cr, C, ld, Aj, Ai, 3 * PK, smem, buf, cb);
   Persist::step<NM, (3 < NM ? 3 : -1)>(acc, cr, C, ld, 4 < nch ? Aj : Ajn, 4 < nch ? Ai : Ain,
                                        4 < nch ? 4 * PK : 0, smem, buf, cb);
@@ -670,17 +671,18 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
   }
   // Workgroup b >= 1 takes tiles b, b + G1, ... of tiles 1 .. ntiles - 1
   // (G1 = gridDim.x - 1).  When the last round would leave more than half
-  // of the workgroups idle, its tiles go as column halves to two
-  // workgroups each (same products, same order: the same results).
+  // (three quarters) of the workgroups idle, its tiles go as column halves
+  // (quarters) to two (four) workgroups each (same products, same order:
+  // the same results).
   const int nrow = nblk - j0 + 1;
   const int ntiles = ncol * nrow - ncol * (ncol - 1) / 2;
   const int G1 = (int)gridDim.x - 1, b = (int)blockIdx.x;
   const int rounds = (ntiles - 1) / G1, rem = ntiles - 1 - rounds * G1;
-  const bool split = rem > 0 && 2 * rem <= G1;
-  const int nfull = rounds + (!split && b <= rem ? 1 : 0);
-  const int nitem = nfull + (split && b <= 2 * rem ? 1 : 0);
+  const int parts = rem == 0 ? 1 : 4 * rem <= G1 ? 4 : 2 * rem <= G1 ? 2 : 1;
+  const int nfull = rounds + (parts == 1 && b <= rem ? 1 : 0);
+  const int nitem = nfull + (parts > 1 && b <= parts * rem ? 1 : 0);
   if (nitem == 0) return;
-  // work item q: block row and column of its tile, and the column half
+  // work item q: block row and column of its tile, and the column part
   // (-1: the whole tile)
   auto item = [&](int q, int& ti, int& tj, int& half) {
     int t;
@@ -688,8 +690,8 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
       t = b + q * G1;
       half = -1;
     } else {
-      t = rounds * G1 + 1 + (b - 1) / 2;
-      half = (b - 1) & 1;
+      t = rounds * G1 + 1 + (b - 1) / parts;
+      half = (b - 1) % parts;
     }
     int r, c;
     trail_tile(t, nrow, r, c);
@@ -714,9 +716,12 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     double* C = S + (size_t)j * NB * ld + (size_t)i * NB;
     if (half < 0)
       persist_tile<4>(C, ld, Aj, Ai, Ajn, Ain, nch, smem, buf, i == j, (uint32_t)i * NB, (uint32_t)j * NB, D, Dp, 0);
-    else
+    else if (parts == 2)
       persist_tile<2>(C, ld, Aj, Ai, Ajn, Ain, nch, smem, buf, i == j, (uint32_t)i * NB, (uint32_t)j * NB, D, Dp,
                       64 * half);
+    else
+      persist_tile<1>(C, ld, Aj, Ai, Ajn, Ain, nch, smem, buf, i == j, (uint32_t)i * NB, (uint32_t)j * NB, D, Dp,
+                      32 * half);
     i = in;
     j = jn;
     half = halfn;
