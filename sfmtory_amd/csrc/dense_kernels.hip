@@ -398,12 +398,11 @@ __global__ __launch_bounds__(256) void k_potrf_block(double* __restrict__ S, siz
 // for X_p^T is exactly the B-operand layout X_p^T needs in the later
 // sub-panels (lane l, register v: X_p[l%16][l/16 + 4v]).  The L_pq and
 // X11_p operands come from Top in lane order (one coalesced 512-byte load
-// per operand): T is block k's Top plus the lane.  Lane l's row of the
-// strip is row l % 16: load(c) gives its A entry in column c, store(c, x)
-// takes its X entry.
-template <typename Load, typename Store>
-__device__ __attribute__((always_inline)) inline void panel_strip(Load load, const double* __restrict__ T,
-                                                                   Store store) {
+// per operand): op(j) is this lane's value of operand j = 4 block + s of
+// block k's Top.  Lane l's row of the strip is row l % 16: load(c) gives
+// its A entry in column c, store(c, x) takes its X entry.
+template <typename Load, typename Op, typename Store>
+__device__ __attribute__((always_inline)) inline void panel_strip(Load load, Op op, Store store) {
   const int lk = (threadIdx.x & 63) >> 4;
   double xb[8][4];
 #pragma unroll
@@ -415,12 +414,11 @@ __device__ __attribute__((always_inline)) inline void panel_strip(Load load, con
     for (int q = 0; q < p; ++q)
 #pragma unroll
       for (int s = 0; s < 4; ++s)
-        r = __builtin_amdgcn_mfma_f64_16x16x4f64(T[((p * (p - 1) / 2 + q) * 4 + s) * 64], xb[q][s], r,
-                                                  0, 0, 0);
+        r = __builtin_amdgcn_mfma_f64_16x16x4f64(op((p * (p - 1) / 2 + q) * 4 + s), xb[q][s], r, 0, 0, 0);
     double4_t x = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int s = 0; s < 4; ++s)
-      x = __builtin_amdgcn_mfma_f64_16x16x4f64(T[((28 + p) * 4 + s) * 64], r[s], x, 0, 0, 0);
+      x = __builtin_amdgcn_mfma_f64_16x16x4f64(op((28 + p) * 4 + s), r[s], x, 0, 0, 0);
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       xb[p][v] = x[v];
@@ -441,7 +439,13 @@ __global__ __launch_bounds__(64) void k_trsm_panel(double* __restrict__ S, size_
   const bool row_ok = grow < D || grow == Dp;
   double* Acol = S + (size_t)k * NB * ld + grow;
   const uint32_t col0 = (uint32_t)k * NB;
-  panel_strip([&](int c) { return Acol[(size_t)c * ld]; }, Top + (size_t)k * TOP_BLOCK + lane,
+  // every operand is loaded up front: the chain then waits on each in turn
+  // as it arrives instead of on eight rounds of loads issued as it goes
+  const double* T = Top + (size_t)k * TOP_BLOCK + lane;
+  double t[TOP_BLOCK / 64];
+#pragma unroll
+  for (int j = 0; j < (int)(TOP_BLOCK / 64); ++j) t[j] = T[j * 64];
+  panel_strip([&](int c) { return Acol[(size_t)c * ld]; }, [&](int j) { return t[j]; },
               [&](int c, double x) {
                 if (row_ok && col0 + (uint32_t)c < D) Acol[(size_t)c * ld] = x;
               });
@@ -754,7 +758,8 @@ __global__ __launch_bounds__(512) void k_block_inverse(uint32_t D, const double*
   const int rho = 16 * strip + (lane & 15);  // row of L^{-T}, column of L^{-1}
   const bool pad_rho = (uint32_t)k * NB + rho >= D;
   double* X = Linv + (size_t)k * LINV_BLOCK;
-  panel_strip([&](int c) { return c == rho ? 1.0 : 0.0; }, Tl + lane,
+  const double* T = Tl + lane;
+  panel_strip([&](int c) { return c == rho ? 1.0 : 0.0; }, [&](int j) { return T[j * 64]; },
               [&](int c, double x) {
                 if (pad_rho || (uint32_t)k * NB + c >= D) x = c == rho ? 1.0 : 0.0;
                 X[(size_t)rho * NB + c] = x;                    // L^{-1}[c][rho]
